@@ -1,0 +1,103 @@
+"""ctypes binding of libeuvl.so (include/euvl.h).
+
+The library is required: importing the GPU entry points without it raises
+immediately; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeuvl.so")
+
+c_int, c_int64, c_double, c_char, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_char, ctypes.c_void_p
+P_int = ctypes.POINTER(ctypes.c_int)
+P_double = ctypes.POINTER(ctypes.c_double)
+
+
+class EuvlError(RuntimeError):
+    pass
+
+
+class EmParams(ctypes.Structure):
+    _fields_ = [
+        ("ndivx", c_int), ("ndivy", c_int), ("mx", c_int), ("my", c_int),
+        ("na", c_double), ("wavelength", c_double), ("k", c_double),
+        ("dx", c_double), ("dy", c_double),
+        ("lsmax_x", c_int), ("lsmax_y", c_int),
+        ("eabs_re", c_double), ("eabs_im", c_double), ("dabs", c_double),
+        ("n_layers", c_int), ("h_layer_eps", P_double), ("h_layer_d", P_double),
+    ]
+
+
+# name -> (argtypes); every entry point returns int
+_SIGS = {
+    "euvl_create": [ctypes.POINTER(c_void_p)],
+    "euvl_destroy": [c_void_p],
+    "euvl_set_stream": [c_void_p, c_void_p],
+    "euvl_workspace_bytes": [c_void_p],
+    "euvl_gemm_stats": [c_void_p, c_int, P_double],
+    "euvl_zgemm_batched": [c_void_p, c_int, c_int, c_int, P_double, c_void_p, c_int64, c_int64, c_void_p, c_int64,
+                           c_int64, P_double, c_void_p, c_int64, c_int64, P_double, c_int],
+    "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
+    "euvl_zgetrs_batched": [c_void_p, c_int, c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
+                            c_int64, c_int],
+    "euvl_zsqrtm_batched": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, P_int],
+    "euvl_zexpm_pair_batched": [c_void_p, c_int, c_void_p, c_double, P_double, c_void_p, c_void_p, c_int, P_int],
+    "euvl_mask_spectrum": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    "euvl_multilayer": [c_void_p, c_char, ctypes.POINTER(EmParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p],
+    "euvl_amp_s": [c_void_p, c_char, ctypes.POINTER(EmParams), P_int, P_int, c_int, c_int, c_int, c_void_p, c_int,
+                   P_double, P_double, c_void_p, P_int],
+    "euvl_abbe_image": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_int, P_int, c_int, P_double,
+                        c_int, P_int, c_int, P_int, c_double, c_double, c_double, c_double, c_int, c_int, c_double,
+                        c_void_p],
+    "euvl_m3d_fit": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
+                     P_int, P_double, P_double, c_double, c_double, c_double, c_int, P_int, c_void_p, c_void_p,
+                     c_void_p],
+}
+
+EXPORTS = tuple(_SIGS) + ("euvl_last_error",)
+
+_lib = None
+
+
+def load():
+    """Load libeuvl.so (raises EuvlError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EuvlError(f"{LIB_PATH} not found: build it with `python -m euvlitho_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int64 if name == "euvl_workspace_bytes" else c_int
+    lib.euvl_last_error.argtypes = []
+    lib.euvl_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def check(status, what=""):
+    if status != 0:
+        msg = load().euvl_last_error().decode(errors="replace")
+        raise EuvlError(f"{what} failed with status {status}: {msg}")
+
+
+def ptr(t):
+    """Raw device pointer of a torch tensor (must be contiguous)."""
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return c_void_p(t.data_ptr())
+
+
+def arr_i(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(P_int)
+
+
+def arr_d(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(P_double)

@@ -1,0 +1,154 @@
+// Element-wise and reduction kernels on batches of n x n complex matrices
+// (contiguous, stride n*n per batch entry) used by the EM solve.
+#include "common.h"
+#include "ew.h"
+
+namespace {
+
+__device__ inline int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// out = a1*X1 + a2*X2 + a3*X3 + g*I  (X2/X3 may be null)
+__global__ void lincomb_kernel(int n, int64_t total, double2* out, double2 a1, const double2* X1, double2 a2,
+                               const double2* X2, double2 a3, const double2* X3, double2 g) {
+  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = cmul(a1, X1[e]);
+    if (X2) v = cadd(v, cmul(a2, X2[e]));
+    if (X3) v = cadd(v, cmul(a3, X3[e]));
+    const int64_t r = e % ((int64_t)n * n);
+    if (r / n == r % n) v = cadd(v, g);
+    out[e] = v;
+  }
+}
+
+// out[b] = X[b] * s[b] (per-batch complex scalar from device memory) + g*I
+__global__ void bscale_kernel(int n, int64_t per, int batch, double2* out, const double2* X, const double2* s,
+                              double2 g) {
+  const int64_t total = per * batch;
+  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    double2 v = cmul(X[e], s[b]);
+    if (r / n == r % n) v = cadd(v, g);
+    out[e] = v;
+  }
+}
+
+// X[b](i,j) *= col[b*n + j] (column scaling) and/or row[b*n + i]
+__global__ void diagscale_kernel(int n, int64_t total, double2* out, const double2* X, const double2* row,
+                                 const double2* col) {
+  const int64_t per = (int64_t)n * n;
+  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / per, r = e - b * per;
+    const int i = (int)(r / n), j = (int)(r % n);
+    double2 v = X[e];
+    if (row) v = cmul(row[b * n + i], v);
+    if (col) v = cmul(v, col[b * n + j]);
+    out[e] = v;
+  }
+}
+
+// per-batch reductions into red[b]: 0 = max |X - I| (elementwise max of |re|+|im|),
+// 1 = 1-norm (max column sum of |x|), 2 = mean of Re diag
+template <int KIND>
+__global__ __launch_bounds__(256) void reduce_kernel(int n, const double2* X, double* red) {
+  const int b = blockIdx.x;
+  X += (int64_t)b * n * n;
+  __shared__ double sh[256];
+  double acc = 0.0;
+  if (KIND == 0) {
+    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
+      double2 v = X[e];
+      if (e / n == e % n) v.x -= 1.0;
+      acc = fmax(acc, fabs(v.x) + fabs(v.y));
+    }
+  } else if (KIND == 1) {
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double s = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double2 v = X[(int64_t)i * n + j];
+        s += hypot(v.x, v.y);
+      }
+      acc = fmax(acc, s);
+    }
+  } else {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += X[(int64_t)i * n + i].x;
+  }
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      if (KIND == 2)
+        sh[threadIdx.x] += sh[threadIdx.x + s];
+      else
+        sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) red[b] = (KIND == 2) ? sh[0] / n : sh[0];
+}
+
+__global__ void copy_kernel(int64_t total, double2* out, const double2* X) {
+  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) out[e] = X[e];
+}
+
+inline dim3 ew_grid(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+}  // namespace
+
+namespace euvl {
+
+int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
+               const double2* X2, double2 a3, const double2* X3, double2 g) {
+  const int64_t total = (int64_t)n * n * batch;
+  hipLaunchKernelGGL(lincomb_kernel, ew_grid(total), dim3(256), 0, h->stream, n, total, out, a1, X1, a2, X2, a3,
+                     X3, g);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* d_s, double2 g) {
+  const int64_t per = (int64_t)n * n;
+  hipLaunchKernelGGL(bscale_kernel, ew_grid(per * batch), dim3(256), 0, h->stream, n, per, batch, out, X, d_s, g);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
+                 const double2* col) {
+  const int64_t total = (int64_t)n * n * batch;
+  hipLaunchKernelGGL(diagscale_kernel, ew_grid(total), dim3(256), 0, h->stream, n, total, out, X, row, col);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X) {
+  hipLaunchKernelGGL(copy_kernel, ew_grid(count), dim3(256), 0, h->stream, count, out, X);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red) {
+  if (kind == 0)
+    hipLaunchKernelGGL(reduce_kernel<0>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
+  else if (kind == 1)
+    hipLaunchKernelGGL(reduce_kernel<1>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
+  else
+    hipLaunchKernelGGL(reduce_kernel<2>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out) {
+  EUVL_TRY(reduce(h, kind, n, batch, X, h->d_red));
+  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * batch, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int b = 0; b < batch; ++b) out[b] = h->h_pinned[b];
+  return EUVL_OK;
+}
+
+}  // namespace euvl

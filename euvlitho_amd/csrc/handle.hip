@@ -1,0 +1,82 @@
+// Handle lifecycle, error reporting and the device workspace of libeuvl.
+#include <cstdarg>
+
+#include "common.h"
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+namespace euvl {
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int ws_reserve(euvl_handle h, size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  h->ws_top = 0;
+  if (bytes <= h->ws_cap) return EUVL_OK;
+  if (h->ws) {
+    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+    EUVL_HIP_CHECK(hipFree(h->ws));
+    h->ws = nullptr;
+    h->ws_cap = 0;
+  }
+  if (hipMalloc(&h->ws, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    set_error("workspace allocation of %zu bytes failed", bytes);
+    return EUVL_ERR_NOMEM;
+  }
+  h->ws_cap = bytes;
+  return EUVL_OK;
+}
+
+void* ws_take(euvl_handle h, size_t bytes) {
+  bytes = (bytes + 255) & ~size_t(255);
+  if (h->ws_top + bytes > h->ws_cap) return nullptr;
+  void* p = h->ws + h->ws_top;
+  h->ws_top += bytes;
+  return p;
+}
+
+}  // namespace euvl
+
+extern "C" const char* euvl_last_error(void) { return g_err; }
+
+extern "C" int euvl_create(euvl_handle* out) {
+  if (!out) return EUVL_ERR_ARG;
+  euvl_handle h = new euvl_handle_s();
+  hipError_t e = hipGetDevice(&h->device);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&h->h_pinned, 4096, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void**)&h->d_red, 4096);
+  if (e != hipSuccess) {
+    euvl::set_error("euvl_create: %s", hipGetErrorString(e));
+    delete h;
+    return EUVL_ERR_HIP;
+  }
+  *out = h;
+  return EUVL_OK;
+}
+
+extern "C" int euvl_destroy(euvl_handle h) {
+  if (!h) return EUVL_OK;
+  (void)hipStreamSynchronize(h->stream);
+  for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  if (h->ws) (void)hipFree(h->ws);
+  if (h->d_red) (void)hipFree(h->d_red);
+  if (h->h_pinned) (void)hipHostFree(h->h_pinned);
+  delete h;
+  return EUVL_OK;
+}
+
+extern "C" int euvl_set_stream(euvl_handle h, void* s) {
+  if (!h) return EUVL_ERR_ARG;
+  h->stream = (hipStream_t)s;
+  return EUVL_OK;
+}
+
+extern "C" int64_t euvl_workspace_bytes(euvl_handle h) { return h ? (int64_t)h->ws_cap : 0; }

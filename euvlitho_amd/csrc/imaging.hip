@@ -1,0 +1,372 @@
+// Image formation from EM amplitudes: the rigorous Abbe accumulation of
+// cnn/intensity/abbe/intensity.cpp:276-408 and the M3D parameter fit of
+// cnn/intensity/m3d/m3d.cpp:329-467.
+//
+// Abbe: the pupil field of every source point is scattered into a small dense
+// spectrum grid (orders |l| <= lpx, |m| <= lpy), and the unnormalised inverse
+// DFT onto the nx x ny image is evaluated separably (y first, then x) with
+// |Ex|^2 + |Ey|^2 + |Ez|^2 summed over source points inside the x pass.
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <vector>
+
+#include "ew.h"
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+struct AbbeSetup {
+  double k, dx, dy, na, kx0, ky0, z0, z;
+  int mx, my, lsmx, lsmy, nsx, nsy, lmax, mmax, nrange;
+  int lpx, lpy;  // spectrum half-widths
+};
+
+// rotate() of intensity.cpp:423-465 (== electro_field.polarization_rotation):
+// first column R(:,0) only is needed for X polarisation (Ay = 0).
+__device__ void rotate_col0(double k, int MX, int MY, double px, double py, double sx0, double sy0, double R[3]) {
+  double s0[3] = {sx0, sy0, -sqrt(k * k - sx0 * sx0 - sy0 * sy0)};
+  double p[3] = {px, py, 0.0};
+  double pp[3] = {MX * px, MY * py, -sqrt(k * k - MX * px * MX * px - MY * py * MY * py)};
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double es[3], esp[3], em[3], emp[3];
+  auto cross = [](const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto norm = [](const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
+  if ((px * px + py * py) > 0.00001) {
+    cross(p, s0, es);
+    double n = norm(es);
+    for (int i = 0; i < 3; ++i) es[i] /= n;
+    cross(pp, ez, esp);
+    n = norm(esp);
+    for (int i = 0; i < 3; ++i) esp[i] /= n;
+    double ps0[3] = {p[0] + s0[0], p[1] + s0[1], p[2] + s0[2]};
+    n = norm(ps0);
+    for (int i = 0; i < 3; ++i) ps0[i] /= n;
+    cross(es, ps0, em);
+    cross(esp, pp, emp);
+    for (int i = 0; i < 3; ++i) emp[i] /= k;
+  } else {
+    cross(ez, s0, es);
+    double n = norm(es);
+    for (int i = 0; i < 3; ++i) es[i] /= n;
+    for (int i = 0; i < 3; ++i) esp[i] = -es[i];
+    cross(es, s0, em);
+    for (int i = 0; i < 3; ++i) em[i] /= k;
+    const double mez[3] = {0.0, 0.0, -1.0};
+    cross(esp, mez, emp);
+  }
+  for (int i = 0; i < 3; ++i) R[i] = esp[i] * es[0] + emp[i] * em[0];
+}
+
+// Pupil fields: F[s][c][l+lpx][m+lpy] (c = x, y, z) for every source point s and pupil point i.
+__global__ void abbe_field_kernel(AbbeSetup a, const double2* Ax, const int* modemap, const double* off,
+                                  const int* src, int n_src, const int* pupil, int ncut, double2* F) {
+  const int64_t total = (int64_t)n_src * ncut;
+  const int nl = 2 * a.lpx + 1, nm = 2 * a.lpy + 1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / ncut), i = (int)(e % ncut);
+    const int o = src[3 * s], l0 = src[3 * s + 1], m0 = src[3 * s + 2];
+    const int lin = pupil[2 * i], min_ = pupil[2 * i + 1];
+    const double sx0 = off[4 * o], fx = off[4 * o + 2], fy = off[4 * o + 3];
+    const double kxn = 2.0 * PI / a.dx * fx + 2.0 * PI / a.dx * l0 + 2.0 * PI * lin / a.dx;
+    const double kyn = 2.0 * PI / a.dy * fy + 2.0 * PI / a.dy * m0 + 2.0 * PI * min_ / a.dy;
+    const double p2 = a.mx * a.mx * kxn * kxn + a.my * a.my * kyn * kyn;
+    if (!(p2 <= (a.na * a.k) * (a.na * a.k))) continue;  // outside the pupil: stays zero
+    // amplitude of mode (l0 + lin, m0 + min) for the incidence (l0, m0); -1000 when missing (intensity.cpp:255)
+    double2 amp = make_double2(-1000.0, 0.0);
+    const int ls = l0 + a.lsmx, ms = m0 + a.lsmy;
+    const int l = l0 + lin, m = m0 + min_;
+    if (ls >= 0 && ls < a.nsx && ms >= 0 && ms < a.nsy && abs(l) <= a.lmax && abs(m) <= a.mmax) {
+      const int n = modemap[(l + a.lmax) * (2 * a.mmax + 1) + (m + a.mmax)];
+      if (n >= 0) amp = Ax[(((int64_t)o * a.nsx + ls) * a.nsy + ms) * a.nrange + n];
+    }
+    const double kx = sx0 + 2.0 * PI / a.dx * l0;
+    const double2 A = cscale(amp, 1.0 / sqrt(a.k * a.k - kx * kx));
+    double R[3];
+    rotate_col0(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, R);
+    const double ph = ((kxn + a.kx0) * (kxn + a.kx0) + (kyn + a.ky0) * (kyn + a.ky0)) / 2.0 / a.k * a.z0 +
+                      p2 / 2.0 / a.k * a.z;
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+    // E_c = i k R(c,0) A, times the phase
+    const double2 ikA = make_double2(-a.k * A.y, a.k * A.x);
+    const double2 t = cmul(ikA, make_double2(cp, sp));
+    const int64_t base = (int64_t)s * 3 * nl * nm + (int64_t)(lin + a.lpx) * nm + (min_ + a.lpy);
+    for (int c = 0; c < 3; ++c) F[base + (int64_t)c * nl * nm] = cscale(t, R[c]);
+  }
+}
+
+// G[s][c][l][y] = sum_m F[s][c][l][m] exp(+2 pi i (m mod ny) y / ny)
+__global__ void abbe_ypass_kernel(int n_sc, int nl, int nm, int lpy, int ny, const double2* F, const double2* twy,
+                                  double2* G) {
+  const int64_t total = (int64_t)n_sc * nl * ny;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(e % ny);
+    const int64_t sl = e / ny;  // (s*3+c)*nl + l
+    const double2* f = F + sl * nm;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int mm = 0; mm < nm; ++mm) {
+      const int m = mm - lpy;
+      const int idx = (int)((((int64_t)m * y) % ny + ny) % ny);
+      acc = cadd(acc, cmul(f[mm], twy[idx]));
+    }
+    G[e] = acc;
+  }
+}
+
+// img[x][y] += w * sum_{s,c} | sum_l exp(2 pi i (l x mod nx)/nx) G[s][c][l][y] |^2
+__global__ __launch_bounds__(256) void abbe_xpass_kernel(int n_sc, int nl, int lpx, int nx, int ny, const double2* G,
+                                                         const double2* twx, double w, double* img) {
+  const int y = blockIdx.x * blockDim.x + threadIdx.x;
+  const int x = blockIdx.y;
+  if (y >= ny) return;
+  extern __shared__ double2 sh_tw[];  // nl twiddles for this x
+  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
+    const int ll = l - lpx;
+    const int idx = (int)((((int64_t)ll * x) % nx + nx) % nx);
+    sh_tw[l] = twx[idx];
+  }
+  __syncthreads();
+  double acc = 0.0;
+  for (int sc = 0; sc < n_sc; ++sc) {
+    const double2* g = G + (int64_t)sc * nl * ny + y;
+    double2 f = make_double2(0.0, 0.0);
+    for (int l = 0; l < nl; ++l) f = cadd(f, cmul(sh_tw[l], g[(int64_t)l * ny]));
+    acc += f.x * f.x + f.y * f.y;
+  }
+  img[(int64_t)x * ny + y] += w * acc;
+}
+
+// M3D fit of one clip, one thread per pupil point (m3d.cpp:329-467).
+struct M3dSetup {
+  double k, dx, dy, na, lambda, kx0, ky0, z0;
+  int mx, my, lsmx, lsmy, nsx, nsy, lpmx, lpmy, lmax, mmax, nrange;
+};
+
+__device__ inline bool in_na_d(const M3dSetup& a, int l, int m) {
+  const double u = l * a.mx / a.dx, v = m * a.my / a.dy, c = a.na / a.lambda;
+  return u * u + v * v <= c * c;
+}
+
+__global__ void m3d_fit_kernel(M3dSetup a, const double2* fmask, const double2* Ax, const int* modemap,
+                               const double2* vc, const double2* ab, const int* pupil, int ncut, double2* a0,
+                               double2* ax, double2* ay) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= ncut) return;
+  const int lin = pupil[3 * n], min_ = pupil[3 * n + 1], xin = pupil[3 * n + 2];
+  const int noy = 2 * a.lpmy + 1;
+  const int cx = a.lsmx, cy = a.lsmy;
+  // centre thin-mask amplitude (m3d.cpp:329-356), divided by its phase (m3d.cpp:385-393)
+  const double kxp = 2.0 * PI * lin / a.dx, kyp = 2.0 * PI * min_ / a.dy;
+  const double ph = -(a.kx0 * kxp + kxp * kxp / 2.0 + a.ky0 * kyp + kyp * kyp / 2.0) / a.k * a.z0;
+  double sp, cp;
+  sincos(ph, &sp, &cp);
+  const double2 vc0 = vc[cx * a.nsy + cy], ab0 = ab[cx * a.nsy + cy];
+  double2 fc = cmul(cmul(fmask[(lin + a.lpmx) * noy + (min_ + a.lpmy)], make_double2(cp, sp)), csub(vc0, ab0));
+  if (lin == 0 && min_ == 0) fc = cadd(fc, ab0);
+  const double2 ph0 = cscale(vc0, 1.0 / hypot(vc0.x, vc0.y));
+  fc = cdiv(fc, ph0);
+  const double lp = lin, mp = min_;
+  double2 f1 = make_double2(0.0, 0.0), fx = f1, fy = f1;
+  double c1 = 0, sx = 0, sy = 0, sx2 = 0, sy2 = 0, sxy = 0;
+  for (int is = 0; is < a.nsx; ++is)
+    for (int js = 0; js < a.nsy; ++js) {
+      const int ls_ = is - a.lsmx, ms_ = js - a.lsmy;
+      if (!in_na_d(a, ls_, ms_) || !in_na_d(a, lin + ls_, min_ + ms_)) continue;
+      const int l = lin + ls_, m = min_ + ms_;
+      double2 am = make_double2(0.0, 0.0);
+      if (abs(l) <= a.lmax && abs(m) <= a.mmax) {
+        const int md = modemap[(l + a.lmax) * (2 * a.mmax + 1) + (m + a.mmax)];
+        if (md >= 0) am = Ax[((int64_t)is * a.nsy + js) * a.nrange + md];
+      }
+      const double2 v = vc[is * a.nsy + js];
+      const double2 phs = cscale(v, 1.0 / hypot(v.x, v.y));
+      const double2 f = csub(cdiv(am, phs), fc);
+      const double ls = ls_ + lp / 2.0, ms = ms_ + mp / 2.0;
+      f1 = cadd(f1, f);
+      fx = cadd(fx, cscale(f, ls));
+      fy = cadd(fy, cscale(f, ms));
+      c1 += 1;
+      sx += ls;
+      sy += ms;
+      sx2 += ls * ls;
+      sy2 += ms * ms;
+      sxy += ls * ms;
+    }
+  if (xin) {
+    const double cd = c1 * sx2 * sy2 + 2.0 * sx * sxy * sy - c1 * sxy * sxy - sx2 * sy * sy - sy2 * sx * sx;
+    const double i = 1.0 / cd;
+    a0[n] = cscale(cadd(cadd(cscale(f1, sx2 * sy2 - sxy * sxy), cscale(fx, -sx * sy2 + sy * sxy)),
+                        cscale(fy, sx * sxy - sy * sx2)), i);
+    ax[n] = cscale(cadd(cadd(cscale(f1, -sx * sy2 + sy * sxy), cscale(fx, c1 * sy2 - sy * sy)),
+                        cscale(fy, -c1 * sxy + sx * sy)), i);
+    ay[n] = cscale(cadd(cadd(cscale(f1, sx * sxy - sy * sx2), cscale(fx, -c1 * sxy + sx * sy)),
+                        cscale(fy, c1 * sx2 - sx * sx)), i);
+  } else {
+    a0[n] = cscale(f1, 1.0 / c1);
+    ax[n] = make_double2(0.0, 0.0);
+    ay[n] = make_double2(0.0, 0.0);
+  }
+}
+
+inline dim3 grid1(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+struct Upload {
+  std::vector<char> host;
+  size_t add(const void* p, size_t bytes) {
+    size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes);
+    if (bytes) memcpy(host.data() + off, p, bytes);
+    return off;
+  }
+};
+
+}  // namespace
+
+namespace euvl {
+int fill(euvl_handle h, int64_t count, double2* out, double2 v);
+}
+
+extern "C" int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const double* d_Ax, int nrange, int lmax,
+                               int mmax, const int* h_modemap, int n_off, const double* h_off, int n_src,
+                               const int* h_src, int ncut, const int* h_pupil, double kx0, double ky0, double z0,
+                               double z, int nx, int ny, double weight, double* d_img) {
+  using namespace euvl;
+  if (!h || !p || !d_Ax || !h_modemap || n_off <= 0 || !h_off || n_src < 0 || ncut <= 0 || !h_pupil || nx <= 0 ||
+      ny <= 0 || !d_img) {
+    set_error("euvl_abbe_image: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  if (n_src == 0) return EUVL_OK;
+  for (int s = 0; s < n_src; ++s)
+    if (h_src[3 * s] < 0 || h_src[3 * s] >= n_off) {
+      set_error("euvl_abbe_image: source %d names offset %d of %d", s, h_src[3 * s], n_off);
+      return EUVL_ERR_ARG;
+    }
+  int lpx = 0, lpy = 0;
+  for (int i = 0; i < ncut; ++i) {
+    lpx = std::max(lpx, std::abs(h_pupil[2 * i]));
+    lpy = std::max(lpy, std::abs(h_pupil[2 * i + 1]));
+  }
+  const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
+  AbbeSetup a;
+  a.k = p->k;
+  a.dx = p->dx;
+  a.dy = p->dy;
+  a.na = p->na;
+  a.kx0 = kx0;
+  a.ky0 = ky0;
+  a.z0 = z0;
+  a.z = z;
+  a.mx = p->mx;
+  a.my = p->my;
+  a.lsmx = p->lsmax_x;
+  a.lsmy = p->lsmax_y;
+  a.nsx = 2 * p->lsmax_x + 1;
+  a.nsy = 2 * p->lsmax_y + 1;
+  a.lmax = lmax;
+  a.mmax = mmax;
+  a.nrange = nrange;
+  a.lpx = lpx;
+  a.lpy = lpy;
+  std::vector<std::complex<double>> twx(nx), twy(ny);
+  const std::complex<double> zi(0.0, 1.0);
+  for (int i = 0; i < nx; ++i) twx[i] = std::exp(2.0 * PI * zi * double(i) / double(nx));
+  for (int i = 0; i < ny; ++i) twy[i] = std::exp(2.0 * PI * zi * double(i) / double(ny));
+  Upload up;
+  const size_t o_map = up.add(h_modemap, sizeof(int) * (2 * lmax + 1) * (2 * mmax + 1));
+  const size_t o_off = up.add(h_off, sizeof(double) * 4 * n_off);
+  const size_t o_src = up.add(h_src, sizeof(int) * 3 * n_src);
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * ncut);
+  const size_t o_twx = up.add(twx.data(), sizeof(double2) * nx);
+  const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
+  // process sources in chunks so the two spectrum buffers stay bounded (~256 MB)
+  const int64_t per_src = (int64_t)3 * nl * (nm + ny);
+  int chunk = (int)std::max<int64_t>(1, (int64_t)(16 << 20) / per_src);
+  if (chunk > n_src) chunk = n_src;
+  const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
+  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
+  EUVL_TRY(ws_reserve(h, up.host.size() + fbytes + gbytes + 4096));
+  char* blk = (char*)ws_take(h, up.host.size());
+  double2* F = (double2*)ws_take(h, fbytes);
+  double2* G = (double2*)ws_take(h, gbytes);
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  for (int s0 = 0; s0 < n_src; s0 += chunk) {
+    const int ns = std::min(chunk, n_src - s0);
+    EUVL_TRY(fill(h, (int64_t)ns * 3 * nl * nm, F, make_double2(0.0, 0.0)));
+    hipLaunchKernelGGL(abbe_field_kernel, grid1((int64_t)ns * ncut), dim3(256), 0, h->stream, a, (const double2*)d_Ax,
+                       (const int*)(blk + o_map), (const double*)(blk + o_off), (const int*)(blk + o_src) + 3 * s0, ns,
+                       (const int*)(blk + o_pup), ncut, F);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl, nm,
+                       lpy, ny, F, (const double2*)(blk + o_twy), G);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + 255) / 256, nx), dim3(256), sizeof(double2) * nl, h->stream,
+                       ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), weight, d_img);
+    EUVL_LAUNCH_CHECK();
+  }
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}
+
+extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
+                            const double* d_Ax, int nrange, int lmax, int mmax, const int* h_modemap,
+                            const double* h_vc, const double* h_ab, double kx0, double ky0, double z0, int ncut,
+                            const int* h_pupil, double* d_a0, double* d_ax, double* d_ay) {
+  using namespace euvl;
+  if (!h || !p || !d_fmask || !d_Ax || !h_modemap || !h_vc || !h_ab || ncut <= 0 || !h_pupil || !d_a0 || !d_ax ||
+      !d_ay) {
+    set_error("euvl_m3d_fit: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  M3dSetup a;
+  a.k = p->k;
+  a.dx = p->dx;
+  a.dy = p->dy;
+  a.na = p->na;
+  a.lambda = p->wavelength;
+  a.kx0 = kx0;
+  a.ky0 = ky0;
+  a.z0 = z0;
+  a.mx = p->mx;
+  a.my = p->my;
+  a.lsmx = p->lsmax_x;
+  a.lsmy = p->lsmax_y;
+  a.nsx = 2 * p->lsmax_x + 1;
+  a.nsy = 2 * p->lsmax_y + 1;
+  a.lpmx = lpmax_x;
+  a.lpmy = lpmax_y;
+  a.lmax = lmax;
+  a.mmax = mmax;
+  a.nrange = nrange;
+  for (int i = 0; i < ncut; ++i)
+    if (std::abs(h_pupil[3 * i]) > lpmax_x || std::abs(h_pupil[3 * i + 1]) > lpmax_y) {
+      set_error("euvl_m3d_fit: pupil point %d outside the thin-mask window", i);
+      return EUVL_ERR_ARG;
+    }
+  Upload up;
+  const size_t o_map = up.add(h_modemap, sizeof(int) * (2 * lmax + 1) * (2 * mmax + 1));
+  const size_t o_vc = up.add(h_vc, sizeof(double2) * a.nsx * a.nsy);
+  const size_t o_ab = up.add(h_ab, sizeof(double2) * a.nsx * a.nsy);
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 3 * ncut);
+  EUVL_TRY(ws_reserve(h, up.host.size() + 4096));
+  char* blk = (char*)ws_take(h, up.host.size());
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(m3d_fit_kernel, dim3((ncut + 127) / 128), dim3(128), 0, h->stream, a, (const double2*)d_fmask,
+                     (const double2*)d_Ax, (const int*)(blk + o_map), (const double2*)(blk + o_vc),
+                     (const double2*)(blk + o_ab), (const int*)(blk + o_pup), ncut, (double2*)d_a0, (double2*)d_ax,
+                     (double2*)d_ay);
+  EUVL_LAUNCH_CHECK();
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}

@@ -1,0 +1,184 @@
+"""Host layer over libeuvl: the EM solve, M3D extraction and rigorous Abbe image
+of one clip, mirroring the reference drivers
+
+  ampS            cnn/include/header.h:1208-1466       -> Engine.amp_s
+  m3d.cpp main    cnn/intensity/m3d/m3d.cpp:28-569      -> Engine.m3d
+  intensity.cpp   cnn/intensity/abbe/intensity.cpp:35-421 -> Engine.abbe
+
+Device memory and streams come from PyTorch (ROCm); all arithmetic runs in
+libeuvl's HIP kernels.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .config import Config
+from .masks import pack_bits
+
+
+class Engine:
+    def __init__(self, cfg: Config, device="cuda:0"):
+        if not torch.cuda.is_available():
+            raise _lib.EuvlError("a ROCm GPU is required (torch.cuda.is_available() is False)")
+        self.cfg = cfg
+        self.device = torch.device(device)
+        self.lib = _lib.load()
+        with torch.cuda.device(self.device):
+            h = ctypes.c_void_p()
+            _lib.check(self.lib.euvl_create(ctypes.byref(h)), "euvl_create")
+        self.h = h
+        self._layers = cfg.stack.layers()
+        eps = np.array([[complex(e).real, complex(e).imag] for e, _ in self._layers], dtype=np.float64).ravel()
+        thk = np.array([d for _, d in self._layers], dtype=np.float64)
+        self._eps, self._thk = eps, thk
+        lsx, lsy = cfg.lsmax
+        ea = cfg.eps_abs
+        self.params = _lib.EmParams(
+            cfg.ndivx, cfg.ndivy, cfg.mx, cfg.my, cfg.na, cfg.wavelength, cfg.k, cfg.dx, cfg.dy, lsx, lsy,
+            ea.real, ea.imag, cfg.d_abs, len(self._layers),
+            eps.ctypes.data_as(_lib.P_double), thk.ctypes.data_as(_lib.P_double))
+        self.lidx, self.midx, self.L, self.M = cfg.order_set()
+        self.modemap = Config.mode_map(self.lidx, self.midx, self.L, self.M)
+        self._blank = None
+        self.last_info = None
+
+    def close(self):
+        if self.h:
+            self.lib.euvl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _bind_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib.euvl_set_stream(self.h, ctypes.c_void_p(s)), "euvl_set_stream")
+
+    # ------------------------------------------------------------------ inputs
+    def upload_mask(self, mask2d):
+        """mask2d (0/1 array of ndivx*ndivy, index ndivx*j + i) or packed mask.bin bytes -> device uint8."""
+        if isinstance(mask2d, (bytes, bytearray)):
+            raw = np.frombuffer(mask2d, dtype=np.uint8)
+        else:
+            raw = np.frombuffer(pack_bits(mask2d), dtype=np.uint8)
+        return torch.from_numpy(raw.copy()).to(self.device)
+
+    def mask_spectrum(self, bits, lw, mw, n_clips=1):
+        """Indicator spectrum window (n_clips, 2lw+1, 2mw+1) complex128 (euvl_mask_spectrum)."""
+        out = torch.empty((n_clips, 2 * lw + 1, 2 * mw + 1), dtype=torch.complex128, device=self.device)
+        self._bind_stream()
+        _lib.check(self.lib.euvl_mask_spectrum(self.h, _lib.ptr(bits), self.cfg.ndivx, self.cfg.ndivy, n_clips,
+                                               lw, mw, _lib.ptr(out)), "euvl_mask_spectrum")
+        return out
+
+    # ---------------------------------------------------------------- EM solve
+    def amp_s(self, bits, polar="X", offsets=None, orders=None):
+        """Ax[o, ls, ms, n] for the incidence offsets (sx0, sy0) (default: the
+        source divisions of the config).  ``orders``: (lidx, midx, L, M)."""
+        cfg = self.cfg
+        lidx, midx, L, M = orders if orders is not None else (self.lidx, self.midx, self.L, self.M)
+        if offsets is None:
+            offsets = cfg.offsets()[:, :2]
+        offsets = np.asarray(offsets, dtype=np.float64).reshape(-1, 2)
+        spec = self.mask_spectrum(bits, 2 * L, 2 * M)
+        nsx, nsy = cfg.nsource
+        n_off = len(offsets)
+        Ax = torch.empty((n_off, nsx, nsy, len(lidx)), dtype=torch.complex128, device=self.device)
+        li, lp = _lib.arr_i(lidx)
+        mi, mp = _lib.arr_i(midx)
+        sx, sxp = _lib.arr_d(offsets[:, 0])
+        sy, syp = _lib.arr_d(offsets[:, 1])
+        info = np.zeros(4 * n_off, dtype=np.int32)
+        self._bind_stream()
+        _lib.check(self.lib.euvl_amp_s(self.h, polar.encode(), ctypes.byref(self.params), lp, mp, len(lidx), L, M,
+                                       _lib.ptr(spec), n_off, sxp, syp, _lib.ptr(Ax),
+                                       info.ctypes.data_as(_lib.P_int)), "euvl_amp_s")
+        self.last_info = info.reshape(n_off, 4)
+        return Ax
+
+    def blank_amplitudes(self, polar="X"):
+        """vcxx, abxx: blank vacuum / absorber amplitudes on the narrow order set at
+        the central offset (m3d.cpp:211-258); cached per engine."""
+        if self._blank is not None:
+            return self._blank
+        cfg = self.cfg
+        orders = cfg.order_set(cfg.narrow_orders)
+        n = cfg.ndivx * cfg.ndivy
+        vc = self.amp_s(self.upload_mask(np.ones(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
+        ab = self.amp_s(self.upload_mask(np.zeros(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
+        lidx, midx, L, M = orders
+        mm = Config.mode_map(lidx, midx, L, M)
+        nsx, nsy = cfg.nsource
+        lsx, lsy = cfg.lsmax
+        vcxx = np.zeros((nsx, nsy), dtype=np.complex128)
+        abxx = np.zeros((nsx, nsy), dtype=np.complex128)
+        vch, abh = vc.cpu().numpy(), ab.cpu().numpy()
+        for is_ in range(nsx):
+            for js in range(nsy):
+                ls, ms = is_ - lsx, js - lsy
+                if cfg.in_na(ls, ms) and abs(ls) <= L and abs(ms) <= M:
+                    n_ = mm[(ls + L) * (2 * M + 1) + (ms + M)]
+                    if n_ >= 0:
+                        vcxx[is_, js] = vch[is_, js, n_]
+                        abxx[is_, js] = abh[is_, js, n_]
+        self._blank = (vcxx, abxx)
+        return self._blank
+
+    # --------------------------------------------------------------- M3D params
+    def m3d(self, bits, Ax0=None):
+        """(a0, ax, ay) of one clip (m3d.cpp main); ax, ay only where xinput = 1.
+        Ax0: wide amplitudes at the central offset (computed when None)."""
+        cfg = self.cfg
+        vcxx, abxx = self.blank_amplitudes()
+        if Ax0 is None:
+            Ax0 = self.amp_s(bits, "X", [(cfg.kx0, cfg.ky0)])[0]
+        lpx, lpy = cfg.lpmax
+        fmask = self.mask_spectrum(bits, lpx, lpy)[0]
+        lin, min_, snum = cfg.pupil
+        pup = np.stack([lin, min_, (snum >= 8).astype(np.int32)], axis=1)
+        ncut = len(lin)
+        a0 = torch.empty(ncut, dtype=torch.complex128, device=self.device)
+        ax = torch.empty_like(a0)
+        ay = torch.empty_like(a0)
+        mm, mmp = _lib.arr_i(self.modemap)
+        pu, pup_p = _lib.arr_i(pup)
+        vc, vcp = _lib.arr_d(np.stack([vcxx.real, vcxx.imag], -1))
+        ab, abp = _lib.arr_d(np.stack([abxx.real, abxx.imag], -1))
+        self._bind_stream()
+        _lib.check(self.lib.euvl_m3d_fit(self.h, ctypes.byref(self.params), _lib.ptr(fmask), lpx, lpy,
+                                         _lib.ptr(Ax0.contiguous()), len(self.lidx), self.L, self.M, mmp, vcp, abp,
+                                         cfg.kx0, cfg.ky0, cfg.z0, ncut, pup_p, _lib.ptr(a0), _lib.ptr(ax),
+                                         _lib.ptr(ay)), "euvl_m3d_fit")
+        return a0, ax, ay
+
+    # ----------------------------------------------------------- Abbe (EM amps)
+    def abbe(self, bits, z=0.0, Ax=None):
+        """Rigorous Abbe image (nx, ny) of one clip, X polarisation (intensity.cpp main)."""
+        cfg = self.cfg
+        if Ax is None:
+            Ax = self.amp_s(bits, "X")
+        offs = cfg.offsets()
+        src = []
+        divs = cfg.source_points()
+        for o, (nsx, nsy, l0, m0) in enumerate(divs):
+            src += [(o, int(a), int(b)) for a, b in zip(l0, m0)]
+        src = np.array(src, dtype=np.int32).reshape(-1, 3)
+        lin, min_, _ = cfg.pupil
+        pup = np.stack([lin, min_], axis=1)
+        nx, ny = cfg.image_shape
+        img = torch.zeros((nx, ny), dtype=torch.float64, device=self.device)
+        mm, mmp = _lib.arr_i(self.modemap)
+        of, ofp = _lib.arr_d(offs)
+        sr, srp = _lib.arr_i(src)
+        pu, pup_p = _lib.arr_i(pup)
+        self._bind_stream()
+        _lib.check(self.lib.euvl_abbe_image(self.h, ctypes.byref(self.params), _lib.ptr(Ax), len(self.lidx), self.L,
+                                            self.M, mmp, len(offs), ofp, len(src), srp, len(lin), pup_p, cfg.kx0,
+                                            cfg.ky0, cfg.z0, float(z), nx, ny, 1.0 / max(len(src), 1),
+                                            _lib.ptr(img)), "euvl_abbe_image")
+        return img

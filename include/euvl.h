@@ -1,0 +1,194 @@
+/*
+ * euvl.h -- C ABI of libeuvl.so, the MI355X (gfx950) implementation of the
+ * EUVlitho mask -> rigorous EM -> M3D -> image path.
+ *
+ * Conventions
+ *   - Every entry point returns an int status: EUVL_OK (0) or a negative code.
+ *     The message of the last failure on the calling thread is returned by
+ *     euvl_last_error().  Nothing aborts the process.
+ *   - "d_" pointers are device memory owned by the caller; "h_" pointers are
+ *     host memory read during the call only.  Complex numbers are interleaved
+ *     (re, im) doubles; matrices are row-major.
+ *   - Work is enqueued on the stream set with euvl_set_stream (default: the
+ *     null stream).  Entry points that need a result on the host to continue
+ *     (convergence checks, the expm scaling choice) synchronise that stream.
+ *   - One handle must not be used by two host threads at once; handles own
+ *     their device workspaces.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the takahashi-edalab/EUVlitho tree).
+ */
+#ifndef EUVL_H
+#define EUVL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EUVL_OK 0
+#define EUVL_ERR_ARG -1          /* bad argument (size, pointer, polarisation) */
+#define EUVL_ERR_HIP -2          /* HIP runtime failure (launch, allocation) */
+#define EUVL_ERR_SINGULAR -3     /* zero pivot in an LU factorisation */
+#define EUVL_ERR_NOT_CONVERGED -4 /* matrix square root / Neumann / expm bound failed */
+#define EUVL_ERR_NOMEM -5        /* workspace allocation failed */
+
+typedef struct euvl_handle_s* euvl_handle;
+
+/* Handle on the current HIP device.  No reference counterpart: the reference
+ * creates a cuSOLVER/cuBLAS handle, stream and workspace inside every call
+ * (cnn/include/header.h:796-804, 1501-1508, 1590-1593). */
+int euvl_create(euvl_handle* out);
+int euvl_destroy(euvl_handle h);
+int euvl_set_stream(euvl_handle h, void* hip_stream);
+const char* euvl_last_error(void);
+/* Device bytes currently held by the handle's workspace. */
+int64_t euvl_workspace_bytes(euvl_handle h);
+/* Per-launch timing of the GEMM kernel with HIP events on the handle's
+ * stream.  If out is non-null, synchronises and writes {launches, total
+ * milliseconds, algorithmic flops (8 m n k per complex GEMM)} of the launches
+ * recorded since the last call; then clears them and enables (1) or disables
+ * (0) recording.  No reference counterpart (instrumentation). */
+int euvl_gemm_stats(euvl_handle h, int enable, double* out);
+
+/* ---------------------------------------------------------------------------
+ * Dense complex linear algebra (batched, row-major, interleaved complex).
+ * ------------------------------------------------------------------------- */
+
+/* C[b] = alpha * A[b] * B[b] + beta * C[b] + gamma * I, b < batch.
+ * A: m x k (leading dim lda, batch stride sa), B: k x n, C: m x n.
+ * Replaces matproduct (cnn/include/header.h:1559-1619: cublasZgemm OP_N/OP_N
+ * above N = 5000, Eigen A*B below). alpha/beta/gamma are (re, im) pairs. */
+int euvl_zgemm_batched(euvl_handle h, int m, int n, int k,
+                       const double* alpha,
+                       const double* d_A, int64_t lda, int64_t sa,
+                       const double* d_B, int64_t ldb, int64_t sb,
+                       const double* beta,
+                       double* d_C, int64_t ldc, int64_t sc,
+                       const double* gamma, int batch);
+
+/* In-place LU with partial pivoting, A[b] = P L U (n x n, leading dim lda,
+ * batch stride sa); d_ipiv[b*n + i] = row swapped with row i (0-based).
+ * Replaces the factorisation half of matinv (cnn/include/header.h:1468-1557:
+ * cusolverDnXgetrf above N = 5000, Eigen colPivHouseholderQr below).
+ * Returns EUVL_ERR_SINGULAR if a pivot is exactly zero. */
+int euvl_zgetrf_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t sa,
+                        int* d_ipiv, int batch);
+
+/* Solve A[b] X = B[b] in place of B (n x nrhs, leading dim ldb) using the
+ * factors of euvl_zgetrf_batched.  Replaces cusolverDnXgetrs in matinv
+ * (cnn/include/header.h:1530-1535). */
+int euvl_zgetrs_batched(euvl_handle h, int n, int nrhs, const double* d_LU, int64_t lda, int64_t sa,
+                        const int* d_ipiv, double* d_B, int64_t ldb, int64_t sb, int batch);
+
+/* Principal square root S = D^(1/2) and S^-1 of D[b] (n x n) by the coupled
+ * Newton-Schulz iteration started from D/c (c = mean of Re diag D).  Stands in
+ * for the eigen-pairs + al1 = sqrt(lambda) of absorberS0
+ * (cnn/include/header.h:834-921): every later use of (V, al1) in the
+ * reference is a matrix function of D, so S and S^-1 carry the same content.
+ * d_S, d_Sinv: n x n per batch entry, contiguous (stride n*n).
+ * *h_iters receives the iteration count; EUVL_ERR_NOT_CONVERGED if the
+ * residual max|T - I| is not below tol after max_iter iterations. */
+int euvl_zsqrtm_batched(euvl_handle h, int n, const double* d_D, double* d_S, double* d_Sinv,
+                        int batch, double tol, int max_iter, int* h_iters);
+
+/* E = exp(i*d*S), Einv = exp(-i*d*S) for each batch entry (d = thickness[b]),
+ * by Taylor degree 18 with scaling and squaring, after factoring out the
+ * scalar exp(i d sqrt(c)).  Stands in for gamma = exp(i al1 dabs) of
+ * absorberS0 (cnn/include/header.h:1085-1140) in matrix form. */
+int euvl_zexpm_pair_batched(euvl_handle h, int n, const double* d_S, double shift_re,
+                            const double* h_thickness, double* d_E, double* d_Einv,
+                            int batch, int* h_squarings);
+
+/* ---------------------------------------------------------------------------
+ * Rigorous EM solve.
+ * ------------------------------------------------------------------------- */
+
+/* Optics and stack shared by the EM entry points (mirrors the scalar
+ * arguments of ampS, cnn/include/header.h:1208-1213, and its material table
+ * header.h:1217-1230 given as an explicit layer list). */
+typedef struct {
+  int ndivx, ndivy;        /* mask pixels (1 nm pixels: FDIVX = NDIVX) */
+  int mx, my;              /* magnification (MX, MY) */
+  double na, wavelength, k;/* NA, lambda (nm), k = 2 pi / lambda */
+  double dx, dy;           /* pitch (nm) */
+  int lsmax_x, lsmax_y;    /* incidence lattice half-widths (lsmaxX, lsmaxY) */
+  double eabs_re, eabs_im; /* absorber permittivity (NABS = 1) */
+  double dabs;             /* absorber thickness (nm) */
+  int n_layers;            /* multilayer layers, bottom (substrate) to top (cap) */
+  const double* h_layer_eps; /* 2*n_layers: (re, im) permittivity per layer */
+  const double* h_layer_d;   /* n_layers thicknesses (layer 0 unused) */
+} euvl_em_params;
+
+/* Fourier coefficients of the binary mask indicator m (1 = vacuum) on the
+ * window |l| <= lw, |m| <= mw:  out[(l+lw)*(2mw+1) + (m+mw)] =
+ * (1/(NDIVX NDIVY)) sum_{i,j} m(i,j) exp(-2 pi i (i l/NDIVX + j m/NDIVY)),
+ * with x = i and y = j counted from the bottom row (the y flip of
+ * header.h:1157-1168).  d_bits: packed mask.bin bytes of n_clips clips.
+ * out is n_clips x (2lw+1) x (2mw+1) complex.
+ * Replaces maskamp + fourier + exponential (cnn/include/header.h:1147-1206,
+ * 1621-1648) and the thin-mask spectrum of m3d.cpp:287-327: eps, 1/eps and
+ * log(eps) tables are affine in this one spectrum. */
+int euvl_mask_spectrum(euvl_handle h, const uint8_t* d_bits, int ndivx, int ndivy, int n_clips,
+                       int lw, int mw, double* d_out);
+
+/* Diagonal T-matrix cascade of the multilayer for n modes with transverse
+ * wave vectors (kx, ky) (device arrays of n doubles), polarisation 'X' or 'Y'.
+ * Writes d_uu, d_ub (n complex each).  Replaces multilayerS
+ * (cnn/include/header.h:5-363). */
+int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n,
+                    const double* d_kx, const double* d_ky, double* d_uu, double* d_ub);
+
+/* ampS for one clip and n_off incidence offsets (sx0[o], sy0[o]) at once
+ * (batched over offsets).  Order set (h_lindex, h_mindex) of length nrange
+ * with |l| <= lmax, |m| <= mmax.  d_spec: the clip's indicator spectrum from
+ * euvl_mask_spectrum with lw = 2 lmax, mw = 2 mmax.
+ * Output d_Ax: n_off x (2 lsmax_x+1) x (2 lsmax_y+1) x nrange complex;
+ * the entry [o][ls+lsmax_x][ms+lsmax_y][:] is Ax of header.h:1394-1465 for
+ * the lattice incidence (ls, ms) inside the NA and 0 elsewhere.
+ * h_info (optional, 4 ints per offset): Newton-Schulz iterations, expm
+ * squarings, Neumann factors, 0.
+ * Replaces ampS (cnn/include/header.h:1208-1466) with absorberS0
+ * (header.h:754-1145) and the top solve (header.h:1355-1391). */
+int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p,
+               const int* h_lindex, const int* h_mindex, int nrange, int lmax, int mmax,
+               const double* d_spec, int n_off, const double* h_sx0, const double* h_sy0,
+               double* d_Ax, int* h_info);
+
+/* ---------------------------------------------------------------------------
+ * Image formation.
+ * ------------------------------------------------------------------------- */
+
+/* Rigorous Abbe image accumulation for n_src source points of one
+ * polarisation (X) from EM amplitudes, the per-source-point loop of
+ * intensity.cpp:276-388 (field, rotate(), defocus phase, unnormalised inverse
+ * DFT, |Ex|^2+|Ey|^2+|Ez|^2).  Adds weight * intensity into d_img (nx x ny
+ * doubles, index x*ny + y).
+ *   d_Ax: amplitudes, n_off x nsx x nsy x nrange complex (euvl_amp_s layout)
+ *   h_src: n_src x 3 ints (offset index o, l0s, m0s); h_off: per offset
+ *          (sx0, sy0, fx, fy) with fx = nsx/ndivs, fy = nsy/ndivs
+ *   h_pupil: ncut x 2 ints (linput, minput)
+ *   h_modemap: (2 lmax+1)(2 mmax+1) ints, mode index of (l, m) or -1
+ *   kx0, ky0: incidence of rotate(); z0, z: defocus terms of intensity.cpp:353 */
+int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const double* d_Ax, int nrange,
+                    int lmax, int mmax, const int* h_modemap,
+                    int n_off, const double* h_off, int n_src, const int* h_src,
+                    int ncut, const int* h_pupil, double kx0, double ky0, double z0, double z,
+                    int nx, int ny, double weight, double* d_img);
+
+/* M3D parameters (a0, ax, ay) of one clip, m3d.cpp:329-467: thin-mask
+ * spectrum d_fmask (noutx x nouty complex, euvl_mask_spectrum with lw =
+ * lpmax_x, mw = lpmax_y), rigorous amplitudes d_Ax of offset 0
+ * (nsx x nsy x nrange), blank amplitudes h_vc, h_ab (nsx x nsy complex, host),
+ * pupil points h_pupil (ncut x 3 ints: linput, minput, xinput flag).
+ * Outputs d_a0, d_ax, d_ay: ncut complex each (ax = ay = 0 where xinput = 0). */
+int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
+                 const double* d_Ax, int nrange, int lmax, int mmax, const int* h_modemap,
+                 const double* h_vc, const double* h_ab, double kx0, double ky0, double z0,
+                 int ncut, const int* h_pupil, double* d_a0, double* d_ax, double* d_ay);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EUVL_H */

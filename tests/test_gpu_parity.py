@@ -1,0 +1,249 @@
+"""GPU parity: libeuvl (through its C ABI) against the FP64 oracle and the
+reference fixtures.  Tolerances: FP64 amplitudes 1e-9 relative to the largest
+amplitude, images 1e-9 of peak (the bar of the workload spec is 1e-9 / 1e-6);
+reference fixtures carry 6 significant digits, so 2e-5 against them."""
+import ctypes
+import gzip
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test without a GPU")
+    from euvlitho_amd import _lib
+    return _lib.load()
+
+
+@pytest.fixture(scope="module")
+def handle(lib):
+    h = ctypes.c_void_p()
+    assert lib.euvl_create(ctypes.byref(h)) == 0
+    yield h
+    lib.euvl_destroy(h)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def P(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def cpx(z):
+    return (ctypes.c_double * 2)(z.real, z.imag)
+
+
+def crand(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def err(lib):
+    return lib.euvl_last_error().decode()
+
+
+# ----------------------------------------------------------------- LA kernels
+@pytest.mark.parametrize("m,n,k,batch", [(1, 1, 1, 1), (17, 33, 5, 2), (64, 64, 64, 1), (100, 70, 130, 3),
+                                         (129, 250, 47, 2)])
+def test_zgemm(lib, handle, m, n, k, batch):
+    say(f"zgemm m={m} n={n} k={k} batch={batch}")
+    rng = np.random.default_rng(m * 1000 + n)
+    A, B, C = crand(rng, batch, m, k), crand(rng, batch, k, n), crand(rng, batch, m, n)
+    al, be, ga = 0.7 - 0.2j, -0.3 + 0.5j, 1.5 + 0.25j
+    dA, dB, dC = dev(A), dev(B), dev(C)
+    st = lib.euvl_zgemm_batched(handle, m, n, k, cpx(al), P(dA), k, m * k, P(dB), n, k * n, cpx(be), P(dC), n, m * n,
+                                cpx(ga), batch)
+    assert st == 0, err(lib)
+    torch.cuda.synchronize()
+    ref = al * (A @ B) + be * C + ga * np.eye(m, n)[None]
+    got = dC.cpu().numpy()
+    assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()) * math.sqrt(k)
+
+
+@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2)])
+def test_getrf_getrs(lib, handle, n, nrhs, batch):
+    say(f"getrf/getrs n={n} nrhs={nrhs} batch={batch}")
+    rng = np.random.default_rng(n)
+    A = crand(rng, batch, n, n)
+    Bm = crand(rng, batch, n, nrhs)
+    dA, dB = dev(A), dev(Bm)
+    piv = torch.zeros(batch * n, dtype=torch.int32, device="cuda")
+    assert lib.euvl_zgetrf_batched(handle, n, P(dA), n, n * n, P(piv), batch) == 0, err(lib)
+    assert lib.euvl_zgetrs_batched(handle, n, nrhs, P(dA), n, n * n, P(piv), P(dB), nrhs, n * nrhs, batch) == 0, err(lib)
+    torch.cuda.synchronize()
+    X = dB.cpu().numpy()
+    ref = np.linalg.solve(A, Bm)
+    assert np.abs(X - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_getrf_singular_reports(lib, handle):
+    say("getrf singular")
+    A = np.zeros((1, 8, 8), complex)
+    dA = dev(A)
+    piv = torch.zeros(8, dtype=torch.int32, device="cuda")
+    assert lib.euvl_zgetrf_batched(handle, 8, P(dA), 8, 64, P(piv), 1) == -3
+
+
+def test_sqrtm_expm(lib, handle):
+    import scipy.linalg as sla
+    say("sqrtm/expm n=200 batch=2")
+    rng = np.random.default_rng(5)
+    n, batch, c = 200, 2, 0.19
+    D = np.stack([c * (np.eye(n) + 0.25 * crand(rng, n, n) / math.sqrt(2 * n)) for _ in range(batch)])
+    dD = dev(D)
+    S = torch.empty_like(dD)
+    Si = torch.empty_like(dD)
+    it = ctypes.c_int()
+    assert lib.euvl_zsqrtm_batched(handle, n, P(dD), P(S), P(Si), batch, 1e-8, 30, ctypes.byref(it)) == 0, err(lib)
+    torch.cuda.synchronize()
+    Sh, Sih = S.cpu().numpy(), Si.cpu().numpy()
+    for b in range(batch):
+        ref = sla.sqrtm(D[b])
+        assert np.abs(Sh[b] - ref).max() < 1e-12 * np.abs(ref).max()
+        assert np.abs(Sih[b] @ Sh[b] - np.eye(n)).max() < 1e-12
+    E = torch.empty_like(dD)
+    Ei = torch.empty_like(dD)
+    th = (ctypes.c_double * batch)(60.0, 45.0)
+    sq = ctypes.c_int()
+    assert lib.euvl_zexpm_pair_batched(handle, n, P(S), math.sqrt(c), th, P(E), P(Ei), batch, ctypes.byref(sq)) == 0, \
+        err(lib)
+    torch.cuda.synchronize()
+    for b, d_ in enumerate([60.0, 45.0]):
+        A = 1j * d_ * (Sh[b] - math.sqrt(c) * np.eye(n))
+        ref = sla.expm(A)
+        refi = sla.expm(-A)
+        assert np.abs(E.cpu().numpy()[b] - ref).max() < 1e-12 * np.abs(ref).max()
+        assert np.abs(Ei.cpu().numpy()[b] - refi).max() < 1e-12 * np.abs(refi).max()
+
+
+# ---------------------------------------------------------------- EM pieces
+def c1_problem(K=10, ndiv=512):
+    from euvlitho_amd.config import high_na
+    return high_na(ndiv, K, dict(kind=1, sigma_out=0.8, sigma_in=0.3), ndivs=2)
+
+
+def oracle_problem(cfg):
+    from oracle import problems as OP, geometry as G
+    p = OP.hna(cfg.ndivx, cfg.orders[1], None)
+    return p, G.derived(p)
+
+
+def test_mask_spectrum_vs_oracle(lib, handle):
+    from euvlitho_amd.masks import manhattan, pack_bits
+    from oracle import em as OE
+    say("mask spectrum 256x128")
+    nx, ny, L, M = 256, 128, 6, 4
+    masks = [manhattan(nx, ny, 0.5, s) for s in range(2)]
+    bits = dev(np.frombuffer(b"".join(pack_bits(m) for m in masks), np.uint8).copy())
+    out = torch.empty((2, 4 * L + 1, 4 * M + 1), dtype=torch.complex128, device="cuda")
+    assert lib.euvl_mask_spectrum(handle, P(bits), nx, ny, 2, 2 * L, 2 * M, P(out)) == 0, err(lib)
+    torch.cuda.synchronize()
+    for c in range(2):
+        ref = OE.maskamp(masks[c], nx, ny, nx, ny, L, M, 0.0, 1.0)
+        assert np.abs(out.cpu().numpy()[c] - ref).max() < 1e-14
+
+
+def test_multilayer_vs_oracle(lib, handle):
+    from euvlitho_amd.engine import Engine
+    from oracle import em as OE, problems as OP
+    say("multilayer")
+    cfg = c1_problem()
+    eng = Engine(cfg)
+    kx = np.linspace(-0.1, 0.1, 33)
+    ky = np.linspace(0.05, -0.12, 33)
+    dkx, dky = dev(kx), dev(ky)
+    uu = torch.empty(33, dtype=torch.complex128, device="cuda")
+    ub = torch.empty_like(uu)
+    for pol in "XY":
+        assert lib.euvl_multilayer(eng.h, pol.encode(), ctypes.byref(eng.params), 33, P(dkx), P(dky), P(uu),
+                                   P(ub)) == 0, err(lib)
+        torch.cuda.synchronize()
+        U, B = OE.multilayer(pol, cfg.k, kx, ky, kx ** 2 + ky ** 2, OP.HNA_STACK)
+        assert np.abs(uu.cpu().numpy() - U).max() < 1e-12 * np.abs(U).max()
+        assert np.abs(ub.cpu().numpy() - B).max() < 1e-12 * np.abs(B).max()
+
+
+@pytest.mark.parametrize("pol", ["X", "Y"])
+def test_amp_s_c1_vs_oracle(pol):
+    """512 nm clip, +-10 orders (N = 441), two incidence offsets in one batch."""
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import em as OE
+    say(f"amp_s C1 pol={pol}")
+    cfg = c1_problem()
+    eng = Engine(cfg)
+    m = manhattan(512, 512, 0.5, 1)
+    offs = cfg.offsets()[[0, 3], :2]
+    Ax = eng.amp_s(eng.upload_mask(m), pol, offs).cpu().numpy()
+    say("  info", eng.last_info.tolist())
+    p, d = oracle_problem(cfg)
+    for o in range(2):
+        ref = OE.amp_s(pol, m, p, d, eng.lidx.astype(np.int64), eng.midx.astype(np.int64), eng.L, eng.M,
+                       offs[o, 0], offs[o, 1])
+        e = np.abs(Ax[o] - ref).max() / np.abs(ref).max()
+        say(f"  offset {o}: max rel err {e:.3e}")
+        assert e < 1e-9
+
+
+def test_m3d_c1_vs_oracle():
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import abbe as OA, em as OE
+    say("m3d C1")
+    cfg = c1_problem()
+    eng = Engine(cfg)
+    m = manhattan(512, 512, 0.5, 2)
+    bits = eng.upload_mask(m)
+    a0, ax, ay = [t.cpu().numpy() for t in eng.m3d(bits)]
+    p, d = oracle_problem(cfg)
+    li, mi = eng.lidx.astype(np.int64), eng.midx.astype(np.int64)
+    Ax = OE.amp_s("X", m, p, d, li, mi, eng.L, eng.M, d["kx0"], d["ky0"])
+    vc, ab = OA.blank_amplitudes(p, d)
+    r0, rx, ry = OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
+    s = np.abs(r0).max()
+    for g, r in ((a0, r0), (ax, rx), (ay, ry)):
+        e = np.abs(g - r).max() / s
+        say(f"  max err / max|a0| = {e:.3e}")
+        assert e < 1e-9
+
+
+def test_abbe_c1_vs_oracle():
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import abbe as OA, em as OE
+    say("abbe C1 (4 offsets, 256x256 image)")
+    import dataclasses
+    cfg = dataclasses.replace(c1_problem(), image=(256, 256))
+    eng = Engine(cfg)
+    m = manhattan(512, 512, 0.5, 3)
+    bits = eng.upload_mask(m)
+    Ax = eng.amp_s(bits, "X")
+    img = eng.abbe(bits, Ax=Ax).cpu().numpy()
+    p, d = oracle_problem(cfg)
+    li, mi = eng.lidx.astype(np.int64), eng.midx.astype(np.int64)
+    offs = cfg.offsets()
+    divs = [((o[0], o[1], o[2], o[3]), l0, m0) for o, (_, _, l0, m0) in zip(offs, cfg.source_points())]
+    amps = [a for a in Ax.cpu().numpy()]
+    ref = OA.abbe_image(p, d, m, li, mi, eng.L, eng.M, divs, 256, 256, amps=amps)
+    e = np.abs(img - ref).max() / ref.max()
+    say(f"  image max err / peak = {e:.3e}")
+    assert e < 1e-12
+    # and the EM amplitudes of every offset against the oracle
+    for q, o in enumerate(offs):
+        r = OE.amp_s("X", m, p, d, li, mi, eng.L, eng.M, o[0], o[1])
+        assert np.abs(amps[q] - r).max() < 1e-9 * np.abs(r).max()

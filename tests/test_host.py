@@ -1,0 +1,90 @@
+"""Host-side logic (CPU): the C ABI library loads and exports what
+include/euvl.h declares; the typed config reproduces the reference geometry
+(checked against the oracle restatement and the reference fixture)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from euvlitho_amd import _lib
+from euvlitho_amd.config import Config, high_na, reference_default
+from euvlitho_amd.masks import manhattan, pack_bits, unpack_bits
+from oracle import geometry as G, problems as P
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    with open(os.path.join(ROOT, "include", "euvl.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(euvl_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libeuvl.so not built")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    syms = header_symbols()
+    assert "euvl_amp_s" in syms and "euvl_zgemm_batched" in syms
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_lib.EXPORTS)
+    _lib.load()  # prototypes bind
+
+
+def test_reference_default_geometry_matches_oracle(golden):
+    cfg = reference_default()
+    p = P.REF_DEFAULT
+    d = G.derived(p)
+    assert cfg.lsmax == (d["lsmaxX"], d["lsmaxY"]) and cfg.lpmax == (d["lpmaxX"], d["lpmaxY"])
+    li, mi, L, M = cfg.order_set()
+    oli, omi, oL, oM = G.order_set(p, "diamond", cutoff=6.0)
+    assert (L, M) == (oL, oM) and np.array_equal(li, oli) and np.array_equal(mi, omi)
+    lin, min_, snum = cfg.pupil
+    ref = np.loadtxt(os.path.join(golden, "ref_inputlm.csv"), delimiter=",", dtype=np.int64)
+    assert np.array_equal(lin, ref[:, 0]) and np.array_equal(min_, ref[:, 1])
+    assert np.array_equal(snum >= 8, ref[:, 2] == 1)
+    (_, _, l0, m0), = cfg.source_points()
+    src = G.source_lattice(p, d, 1, 1, 0.9, 0.55, 90.0)[(0, 0)]
+    assert np.array_equal(l0, src[0]) and np.array_equal(m0, src[1])
+    assert abs(cfg.kx0 - d["kx0"]) == 0 and abs(cfg.ky0 - d["ky0"]) == 0
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_high_na_sources_match_oracle(nd):
+    cfg = high_na(512, 10, dict(kind=2, sigma_out=0.9, sigma_in=0.5, open_angle=60.0,
+                                extra=((1, 0.8, 0.7, 90.0),)), ndivs=nd)
+    p = P.hna(512, 10, None)
+    d = G.derived(p)
+    ref = G.source_lattice(p, d, nd, ((2, 0.9, 0.5, 60.0), (1, 0.8, 0.7, 90.0)), None, None, None)
+    for nsx, nsy, l0, m0 in cfg.source_points():
+        assert np.array_equal(l0, ref[(nsx, nsy)][0]) and np.array_equal(m0, ref[(nsx, nsy)][1])
+
+
+def test_stack_layers_match_oracle():
+    from euvlitho_amd.config import HIGH_NA_STACK, REFERENCE_STACK
+    for st, ost in ((REFERENCE_STACK, P.REF_STACK), (HIGH_NA_STACK, P.HNA_STACK)):
+        a = st.layers()
+        b = P.layer_list(ost)
+        assert len(a) == len(b)
+        for (e1, d1), (e2, d2) in zip(a, b):
+            assert abs(e1 - e2) == 0 and d1 == d2
+
+
+def test_mask_pack_roundtrip_and_density():
+    m = manhattan(256, 128, 0.5, 9)
+    assert 0.5 <= m.mean() < 0.56
+    assert np.array_equal(unpack_bits(pack_bits(m), 256, 128), m)
+    m2 = manhattan(256, 128, 0.5, 9)
+    assert np.array_equal(m, m2)
+
+
+def test_engine_requires_gpu():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from euvlitho_amd.engine import Engine
+    with pytest.raises(_lib.EuvlError):
+        Engine(reference_default())
