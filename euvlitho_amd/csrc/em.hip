@@ -461,7 +461,18 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, G2, N, sm, ZERO, B));           // G^2
   EUVL_TRY(ew_lincomb(h, N, B, F, ONE, G, ONE, G2, ZERO, nullptr, ONE));                    // I + G + G^2
   EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G2, N, sm, ONE, F, N, sm, ZERO, B));            // + G^3
-  for (int f = 0; f < 2; ++f) {
+  // F = prod_j (I + G^(2^j)) = sum_{i < 2^(j+1)} G^i; stop once the remainder
+  // bound ||G^(2^(j+1))||_1 <= ||G^(2^j)||_1^2 is below 1e-16
+  for (;;) {
+    std::vector<double> nr(B);
+    EUVL_TRY(reduce_to_host(h, 1, N, B, G2, nr.data()));
+    double worst = 0.0;
+    for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
+    if (worst * worst < 1e-16) break;
+    if (!(worst < 0.5) || nfac >= 8) {
+      set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)||_1 = %.3e)", nfac, worst);
+      return EUVL_ERR_NOT_CONVERGED;
+    }
     // G <- G2^2 ; F <- F (I + G)
     EUVL_TRY(zgemm(h, N, N, N, ONE, G2, N, sm, G2, N, sm, ZERO, G, N, sm, ZERO, B));
     std::swap(G, G2);  // G2 = new power
@@ -469,17 +480,6 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
     EUVL_TRY(zgemm(h, N, N, N, ONE, F, N, sm, G2, N, sm, ONE, Fn, N, sm, ZERO, B));
     std::swap(F, Fn);
     ++nfac;
-  }
-  // remainder bound: ||G^16||_1 <= ||G^8||_1^2
-  {
-    std::vector<double> nr(B);
-    EUVL_TRY(reduce_to_host(h, 1, N, B, G2, nr.data()));
-    double worst = 0.0;
-    for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
-    if (!(worst * worst < 1e-15)) {
-      set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^8||_1 = %.3e)", worst);
-      return EUVL_ERR_NOT_CONVERGED;
-    }
   }
   // Wp (in F), Wm (in Sinv)
   hipLaunchKernelGGL(wbuild_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, F, Sinv, v, sqc, p->dabs);
