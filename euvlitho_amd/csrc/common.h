@@ -76,6 +76,9 @@ struct euvl_handle_s {
   size_t ws_cap = 0;
   size_t ws_top = 0;
   size_t ws_peak = 0;  // largest request seen (grow target)
+  // separate growable buffer for kernel-internal scratch (LU panels)
+  char* aux = nullptr;
+  size_t aux_cap = 0;
   // small pinned host buffer for reductions read back by the host
   double* h_pinned = nullptr;
   double* d_red = nullptr;  // device reduction slots
@@ -91,4 +94,6 @@ namespace euvl {
 int ws_reserve(euvl_handle h, size_t bytes);
 // Carve `bytes` (256-byte aligned) from the reserved workspace.
 void* ws_take(euvl_handle h, size_t bytes);
+// Ensure the aux buffer holds at least `bytes` and return it.
+void* aux_reserve(euvl_handle h, size_t bytes);
 }  // namespace euvl

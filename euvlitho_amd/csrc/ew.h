@@ -17,7 +17,7 @@ int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, c
 int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
                  const double2* col);
 int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X);
-// kind 0: max|X-I| (|re|+|im|), 1: 1-norm, 2: mean Re diag
+// kind 0: max|X-I| (|re|+|im|), 1: infinity norm (max row sum of |x|), 2: mean Re diag
 int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red);
 int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out);
 

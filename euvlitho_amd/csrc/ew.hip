@@ -47,44 +47,73 @@ __global__ void diagscale_kernel(int n, int64_t total, double2* out, const doubl
   }
 }
 
-// per-batch reductions into red[b]: 0 = max |X - I| (elementwise max of |re|+|im|),
-// 1 = 1-norm (max column sum of |x|), 2 = mean of Re diag
-template <int KIND>
-__global__ __launch_bounds__(256) void reduce_kernel(int n, const double2* X, double* red) {
+// Per-batch reductions into red[b] (red zeroed beforehand for kinds 0 and 1).
+// Non-negative doubles order like their bit patterns, so the maxima use an
+// integer atomicMax; NaN maps above +inf and propagates as "too large".
+__device__ inline void atomic_max_nonneg(double* addr, double v) {
+  atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ inline double wave_max(double v) {  // NaN wins
+  for (int off = 32; off > 0; off >>= 1) {
+    const double o = __shfl_xor(v, off);
+    v = (o > v || o != o) ? o : v;
+  }
+  return v;
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// kind 0: max |X - I| (elementwise |re| + |im|); grid (blocks, batch)
+__global__ __launch_bounds__(256) void reduce_maxdev_kernel(int n, const double2* X, double* red) {
+  const int b = blockIdx.y;
+  const int64_t per = (int64_t)n * n;
+  X += (int64_t)b * per;
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = X[e];
+    if (e / n == e % n) v.x -= 1.0;
+    const double a = fabs(v.x) + fabs(v.y);
+    acc = (a > acc || a != a) ? a : acc;
+  }
+  acc = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(red + b, acc);
+}
+
+// kind 1: infinity norm (max row sum of |x|), one wave per row; grid (ceil(n/4), batch)
+__global__ __launch_bounds__(256) void reduce_infnorm_kernel(int n, const double2* X, double* red) {
+  const int b = blockIdx.y;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double2* x = X + (int64_t)b * n * n + (int64_t)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += hypot(x[j].x, x[j].y);
+  s = wave_sum(s);
+  if (lane == 0) atomic_max_nonneg(red + b, s);
+}
+
+// kind 2: mean of Re diag, one block per batch entry
+__global__ __launch_bounds__(256) void reduce_meandiag_kernel(int n, const double2* X, double* red) {
   const int b = blockIdx.x;
   X += (int64_t)b * n * n;
   __shared__ double sh[256];
   double acc = 0.0;
-  if (KIND == 0) {
-    for (int64_t e = threadIdx.x; e < (int64_t)n * n; e += blockDim.x) {
-      double2 v = X[e];
-      if (e / n == e % n) v.x -= 1.0;
-      acc = fmax(acc, fabs(v.x) + fabs(v.y));
-    }
-  } else if (KIND == 1) {
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
-      double s = 0.0;
-      for (int i = 0; i < n; ++i) {
-        double2 v = X[(int64_t)i * n + j];
-        s += hypot(v.x, v.y);
-      }
-      acc = fmax(acc, s);
-    }
-  } else {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += X[(int64_t)i * n + i].x;
-  }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += X[(int64_t)i * n + i].x;
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) {
-      if (KIND == 2)
-        sh[threadIdx.x] += sh[threadIdx.x + s];
-      else
-        sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
-    }
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) red[b] = (KIND == 2) ? sh[0] / n : sh[0];
+  if (threadIdx.x == 0) red[b] = sh[0] / n;
+}
+
+__global__ void zero_kernel(double* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
 
 __global__ void copy_kernel(int64_t total, double2* out, const double2* X) {
@@ -133,12 +162,20 @@ int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X) {
 }
 
 int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red) {
-  if (kind == 0)
-    hipLaunchKernelGGL(reduce_kernel<0>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
-  else if (kind == 1)
-    hipLaunchKernelGGL(reduce_kernel<1>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
-  else
-    hipLaunchKernelGGL(reduce_kernel<2>, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
+  if (kind == 2) {
+    hipLaunchKernelGGL(reduce_meandiag_kernel, dim3(batch), dim3(256), 0, h->stream, n, X, d_red);
+    EUVL_LAUNCH_CHECK();
+    return EUVL_OK;
+  }
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, h->stream, d_red, batch);
+  EUVL_LAUNCH_CHECK();
+  if (kind == 0) {
+    int64_t blocks = ((int64_t)n * n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(reduce_maxdev_kernel, dim3((unsigned)blocks, batch), dim3(256), 0, h->stream, n, X, d_red);
+  } else {
+    hipLaunchKernelGGL(reduce_infnorm_kernel, dim3((n + 3) / 4, batch), dim3(256), 0, h->stream, n, X, d_red);
+  }
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

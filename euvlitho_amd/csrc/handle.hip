@@ -35,6 +35,22 @@ int ws_reserve(euvl_handle h, size_t bytes) {
   return EUVL_OK;
 }
 
+void* aux_reserve(euvl_handle h, size_t bytes) {
+  if (bytes <= h->aux_cap) return h->aux;
+  if (h->aux) {
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return nullptr;
+    (void)hipFree(h->aux);
+    h->aux = nullptr;
+    h->aux_cap = 0;
+  }
+  if (hipMalloc(&h->aux, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  h->aux_cap = bytes;
+  return h->aux;
+}
+
 void* ws_take(euvl_handle h, size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
   if (h->ws_top + bytes > h->ws_cap) return nullptr;
@@ -67,6 +83,7 @@ extern "C" int euvl_destroy(euvl_handle h) {
   (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   if (h->ws) (void)hipFree(h->ws);
+  if (h->aux) (void)hipFree(h->aux);
   if (h->d_red) (void)hipFree(h->d_red);
   if (h->h_pinned) (void)hipHostFree(h->h_pinned);
   delete h;

@@ -83,17 +83,29 @@ __global__ __launch_bounds__(256) void zgemm_mfma_kernel(
       for (int t = 0; t < 2; ++t) fa[t] = As[ks + lk][wm * 32 + t * 16 + li];
 #pragma unroll
       for (int u = 0; u < 2; ++u) fb[u] = Bs[ks + lk][wn * 32 + u * 16 + li];
+      // four passes over the 2x2 tiles so that successive updates of one
+      // accumulator are 7 independent MFMAs apart
+      double nai[2] = {-fa[0].y, -fa[1].y};
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const double nai = -fa[t].y;
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
+        for (int u = 0; u < 2; ++u)
           acc_re[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].x, fb[u].x, acc_re[t][u], 0, 0, 0);
-          acc_re[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, fb[u].y, acc_re[t][u], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
           acc_im[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].x, fb[u].y, acc_im[t][u], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          acc_re[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[t], fb[u].y, acc_re[t][u], 0, 0, 0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
           acc_im[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].y, fb[u].x, acc_im[t][u], 0, 0, 0);
-        }
-      }
     }
   }
 

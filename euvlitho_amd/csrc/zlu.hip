@@ -8,7 +8,7 @@
 namespace {
 
 constexpr int NB = 32;
-constexpr int PT = 256;  // panel threads
+constexpr int PT = 1024;  // panel threads
 
 __device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 
@@ -16,65 +16,97 @@ __global__ void zero_int_kernel(int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// Factor columns [kb, kb+nbc) of A[b] (rows kb..n-1) in place; writes ipiv.
-__global__ __launch_bounds__(PT) void getrf_panel_kernel(int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
-                                                         int kb, int nbc, int* info) {
+// Copy the panel A[kb:n, kb:kb+nbc] of every batch entry to Pt[b][q][r]
+// (column q contiguous over the m = n-kb panel rows) and back.
+__global__ void panel_in_kernel(int n, const double2* A, int64_t lda, int64_t sa, int kb, int nbc, double2* Pt) {
+  const int b = blockIdx.y;
+  const int m = n - kb;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const double2* row = A + (int64_t)b * sa + (int64_t)(kb + r) * lda + kb;
+  double2* p = Pt + (int64_t)b * NB * n;
+  for (int q = 0; q < nbc; ++q) p[(int64_t)q * n + r] = row[q];
+}
+
+__global__ void panel_out_kernel(int n, double2* A, int64_t lda, int64_t sa, int kb, int nbc, const double2* Pt) {
+  const int b = blockIdx.y;
+  const int m = n - kb;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  double2* row = A + (int64_t)b * sa + (int64_t)(kb + r) * lda + kb;
+  const double2* p = Pt + (int64_t)b * NB * n;
+  for (int q = 0; q < nbc; ++q) row[q] = p[(int64_t)q * n + r];
+}
+
+// Factor the transposed panel Pt[b] (nbc columns of m = n-kb rows, column
+// stride n) with partial pivoting; one workgroup per batch entry.  Column
+// accesses are contiguous across threads.  Writes ipiv[kb + j] (absolute rows).
+__global__ __launch_bounds__(PT) void getrf_panel_kernel(int n, double2* Pt, int* ipiv, int kb, int nbc, int* info) {
   __shared__ double s_val[PT / 64];
   __shared__ int s_idx[PT / 64];
   __shared__ int s_piv;
+  __shared__ double2 s_prow[NB];
   const int b = blockIdx.x;
-  A += (int64_t)b * sa;
+  const int m = n - kb;
+  double2* P = Pt + (int64_t)b * NB * n;
   ipiv += (int64_t)b * n;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
 
   for (int j = 0; j < nbc; ++j) {
-    const int c = kb + j;
+    double2* col = P + (int64_t)j * n;
     // pivot search: first row of largest |re|+|im| (LAPACK izamax rule)
     double best = -1.0;
-    int bidx = n;
-    for (int r = c + tid; r < n; r += PT) {
-      double v = cabs1(A[(int64_t)r * lda + c]);
+    int bidx = m;
+    for (int r = j + tid; r < m; r += PT) {
+      const double v = cabs1(col[r]);
       if (v > best) { best = v; bidx = r; }
     }
     for (int off = 32; off > 0; off >>= 1) {
-      double ov = __shfl_xor(best, off);
-      int oi = __shfl_xor(bidx, off);
+      const double ov = __shfl_xor(best, off);
+      const int oi = __shfl_xor(bidx, off);
       if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
     }
     if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
     __syncthreads();
-    if (tid == 0) {
-      double bv = s_val[0];
-      int bi = s_idx[0];
-      for (int w = 1; w < PT / 64; ++w)
-        if (s_val[w] > bv || (s_val[w] == bv && s_idx[w] < bi)) { bv = s_val[w]; bi = s_idx[w]; }
-      if (bi >= n) bi = c;
-      s_piv = bi;
-      ipiv[c] = bi;
-      if (bv == 0.0 && info[b] == 0) info[b] = c + 1;
+    if (tid < 64) {
+      double bv = (tid < PT / 64) ? s_val[tid] : -2.0;
+      int bi = (tid < PT / 64) ? s_idx[tid] : m;
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (tid == 0) {
+        if (bi >= m) bi = j;
+        s_piv = bi;
+        ipiv[kb + j] = kb + bi;
+        if (bv == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+      }
     }
     __syncthreads();
     const int piv = s_piv;
-    if (piv != c) {
-      for (int q = kb + tid; q < kb + nbc; q += PT) {
-        double2 t = A[(int64_t)c * lda + q];
-        A[(int64_t)c * lda + q] = A[(int64_t)piv * lda + q];
-        A[(int64_t)piv * lda + q] = t;
-      }
+    // swap rows j and piv across the panel columns and stage the pivot row
+    if (tid < nbc) {
+      double2* cq = P + (int64_t)tid * n;
+      const double2 t = cq[j];
+      const double2 u = cq[piv];
+      cq[j] = u;
+      if (piv != j) cq[piv] = t;
+      s_prow[tid] = u;
     }
     __syncthreads();
-    const double2 d = A[(int64_t)c * lda + c];
+    const double2 d = s_prow[j];
     const bool nz = (d.x != 0.0 || d.y != 0.0);
     const double2 dinv = nz ? cdiv(make_double2(1.0, 0.0), d) : make_double2(0.0, 0.0);
-    for (int r = c + 1 + tid; r < n; r += PT) {
-      double2* row = A + (int64_t)r * lda;
-      double2 l = nz ? cmul(row[c], dinv) : row[c];
-      row[c] = l;
-      if (nz) {
-        const double2* prow = A + (int64_t)c * lda;
-        for (int q = c + 1; q < kb + nbc; ++q) row[q] = csub(row[q], cmul(l, prow[q]));
-      }
+    for (int r = j + 1 + tid; r < m; r += PT) {
+      const double2 l = nz ? cmul(col[r], dinv) : col[r];
+      col[r] = l;
+      if (nz)
+        for (int q = j + 1; q < nbc; ++q) {
+          double2* cq = P + (int64_t)q * n;
+          cq[r] = csub(cq[r], cmul(l, s_prow[q]));
+        }
     }
     __syncthreads();
   }
@@ -151,12 +183,23 @@ namespace euvl {
 
 int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info) {
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  double2* Pt = (double2*)aux_reserve(h, sizeof(double2) * (size_t)batch * NB * n);
+  if (!Pt) {
+    set_error("zgetrf: panel scratch allocation failed");
+    return EUVL_ERR_NOMEM;
+  }
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, h->stream, d_info, batch);
   EUVL_LAUNCH_CHECK();
   for (int kb = 0; kb < n; kb += NB) {
     const int nbc = (n - kb < NB) ? n - kb : NB;
-    hipLaunchKernelGGL(getrf_panel_kernel, dim3(batch), dim3(PT), 0, h->stream, n, A, lda, sa, ipiv, kb, nbc,
-                       d_info);
+    const int m = n - kb;
+    hipLaunchKernelGGL(panel_in_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb, nbc,
+                       Pt);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(getrf_panel_kernel, dim3(batch), dim3(PT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(panel_out_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb,
+                       nbc, Pt);
     EUVL_LAUNCH_CHECK();
     hipLaunchKernelGGL(laswp_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, ipiv,
                        (int64_t)n, kb, kb + nbc, kb, kb + nbc);
