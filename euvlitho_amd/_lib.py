@@ -41,8 +41,7 @@ _SIGS = {
     "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
     "euvl_zgetrs_batched": [c_void_p, c_int, c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
                             c_int64, c_int],
-    "euvl_zsqrtm_batched": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, P_int],
-    "euvl_zexpm_pair_batched": [c_void_p, c_int, c_void_p, c_double, P_double, c_void_p, c_void_p, c_int, P_int],
+    "euvl_zcos_sinc_batched": [c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_int, P_int],
     "euvl_mask_spectrum": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
     "euvl_multilayer": [c_void_p, c_char, ctypes.POINTER(EmParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "euvl_amp_s": [c_void_p, c_char, ctypes.POINTER(EmParams), P_int, P_int, c_int, c_int, c_int, c_void_p, c_int,

@@ -4,16 +4,17 @@
 //
 // Math (all matrices N x N over the order set, per incidence offset):
 //   D  = k^2 eps^ - i eta^ diag(kt) - diag(kxy2)          (header.h:806-821)
-//   S  = D^(1/2) (principal), E = exp(i d S)               (al1, gamma of header.h:918-921, 1088)
+//   S  = D^(1/2), E = exp(i d S)                           (V, al1, gamma of header.h:834-921, 1088)
 //   M  = i (k I - diag(kt/k) Sigma), Sigma = sigma^ diag(kt) (header.h:979-1052)
-//   X  = M^-1 diag(b2), Y = S^-1 diag(alru)                 (Cj, al2/al1 Cjp of header.h:1053-1144)
-//   W1 = X diag(uu+ub), W2 = Y diag(uu-ub)
+//   W1 = M^-1 diag(b2 (uu+ub)), W2 = S^-1 diag(alru (uu-ub)) (Cj, al2/al1 Cjp of header.h:1053-1144)
 //   P  = 1/2 E^-1 (W1+W2) = V U1U,  Q = 1/2 E (W1-W2) = V U1B
 //   R  = S (P-Q),  U0 = diag(klm)(P+Q) + R                  (header.h:1355-1370)
 //   FG = diag(1/klm) R U0^-1                                (header.h:1371-1391)
 //   Ax(:, inc) = -2 kz FG(:, idx(inc)) + e_idx(inc)         (header.h:1394-1465)
-// Every use of (V, al1) in the reference is a function of D, so this route
-// computes the same amplitudes without the eigendecomposition.
+// With C = cos(d S), F = sin(d S) S^-1, G2 = D F (even in S: power series in D):
+//   P - Q = C W2 - i S F W1,  P + Q = C W1 - i F diag(alru (uu-ub))
+//   R = C diag(alru (uu-ub)) - i G2 W1,  U0 = diag(klm)(C W1 - i F diag(alru (uu-ub))) + R
+// so neither the eigenvectors, nor S, nor a square-root branch is needed.
 #include <cmath>
 #include <complex>
 #include <cstdlib>
@@ -168,38 +169,38 @@ __global__ void toeplitz_kernel(int n, int batch, char polar, double k, double d
   }
 }
 
-// Wp = 1/2 e^{-i th}(W1 + W2), Wm = 1/2 e^{i th}(W1 - W2) with
-// W1 = F diag(b2/(i k)) diag(uu+ub), W2 = Sinv diag(alru) diag(uu-ub); in place over F, Sinv.
-__global__ void wbuild_kernel(int n, int batch, double k, double2* F, double2* Sinv, ModeVecs v, const double2* sqc,
-                              double dabs) {
+// W1 = Minv diag(b2 (uu+ub) / (i k)), in place over Minv (= (I - G)^-1)
+__global__ void w1_kernel(int n, int batch, double k, double2* Minv, ModeVecs v) {
   const int64_t per = (int64_t)n * n;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / per);
     const int j = (int)((e - (int64_t)b * per) % n);
     const int64_t vj = (int64_t)b * n + j;
-    const double2 uu = v.uu[vj], ub = v.ub[vj];
-    const double2 x = cmul(cmul(F[e], cdiv(v.b2[vj], make_double2(0.0, k))), cadd(uu, ub));
-    const double2 y = cmul(cmul(Sinv[e], v.alru[vj]), csub(uu, ub));
-    const double th = dabs * sqc[b].x;
-    double s, c;
-    sincos(th, &s, &c);
-    const double2 em = make_double2(0.5 * c, -0.5 * s), ep = make_double2(0.5 * c, 0.5 * s);
-    F[e] = cmul(em, cadd(x, y));
-    Sinv[e] = cmul(ep, csub(x, y));
+    const double2 s = cmul(cdiv(v.b2[vj], make_double2(0.0, k)), cadd(v.uu[vj], v.ub[vj]));
+    Minv[e] = cmul(Minv[e], s);
   }
 }
 
-// P <- P - Q ; Q <- diag(klm) (P + Q)
-__global__ void pq_kernel(int n, int batch, double2* P, double2* Q, const double2* klm) {
+// With L2 = diag(alru (uu-ub)):  T2 <- R = C L2 - i T2,
+// T1 <- U0 = diag(klm) (T1 - i F L2) + R   (T1 = C W1, T2 = G2 W1 on entry)
+__global__ void u0r_kernel(int n, int batch, const double2* C, const double2* F, double2* T1, double2* T2,
+                           ModeVecs v) {
   const int64_t per = (int64_t)n * n;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / per);
-    const int i = (int)((e - (int64_t)b * per) / n);
-    const double2 p = P[e], q = Q[e];
-    P[e] = csub(p, q);
-    Q[e] = cmul(klm[(int64_t)b * n + i], cadd(p, q));
+    const int64_t r = e - (int64_t)b * per;
+    const int i = (int)(r / n), j = (int)(r % n);
+    const int64_t vj = (int64_t)b * n + j;
+    const double2 l2 = cmul(v.alru[vj], csub(v.uu[vj], v.ub[vj]));
+    const double2 t2 = T2[e];
+    const double2 R = csub(cmul(C[e], l2), make_double2(-t2.y, t2.x));  // C l2 - i t2
+    const double2 fl = cmul(F[e], l2);
+    const double2 t1 = T1[e];
+    const double2 p = make_double2(t1.x + fl.y, t1.y - fl.x);  // t1 - i fl
+    T2[e] = R;
+    T1[e] = cadd(cmul(v.klm[(int64_t)b * n + i], p), R);
   }
 }
 
@@ -406,7 +407,7 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   const size_t mat = sizeof(double2) * (size_t)N * N * B;
   const size_t vec = sizeof(double2) * (size_t)N * B;
   const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * B;
-  const size_t need = up.host.size() + 16 * mat + 8 * vec + 2 * rhs + sizeof(int) * (size_t)N * B + 64 * 1024;
+  const size_t need = up.host.size() + 13 * mat + 8 * vec + 2 * rhs + sizeof(int) * (size_t)N * B + 64 * 1024;
   EUVL_TRY(ws_reserve(h, need));
   char* blk = (char*)ws_take(h, up.host.size());
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
@@ -420,13 +421,12 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   v.ub = (double2*)ws_take(h, vec);
   v.alru = (double2*)ws_take(h, vec);
   v.b2 = (double2*)ws_take(h, vec);
-  double2* sqc = (double2*)ws_take(h, vec);  // per-batch sqrt(c) (first B entries)
-  double2* Mx[6];
-  for (int i = 0; i < 6; ++i) Mx[i] = (double2*)ws_take(h, mat);
+  double2* Mx[7];
+  for (int i = 0; i < 7; ++i) Mx[i] = (double2*)ws_take(h, mat);
   double2* Brhs = (double2*)ws_take(h, rhs);
   double2* FGc = (double2*)ws_take(h, rhs);
   int* ipiv = (int*)ws_take(h, sizeof(int) * (size_t)N * B);
-  for (int i = 0; i < 6; ++i)
+  for (int i = 0; i < 7; ++i)
     if (!Mx[i] || !Brhs || !FGc || !ipiv) {
       set_error("euvl_amp_s: workspace carve failed");
       return EUVL_ERR_NOMEM;
@@ -447,88 +447,51 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
                      make_double2(eai.real(), eai.imag()), make_double2(lea.real(), lea.imag()), v, D, G);
   EUVL_LAUNCH_CHECK();
 
-  // S, S^-1 (uses 3 scratch matrices from the workspace)
-  double2* S = Mx[2];
-  double2* Sinv = Mx[3];
-  int ns_it = 0;
-  EUVL_TRY(sqrtm_ns(h, N, B, D, S, Sinv, 1e-8, 30, &ns_it, (double*)sqc));
+  // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (6 scratch matrices from the workspace)
+  double2* C = Mx[2];
+  double2* Fs = Mx[3];
+  double2* G2c = Mx[4];
+  int ndbl = 0;
+  EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl));
 
-  // F = (I+G)(I+G^2)(I+G^4)(I+G^8) ~ (I - G)^-1 ; ||G|| is ~0.1 for these stacks.
-  double2* G2 = Mx[0];  // D no longer needed
-  double2* F = Mx[4];
-  double2* Fn = Mx[5];
+  // Minv = (I+G)(I+G^2)(I+G^4)... ~ (I - G)^-1 with M = i k (I - G); ||G|| is ~0.1 for these stacks.
+  double2* Gp = Mx[0];  // D no longer needed
+  double2* Fm = Mx[5];
+  double2* Fn = Mx[6];
   int nfac = 1;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, G2, N, sm, ZERO, B));           // G^2
-  EUVL_TRY(ew_lincomb(h, N, B, F, ONE, G, ONE, G2, ZERO, nullptr, ONE));                    // I + G + G^2
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G2, N, sm, ONE, F, N, sm, ZERO, B));            // + G^3
-  // F = prod_j (I + G^(2^j)) = sum_{i < 2^(j+1)} G^i; stop once the remainder
-  // bound ||G^(2^(j+1))||_1 <= ||G^(2^j)||_1^2 is below 1e-16
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, Gp, N, sm, ZERO, B));           // G^2
+  EUVL_TRY(ew_lincomb(h, N, B, Fm, ONE, G, ONE, Gp, ZERO, nullptr, ONE));                   // I + G + G^2
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, Gp, N, sm, ONE, Fm, N, sm, ZERO, B));           // + G^3
+  // stop once the remainder bound ||G^(2^(j+1))|| <= ||G^(2^j)||^2 is below 1e-16
   for (;;) {
     std::vector<double> nr(B);
-    EUVL_TRY(reduce_to_host(h, 1, N, B, G2, nr.data()));
+    EUVL_TRY(reduce_to_host(h, 1, N, B, Gp, nr.data()));
     double worst = 0.0;
     for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
     if (worst * worst < 1e-16) break;
     if (!(worst < 0.5) || nfac >= 8) {
-      set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)||_1 = %.3e)", nfac, worst);
+      set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)|| = %.3e)", nfac, worst);
       return EUVL_ERR_NOT_CONVERGED;
     }
-    // G <- G2^2 ; F <- F (I + G)
-    EUVL_TRY(zgemm(h, N, N, N, ONE, G2, N, sm, G2, N, sm, ZERO, G, N, sm, ZERO, B));
-    std::swap(G, G2);  // G2 = new power
-    EUVL_TRY(ew_copy(h, sm * B, Fn, F));
-    EUVL_TRY(zgemm(h, N, N, N, ONE, F, N, sm, G2, N, sm, ONE, Fn, N, sm, ZERO, B));
-    std::swap(F, Fn);
+    EUVL_TRY(zgemm(h, N, N, N, ONE, Gp, N, sm, Gp, N, sm, ZERO, G, N, sm, ZERO, B));        // next power
+    std::swap(G, Gp);
+    EUVL_TRY(ew_copy(h, sm * B, Fn, Fm));
+    EUVL_TRY(zgemm(h, N, N, N, ONE, Fm, N, sm, Gp, N, sm, ONE, Fn, N, sm, ZERO, B));        // F (I + G^(2^j))
+    std::swap(Fm, Fn);
     ++nfac;
   }
-  // Wp (in F), Wm (in Sinv)
-  hipLaunchKernelGGL(wbuild_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, F, Sinv, v, sqc, p->dabs);
+  // W1 = M^-1 diag(b2 (uu+ub)) in place over Fm
+  hipLaunchKernelGGL(w1_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, Fm, v);
   EUVL_LAUNCH_CHECK();
-  double2* Wp = F;
-  double2* Wm = Sinv;
-  // free now: G, G2, Fn.  A = i d (S - sqrt(c) I) into G.
-  double2* A = (G == Mx[0] || G == Mx[1]) ? G : Mx[1];
-  {
-    std::vector<double2> hs(B);
-    EUVL_HIP_CHECK(hipMemcpyAsync(hs.data(), sqc, sizeof(double2) * B, hipMemcpyDeviceToHost, h->stream));
-    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-    std::vector<double2> minus(B);
-    for (int b = 0; b < B; ++b) minus[b] = make_double2(-hs[b].x, 0.0);
-    // A = i d S - i d sqrt(c) I : per-batch shift differs, so form (S - sqrt(c) I) per batch entry
-    for (int b = 0; b < B; ++b)
-      EUVL_TRY(ew_lincomb(h, N, 1, A + b * sm, make_double2(0.0, p->dabs), S + b * sm, ZERO, nullptr, ZERO, nullptr,
-                          make_double2(0.0, -p->dabs * hs[b].x)));
-  }
-  double2* E = (Fn == Mx[4] || Fn == Mx[5]) ? Fn : Mx[4];
-  double2* Einv = (A == Mx[0]) ? Mx[1] : Mx[0];
-  // E, Einv must not alias S, Wp, Wm, A
-  {
-    double2* live[4] = {S, Wp, Wm, A};
-    double2* cand[6] = {Mx[0], Mx[1], Mx[2], Mx[3], Mx[4], Mx[5]};
-    double2* pick[2] = {nullptr, nullptr};
-    int np = 0;
-    for (double2* c : cand) {
-      bool used = false;
-      for (double2* l : live) used |= (c == l);
-      if (!used && np < 2) pick[np++] = c;
-    }
-    E = pick[0];
-    Einv = pick[1];
-  }
-  int sq = 0;
-  EUVL_TRY(expm_pair(h, N, B, A, E, Einv, &sq));
-  // P = Einv Wp (into A), Q = E Wm (into Wp's old slot after use? use two free buffers: A and E/Einv reuse)
-  double2* Pm = A;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, Einv, N, sm, Wp, N, sm, ZERO, Pm, N, sm, ZERO, B));
-  double2* Qm = Einv;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, E, N, sm, Wm, N, sm, ZERO, Qm, N, sm, ZERO, B));
-  hipLaunchKernelGGL(pq_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, Pm, Qm, v.klm);
+  // T1 = C W1 (into G), T2 = G2 W1 (into Gp)
+  double2* T1 = G;
+  double2* T2 = Gp;
+  EUVL_TRY(zgemm(h, N, N, N, ONE, C, N, sm, Fm, N, sm, ZERO, T1, N, sm, ZERO, B));
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G2c, N, sm, Fm, N, sm, ZERO, T2, N, sm, ZERO, B));
+  hipLaunchKernelGGL(u0r_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, C, Fs, T1, T2, v);
   EUVL_LAUNCH_CHECK();
-  // R = S (P - Q) into E ; U0 = diag(klm)(P+Q) + R into Qm
-  double2* R = E;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, S, N, sm, Pm, N, sm, ZERO, R, N, sm, ZERO, B));
-  EUVL_TRY(ew_lincomb(h, N, B, Qm, ONE, Qm, ONE, R, ZERO, nullptr, ZERO));
-  double2* U0 = Qm;
+  double2* R = T2;
+  double2* U0 = T1;
   // LU of U0, solve for the incidence columns
   int* d_info = (int*)h->d_red;
   EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, d_info));
@@ -558,8 +521,8 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   }
   if (h_info) {
     for (int b = 0; b < B; ++b) {
-      h_info[4 * b + 0] = ns_it;
-      h_info[4 * b + 1] = sq;
+      h_info[4 * b + 0] = 0;
+      h_info[4 * b + 1] = ndbl;
       h_info[4 * b + 2] = nfac + 1;
       h_info[4 * b + 3] = ninc;
     }

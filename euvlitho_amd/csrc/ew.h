@@ -21,9 +21,9 @@ int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X);
 int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red);
 int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out);
 
-// matrix functions (matfun.hip); buffers are batch x n x n contiguous
-int sqrtm_ns(euvl_handle h, int n, int batch, const double2* D, double2* S, double2* Sinv, double tol,
-             int max_iter, int* iters, double* d_c /* out: per-batch c (device) */);
-int expm_pair(euvl_handle h, int n, int batch, const double2* A, double2* E, double2* Einv, int* squarings);
+// matrix functions (matfun.hip); buffers are batch x n x n contiguous.
+// C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (G2 may be null); takes 6 workspace matrices.
+int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
+             int* doublings);
 
 }  // namespace euvl

@@ -82,24 +82,17 @@ int euvl_zgetrf_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t 
 int euvl_zgetrs_batched(euvl_handle h, int n, int nrhs, const double* d_LU, int64_t lda, int64_t sa,
                         const int* d_ipiv, double* d_B, int64_t ldb, int64_t sb, int batch);
 
-/* Principal square root S = D^(1/2) and S^-1 of D[b] (n x n) by the coupled
- * Newton-Schulz iteration started from D/c (c = mean of Re diag D).  Stands in
- * for the eigen-pairs + al1 = sqrt(lambda) of absorberS0
- * (cnn/include/header.h:834-921): every later use of (V, al1) in the
- * reference is a matrix function of D, so S and S^-1 carry the same content.
- * d_S, d_Sinv: n x n per batch entry, contiguous (stride n*n).
- * *h_iters receives the iteration count; EUVL_ERR_NOT_CONVERGED if the
- * residual max|T - I| is not below tol after max_iter iterations. */
-int euvl_zsqrtm_batched(euvl_handle h, int n, const double* d_D, double* d_S, double* d_Sinv,
-                        int batch, double tol, int max_iter, int* h_iters);
-
-/* E = exp(i*d*S), Einv = exp(-i*d*S) for each batch entry (d = thickness[b]),
- * by Taylor degree 18 with scaling and squaring, after factoring out the
- * scalar exp(i d sqrt(c)).  Stands in for gamma = exp(i al1 dabs) of
- * absorberS0 (cnn/include/header.h:1085-1140) in matrix form. */
-int euvl_zexpm_pair_batched(euvl_handle h, int n, const double* d_S, double shift_re,
-                            const double* h_thickness, double* d_E, double* d_Einv,
-                            int batch, int* h_squarings);
+/* Layer propagation functions of the absorber operator D[b] (n x n, batch
+ * entries contiguous with stride n*n):
+ *   C = cos(d sqrt(D)),  F = sin(d sqrt(D)) / sqrt(D),  G2 = D F
+ * (d = thickness; d_G2 may be null).  Even in sqrt(D), so they are power
+ * series in D: evaluated by a degree-9 Taylor series in (d/2^s)^2 D and s
+ * angle doublings (*h_doublings receives s).  Replaces the eigen-pairs,
+ * al1 = sqrt(lambda) and gamma = exp(i al1 dabs) of absorberS0
+ * (cnn/include/header.h:834-921, 1085-1140): every use of (V, al1, gamma) in
+ * ampS reduces to these three matrices, with no square-root branch choice. */
+int euvl_zcos_sinc_batched(euvl_handle h, int n, const double* d_D, double thickness, double* d_C,
+                           double* d_F, double* d_G2, int batch, int* h_doublings);
 
 /* ---------------------------------------------------------------------------
  * Rigorous EM solve.

@@ -99,36 +99,28 @@ def test_getrf_singular_reports(lib, handle):
     assert lib.euvl_zgetrf_batched(handle, 8, P(dA), 8, 64, P(piv), 1) == -3
 
 
-def test_sqrtm_expm(lib, handle):
+@pytest.mark.parametrize("n,batch", [(200, 2), (37, 3)])
+def test_cos_sinc(lib, handle, n, batch):
+    """C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F against scipy's
+    principal sqrtm + cosm/sinm on a non-normal D with a lossy spectrum."""
     import scipy.linalg as sla
-    say("sqrtm/expm n=200 batch=2")
-    rng = np.random.default_rng(5)
-    n, batch, c = 200, 2, 0.19
-    D = np.stack([c * (np.eye(n) + 0.25 * crand(rng, n, n) / math.sqrt(2 * n)) for _ in range(batch)])
+    say(f"cos/sinc n={n} batch={batch}")
+    rng = np.random.default_rng(5 + n)
+    c, d = 0.19, 60.0
+    D = np.stack([c * (np.eye(n) * (1 + 0.05j) + 0.25 * crand(rng, n, n) / math.sqrt(2 * n)) for _ in range(batch)])
     dD = dev(D)
-    S = torch.empty_like(dD)
-    Si = torch.empty_like(dD)
-    it = ctypes.c_int()
-    assert lib.euvl_zsqrtm_batched(handle, n, P(dD), P(S), P(Si), batch, 1e-8, 30, ctypes.byref(it)) == 0, err(lib)
-    torch.cuda.synchronize()
-    Sh, Sih = S.cpu().numpy(), Si.cpu().numpy()
-    for b in range(batch):
-        ref = sla.sqrtm(D[b])
-        assert np.abs(Sh[b] - ref).max() < 1e-12 * np.abs(ref).max()
-        assert np.abs(Sih[b] @ Sh[b] - np.eye(n)).max() < 1e-12
-    E = torch.empty_like(dD)
-    Ei = torch.empty_like(dD)
-    th = (ctypes.c_double * batch)(60.0, 45.0)
+    C, F, G2 = torch.empty_like(dD), torch.empty_like(dD), torch.empty_like(dD)
     sq = ctypes.c_int()
-    assert lib.euvl_zexpm_pair_batched(handle, n, P(S), math.sqrt(c), th, P(E), P(Ei), batch, ctypes.byref(sq)) == 0, \
-        err(lib)
+    assert lib.euvl_zcos_sinc_batched(handle, n, P(dD), d, P(C), P(F), P(G2), batch, ctypes.byref(sq)) == 0, err(lib)
     torch.cuda.synchronize()
-    for b, d_ in enumerate([60.0, 45.0]):
-        A = 1j * d_ * (Sh[b] - math.sqrt(c) * np.eye(n))
-        ref = sla.expm(A)
-        refi = sla.expm(-A)
-        assert np.abs(E.cpu().numpy()[b] - ref).max() < 1e-12 * np.abs(ref).max()
-        assert np.abs(Ei.cpu().numpy()[b] - refi).max() < 1e-12 * np.abs(refi).max()
+    say(f"  doublings {sq.value}")
+    for b in range(batch):
+        S = sla.sqrtm(D[b])
+        rc = sla.cosm(d * S)
+        rf = sla.sinm(d * S) @ np.linalg.inv(S)
+        assert np.abs(C.cpu().numpy()[b] - rc).max() < 1e-11 * np.abs(rc).max()
+        assert np.abs(F.cpu().numpy()[b] - rf).max() < 1e-11 * np.abs(rf).max()
+        assert np.abs(G2.cpu().numpy()[b] - D[b] @ rf).max() < 1e-11 * np.abs(D[b] @ rf).max()
 
 
 # ---------------------------------------------------------------- EM pieces
