@@ -112,6 +112,206 @@ __global__ __launch_bounds__(PT) void getrf_panel_kernel(int n, double2* Pt, int
   }
 }
 
+// Register-resident panel factorisation for panels of at most RPT * RT rows.
+// The nbc <= 32 columns are processed in leaves of LW: a leaf lives in
+// registers (row r of the panel is held by thread r % 512, slot r / 512), is
+// factored column by column with partial pivoting (pivot rows exchanged
+// through LDS, the swap mirrored on the panel's other columns in global
+// memory), and then the columns to its right are updated in one pass:
+// U12 = L11^-1 A12 on the 8 pivot rows and A22 -= L21 U12 on the rest.
+constexpr int RT = 512;  // threads of the register-resident panel kernel
+constexpr int RPT = 4;   // rows per thread (panel height <= 2048)
+constexpr int LW = 4;    // leaf width (RPT x LW complex per thread stay in registers)
+
+__global__ __launch_bounds__(RT) void getrf_panel_reg_kernel(int n, double2* Pt, int* ipiv, int kb, int nbc,
+                                                             int* info) {
+  __shared__ double s_val[RT / 64];
+  __shared__ int s_idx[RT / 64];
+  __shared__ int s_piv;
+  __shared__ double2 s_row[2][LW];
+  __shared__ double2 s_l11[LW][LW];
+  __shared__ double2 s_u[LW][NB];
+  __shared__ int s_pivs[LW];
+  const int b = blockIdx.x;
+  const int m = n - kb;
+  double2* P = Pt + (int64_t)b * NB * n;
+  ipiv += (int64_t)b * n;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+
+  for (int c0 = 0; c0 < nbc; c0 += LW) {
+    const int lw = (nbc - c0 < LW) ? nbc - c0 : LW;
+    double2 x[RPT][LW];
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int r = tid + s * RT;
+#pragma unroll
+      for (int q = 0; q < LW; ++q)
+        x[s][q] = (r < m && q < lw) ? P[(int64_t)(c0 + q) * n + r] : make_double2(0.0, 0.0);
+    }
+    for (int jj = 0; jj < lw; ++jj) {
+      const int j = c0 + jj;
+      // pivot search over rows >= j (first of the largest |re|+|im|)
+      double best = -1.0;
+      int bidx = m;
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        const int r = tid + s * RT;
+        double v = -1.0;
+#pragma unroll
+        for (int q = 0; q < LW; ++q)
+          if (q == jj) v = cabs1(x[s][q]);
+        if (r >= j && r < m && v > best) { best = v; bidx = r; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bidx, off);
+        if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
+      }
+      if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
+      __syncthreads();
+      if (tid < 64) {
+        double bv = (tid < RT / 64) ? s_val[tid] : -2.0;
+        int bi = (tid < RT / 64) ? s_idx[tid] : m;
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(bv, off);
+          const int oi = __shfl_xor(bi, off);
+          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        if (tid == 0) {
+          if (bi >= m) bi = j;
+          s_piv = bi;
+          ipiv[kb + j] = kb + bi;
+          if (bv == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+        }
+      }
+      __syncthreads();
+      const int piv = s_piv;
+      // exchange leaf rows j and piv through LDS
+      const int oj = j % RT, sj = j / RT, op = piv % RT, sp = piv / RT;
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        if (tid == oj && s == sj)
+#pragma unroll
+          for (int q = 0; q < LW; ++q) s_row[0][q] = x[s][q];
+        if (tid == op && s == sp)
+#pragma unroll
+          for (int q = 0; q < LW; ++q) s_row[1][q] = x[s][q];
+      }
+      if (tid == 0) s_pivs[jj] = piv;
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        if (tid == oj && s == sj)
+#pragma unroll
+          for (int q = 0; q < LW; ++q) x[s][q] = s_row[1][q];
+        if (piv != j && tid == op && s == sp)
+#pragma unroll
+          for (int q = 0; q < LW; ++q) x[s][q] = s_row[0][q];
+      }
+      // pivot row = old row piv (s_row[1]); eliminate below it
+      const double2 d = s_row[1][jj];
+      const bool nz = (d.x != 0.0 || d.y != 0.0);
+      const double2 dinv = nz ? cdiv(make_double2(1.0, 0.0), d) : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        const int r = tid + s * RT;
+        if (r > j && r < m && nz) {
+          double2 l = make_double2(0.0, 0.0);
+#pragma unroll
+          for (int q = 0; q < LW; ++q)
+            if (q == jj) {
+              l = cmul(x[s][q], dinv);
+              x[s][q] = l;
+            }
+#pragma unroll
+          for (int q = 0; q < LW; ++q)
+            if (q > jj && q < lw) x[s][q] = csub(x[s][q], cmul(l, s_row[1][q]));
+        }
+      }
+      __syncthreads();
+    }
+    // write the leaf back and stage L11 (rows c0..c0+lw-1 of the leaf)
+#pragma unroll
+    for (int s = 0; s < RPT; ++s) {
+      const int r = tid + s * RT;
+      if (r < m)
+#pragma unroll
+        for (int q = 0; q < LW; ++q)
+          if (q < lw) P[(int64_t)(c0 + q) * n + r] = x[s][q];
+      if (r >= c0 && r < c0 + lw)
+#pragma unroll
+        for (int q = 0; q < LW; ++q) s_l11[r - c0][q] = x[s][q];
+    }
+    // apply this leaf's interchanges, in order, to the panel's other columns
+    if (tid < nbc && (tid < c0 || tid >= c0 + lw)) {
+      double2* cq = P + (int64_t)tid * n;
+      for (int jj = 0; jj < lw; ++jj) {
+        const int j = c0 + jj, piv = s_pivs[jj];
+        if (piv != j) {
+          const double2 t = cq[j];
+          cq[j] = cq[piv];
+          cq[piv] = t;
+        }
+      }
+    }
+    const int c1 = c0 + lw;
+    const int nr = nbc - c1;  // columns right of the leaf
+    if (nr <= 0) break;
+    __syncthreads();
+    // U12 = L11^-1 A12 on the pivot rows, one thread per right column
+    if (tid < nr) {
+      double2* cq = P + (int64_t)(c1 + tid) * n;
+      double2 u[LW];
+#pragma unroll
+      for (int i = 0; i < LW; ++i) {
+        if (i < lw) {
+          double2 v = cq[c0 + i];
+#pragma unroll
+          for (int p = 0; p < LW; ++p)
+            if (p < i) v = csub(v, cmul(s_l11[i][p], u[p]));
+          u[i] = v;
+          cq[c0 + i] = v;
+          s_u[i][tid] = v;
+        } else {
+          u[i] = make_double2(0.0, 0.0);
+        }
+      }
+    }
+    __syncthreads();
+    // A22 -= L21 U12 for the rows below the leaf's pivot rows; columns in
+    // chunks of 8 so that the thread's RPT x 8 loads are in flight together
+    for (int q0 = 0; q0 < nr; q0 += 8) {
+      double2 v[RPT][8];
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        const int r = tid + s * RT;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+          v[s][qq] = (r >= c1 && r < m && q0 + qq < nr) ? P[(int64_t)(c1 + q0 + qq) * n + r]
+                                                         : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int s = 0; s < RPT; ++s) {
+        const int r = tid + s * RT;
+        if (r >= c1 && r < m) {
+#pragma unroll
+          for (int qq = 0; qq < 8; ++qq) {
+            if (q0 + qq < nr) {
+              double2 t = v[s][qq];
+#pragma unroll
+              for (int p = 0; p < LW; ++p)
+                if (p < lw) t = csub(t, cmul(x[s][p], s_u[p][q0 + qq]));
+              P[(int64_t)(c1 + q0 + qq) * n + r] = t;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Apply the interchanges ipiv[k1..k2) to every column outside [skip0, skip1).
 __global__ void laswp_kernel(int ncols, double2* A, int64_t lda, int64_t sa, const int* ipiv, int64_t spiv,
                              int k1, int k2, int skip0, int skip1) {
@@ -196,7 +396,10 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
     hipLaunchKernelGGL(panel_in_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb, nbc,
                        Pt);
     EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(getrf_panel_kernel, dim3(batch), dim3(PT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
+    if (m <= RT * RPT)
+      hipLaunchKernelGGL(getrf_panel_reg_kernel, dim3(batch), dim3(RT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
+    else
+      hipLaunchKernelGGL(getrf_panel_kernel, dim3(batch), dim3(PT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
     EUVL_LAUNCH_CHECK();
     hipLaunchKernelGGL(panel_out_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb,
                        nbc, Pt);

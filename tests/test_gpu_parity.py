@@ -75,7 +75,7 @@ def test_zgemm(lib, handle, m, n, k, batch):
     assert np.abs(got - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()) * math.sqrt(k)
 
 
-@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2)])
+@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2), (1681, 12, 2), (2100, 5, 1)])
 def test_getrf_getrs(lib, handle, n, nrhs, batch):
     say(f"getrf/getrs n={n} nrhs={nrhs} batch={batch}")
     rng = np.random.default_rng(n)
