@@ -52,6 +52,14 @@ _SIGS = {
     "euvl_m3d_fit": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,
                      P_int, P_double, P_double, c_double, c_double, c_double, c_int, P_int, c_void_p, c_void_p,
                      c_void_p],
+    "euvl_linear_images": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, P_double, P_double, P_double,
+                           c_void_p, c_void_p, c_void_p, c_int, P_int, c_int, P_double, c_double, c_double, c_double,
+                           c_double, c_double, c_int, c_int, c_void_p, c_void_p],
+    "euvl_tcc_kernels": [c_void_p, ctypes.POINTER(EmParams), c_int, P_int, c_int, P_double, c_double, c_double,
+                         c_double, c_double, c_int, c_void_p, c_void_p, P_int],
+    "euvl_socs_image": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, P_double, P_double, P_double,
+                        c_void_p, c_void_p, c_void_p, c_int, P_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                        c_double, c_double, c_double, c_int, c_int, c_void_p],
 }
 
 EXPORTS = tuple(_SIGS) + ("euvl_last_error",)

@@ -219,6 +219,33 @@ class Config:
                 out.append((nsx, nsy, l[ok].astype(np.int32), m[ok].astype(np.int32)))
         return out
 
+    def mesh_source(self, mesh_deg=0.2):
+        """(dsx, dsy) wave-number offsets of the source mesh of linear.py:46-97
+        (eigenfunction.py:6-56): a square mesh of mesh_deg degrees, filtered by
+        the source shape (union of its members)."""
+        k, NA = self.k, self.na
+        kmesh = k * mesh_deg * (PI / 180.0)
+        mem = self.source.members()
+        s1 = max(m[1] for m in mem)
+        l0max = int(k * NA / self.mx * s1 / kmesh) + 1
+        m0max = int(k * NA / self.my * s1 / kmesh) + 1
+        l, m = np.meshgrid(np.arange(-l0max, l0max + 1), np.arange(-m0max, m0max + 1), indexing="ij")
+        skx, sky = (l * kmesh).ravel(), (m * kmesh).ravel()
+        sxo, syo = skx * self.mx, sky * self.my
+        r = np.sqrt(sxo ** 2 + syo ** 2)
+        ok = np.zeros(len(skx), dtype=bool)
+        for (t, so, si, oa) in mem:
+            ring = (k * NA * si <= r) & (r <= k * NA * so)
+            if t == 0:
+                ok |= (sxo ** 2 + syo ** 2) <= (k * NA * so) ** 2
+            elif t == 1:
+                ok |= ring
+            elif t == 2:
+                ok |= ring & (np.abs(syo) <= np.abs(sxo) * np.tan(PI * oa / 180.0 / 2.0))
+            elif t == 3:
+                ok |= ring & (np.abs(sxo) <= np.abs(syo) * np.tan(PI * oa / 180.0 / 2.0))
+        return skx[ok], sky[ok]
+
     def offsets(self):
         """Incidence offsets (sx0, sy0, fx, fy) per division (intensity.cpp:234-238)."""
         nd = self.ndivs

@@ -214,6 +214,68 @@ __global__ void m3d_fit_kernel(M3dSetup a, const double2* fmask, const double2* 
   }
 }
 
+// Pupil fields of the thin-mask and M3D-linear models on the 0.2-degree
+// source mesh (linear.py:130-184 with electro_field.high_na_electro_field and
+// the phase of int_abbe, linear.py:24-28).  Ft: thin mask, Fl: linear model;
+// layout [s][c][l+lpx][m+lpy] as abbe_field_kernel.
+struct LinearSetup {
+  double2 ampvc, ampab, phase0;
+  double co;
+  int lpmx, lpmy;  // window of the indicator spectrum
+};
+
+__global__ void linear_field_kernel(AbbeSetup a, LinearSetup L, const double2* fmask, const double2* a0,
+                                    const double2* ax, const double2* ay, const int* pupil, int ncut,
+                                    const double* src, int n_src, double2* Fl, double2* Ft) {
+  const int64_t total = (int64_t)n_src * ncut;
+  const int nl = 2 * a.lpx + 1, nm = 2 * a.lpy + 1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int s = (int)(e / ncut), i = (int)(e % ncut);
+    const int lp = pupil[2 * i], mp = pupil[2 * i + 1];
+    const double dsx = src[2 * s], dsy = src[2 * s + 1];
+    const double kxn = dsx + 2.0 * PI * lp / a.dx;
+    const double kyn = dsy + 2.0 * PI * mp / a.dy;
+    const double p2 = a.mx * a.mx * kxn * kxn + a.my * a.my * kyn * kyn;
+    const double nak = a.na * a.k;
+    if (!(p2 <= nak * nak) || p2 < (nak * L.co) * (nak * L.co)) continue;  // zero outside the (annular) pupil
+    // thin-mask amplitude: fmask = ampab delta + (ampvc - ampab) m^, times phaseX phaseY / phase0
+    double2 fm = cmul(csub(L.ampvc, L.ampab), fmask[(int64_t)(lp + L.lpmx) * (2 * L.lpmy + 1) + (mp + L.lpmy)]);
+    if (lp == 0 && mp == 0) fm = cadd(fm, L.ampab);
+    const double kxp = 2.0 * PI * lp / a.dx, kyp = 2.0 * PI * mp / a.dy;
+    const double ph = -(a.kx0 * kxp + kxp * kxp / 2.0) / a.k * a.z0 - (a.ky0 * kyp + kyp * kyp / 2.0) / a.k * a.z0;
+    double sp, cp;
+    sincos(ph, &sp, &cp);
+    const double2 famp = cdiv(cmul(fm, make_double2(cp, sp)), L.phase0);
+    const double kx = a.kx0 + dsx;
+    const double inv = 1.0 / sqrt(a.k * a.k - kx * kx);
+    const double ls = dsx / (2.0 * PI / a.dx), ms = dsy / (2.0 * PI / a.dy);
+    const double2 At = cscale(famp, inv);
+    double2 Al = cadd(famp, a0[i]);
+    Al = cadd(Al, cscale(ax[i], ls + lp / 2.0));
+    Al = cadd(Al, cscale(ay[i], ms + mp / 2.0));
+    Al = cscale(Al, inv);
+    double R[3];
+    rotate_col0(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, R);
+    const double phz = ((kxn + a.kx0) * (kxn + a.kx0) + (kyn + a.ky0) * (kyn + a.ky0)) / (2.0 * a.k) * a.z0 +
+                       p2 / (2.0 * a.k) * a.z;
+    double sz, cz;
+    sincos(phz, &sz, &cz);
+    const double2 pz = make_double2(cz, sz);
+    const double2 tt = cmul(make_double2(-a.k * At.y, a.k * At.x), pz);  // i k At e^{i phz}
+    const double2 tl = cmul(make_double2(-a.k * Al.y, a.k * Al.x), pz);
+    const int64_t base = (int64_t)s * 3 * nl * nm + (int64_t)(lp + a.lpx) * nm + (mp + a.lpy);
+    for (int c = 0; c < 3; ++c) {
+      Ft[base + (int64_t)c * nl * nm] = cscale(tt, R[c]);
+      Fl[base + (int64_t)c * nl * nm] = cscale(tl, R[c]);
+    }
+  }
+}
+
+__global__ void zero_d_kernel(int64_t count, double* out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = 0.0;
+}
+
 inline dim3 grid1(int64_t total) {
   int64_t b = (total + 255) / 256;
   if (b > 65536) b = 65536;
@@ -367,6 +429,95 @@ extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double
                      (const double2*)(blk + o_ab), (const int*)(blk + o_pup), ncut, (double2*)d_a0, (double2*)d_ax,
                      (double2*)d_ay);
   EUVL_LAUNCH_CHECK();
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}
+
+extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x,
+                                  int lpmax_y, const double* ampvc, const double* ampab, const double* phase0,
+                                  const double* d_a0, const double* d_ax, const double* d_ay, int ncut,
+                                  const int* h_pupil, int n_src, const double* h_src, double kx0, double ky0,
+                                  double z0, double z, double co, int nx, int ny, double* d_intlinear,
+                                  double* d_intft) {
+  using namespace euvl;
+  if (!h || !p || !d_fmask || !ampvc || !ampab || !phase0 || !d_a0 || !d_ax || !d_ay || ncut <= 0 || !h_pupil ||
+      n_src <= 0 || !h_src || nx <= 0 || ny <= 0 || !d_intlinear || !d_intft) {
+    set_error("euvl_linear_images: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  int lpx = 0, lpy = 0;
+  for (int i = 0; i < ncut; ++i) {
+    lpx = std::max(lpx, std::abs(h_pupil[2 * i]));
+    lpy = std::max(lpy, std::abs(h_pupil[2 * i + 1]));
+  }
+  if (lpx > lpmax_x || lpy > lpmax_y) {
+    set_error("euvl_linear_images: pupil outside the thin-mask window");
+    return EUVL_ERR_ARG;
+  }
+  const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
+  AbbeSetup a{};
+  a.k = p->k;
+  a.dx = p->dx;
+  a.dy = p->dy;
+  a.na = p->na;
+  a.kx0 = kx0;
+  a.ky0 = ky0;
+  a.z0 = z0;
+  a.z = z;
+  a.mx = p->mx;
+  a.my = p->my;
+  a.lpx = lpx;
+  a.lpy = lpy;
+  LinearSetup L;
+  L.ampvc = make_double2(ampvc[0], ampvc[1]);
+  L.ampab = make_double2(ampab[0], ampab[1]);
+  L.phase0 = make_double2(phase0[0], phase0[1]);
+  L.co = co;
+  L.lpmx = lpmax_x;
+  L.lpmy = lpmax_y;
+  std::vector<std::complex<double>> twx(nx), twy(ny);
+  const std::complex<double> zi(0.0, 1.0);
+  for (int i = 0; i < nx; ++i) twx[i] = std::exp(2.0 * PI * zi * double(i) / double(nx));
+  for (int i = 0; i < ny; ++i) twy[i] = std::exp(2.0 * PI * zi * double(i) / double(ny));
+  Upload up;
+  const size_t o_src = up.add(h_src, sizeof(double) * 2 * n_src);
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * ncut);
+  const size_t o_twx = up.add(twx.data(), sizeof(double2) * nx);
+  const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
+  const int64_t per_src = (int64_t)3 * nl * (2 * nm + ny);
+  int chunk = (int)std::max<int64_t>(1, (int64_t)(16 << 20) / per_src);
+  if (chunk > n_src) chunk = n_src;
+  const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
+  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
+  EUVL_TRY(ws_reserve(h, up.host.size() + 2 * fbytes + gbytes + 4096));
+  char* blk = (char*)ws_take(h, up.host.size());
+  double2* Fl = (double2*)ws_take(h, fbytes);
+  double2* Ft = (double2*)ws_take(h, fbytes);
+  double2* G = (double2*)ws_take(h, gbytes);
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, d_intlinear);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, d_intft);
+  EUVL_LAUNCH_CHECK();
+  const double w = 1.0 / n_src;
+  for (int s0 = 0; s0 < n_src; s0 += chunk) {
+    const int ns = std::min(chunk, n_src - s0);
+    EUVL_TRY(fill(h, (int64_t)ns * 3 * nl * nm, Fl, make_double2(0.0, 0.0)));
+    EUVL_TRY(fill(h, (int64_t)ns * 3 * nl * nm, Ft, make_double2(0.0, 0.0)));
+    hipLaunchKernelGGL(linear_field_kernel, grid1((int64_t)ns * ncut), dim3(256), 0, h->stream, a, L,
+                       (const double2*)d_fmask, (const double2*)d_a0, (const double2*)d_ax, (const double2*)d_ay,
+                       (const int*)(blk + o_pup), ncut, (const double*)(blk + o_src) + 2 * s0, ns, Fl, Ft);
+    EUVL_LAUNCH_CHECK();
+    for (int which = 0; which < 2; ++which) {
+      hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl, nm,
+                         lpy, ny, which == 0 ? Fl : Ft, (const double2*)(blk + o_twy), G);
+      EUVL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + 255) / 256, nx), dim3(256), sizeof(double2) * nl, h->stream,
+                         ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), w,
+                         which == 0 ? d_intlinear : d_intft);
+      EUVL_LAUNCH_CHECK();
+    }
+  }
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
 }

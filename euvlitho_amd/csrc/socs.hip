@@ -1,0 +1,615 @@
+// STCC-SOCS imaging: transmission cross coefficients from the source mesh,
+// their Hermitian eigendecomposition (kernels), and the SOCS image.
+//   eigenfunction.py:5-179  -> euvl_tcc_kernels
+//   socs.py:12-154          -> euvl_socs_image
+// TCC_w = sum_c P_c diag(w) P_c^H / S is one MFMA GEMM per weight (the three
+// polarisation components stacked along K); the eigendecomposition is a
+// parallel two-sided Jacobi method (round-robin pairs, n/2 rotations per
+// step applied as one row pass and one column pass).
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ew.h"
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+struct Upload {
+  std::vector<char> host;
+  size_t add(const void* p, size_t bytes) {
+    size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes);
+    if (bytes) memcpy(host.data() + off, p, bytes);
+    return off;
+  }
+};
+
+inline dim3 grid1(int64_t total) {
+  int64_t b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b);
+}
+
+// rotate() first column (intensity.cpp:423-465 / electro_field.py:5-40)
+__device__ void rot_col0(double k, int MX, int MY, double px, double py, double sx0, double sy0, double R[3]) {
+  double s0[3] = {sx0, sy0, -sqrt(k * k - sx0 * sx0 - sy0 * sy0)};
+  double p[3] = {px, py, 0.0};
+  double pp[3] = {MX * px, MY * py, -sqrt(k * k - MX * px * MX * px - MY * py * MY * py)};
+  const double ez[3] = {0.0, 0.0, 1.0};
+  double es[3], esp[3], em[3], emp[3];
+  auto cross = [](const double* a, const double* b, double* c) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = a[2] * b[0] - a[0] * b[2];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  auto norm = [](const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
+  if ((px * px + py * py) > 0.00001) {
+    cross(p, s0, es);
+    double n = norm(es);
+    for (int i = 0; i < 3; ++i) es[i] /= n;
+    cross(pp, ez, esp);
+    n = norm(esp);
+    for (int i = 0; i < 3; ++i) esp[i] /= n;
+    double ps0[3] = {p[0] + s0[0], p[1] + s0[1], p[2] + s0[2]};
+    n = norm(ps0);
+    for (int i = 0; i < 3; ++i) ps0[i] /= n;
+    cross(es, ps0, em);
+    cross(esp, pp, emp);
+    for (int i = 0; i < 3; ++i) emp[i] /= k;
+  } else {
+    cross(ez, s0, es);
+    double n = norm(es);
+    for (int i = 0; i < 3; ++i) es[i] /= n;
+    for (int i = 0; i < 3; ++i) esp[i] = -es[i];
+    cross(es, s0, em);
+    for (int i = 0; i < 3; ++i) em[i] /= k;
+    const double mez[3] = {0.0, 0.0, -1.0};
+    cross(esp, mez, emp);
+  }
+  for (int i = 0; i < 3; ++i) R[i] = esp[i] * es[0] + emp[i] * em[0];
+}
+
+struct TccSetup {
+  double k, dx, dy, na, kx0, ky0, z0, z;
+  int mx, my;
+};
+
+// P[i][c*S + s] = phase R_c (zero outside the pupil), and
+// Q[w][c*S + s][i] = conj(P[i][c*S + s]) wt_w[s] / S for w = (1, sx, sy) * k^2/(k^2-(kx0+sx)^2)
+// (eigenfunction.py:84-111)
+__global__ void tcc_phase_kernel(TccSetup a, const int* pupil, int ncut, const double* src, int S, double2* P,
+                                 double2* Q) {
+  const int64_t total = (int64_t)ncut * S;
+  const int64_t cols = 3 * (int64_t)S;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e / S), s = (int)(e % S);
+    const double sx = src[2 * s], sy = src[2 * s + 1];
+    const double ksx = 2.0 * PI / a.dx * pupil[2 * i] + sx;
+    const double ksy = 2.0 * PI / a.dy * pupil[2 * i + 1] + sy;
+    const double p2 = a.mx * a.mx * ksx * ksx + a.my * a.my * ksy * ksy;
+    double2 v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    if (p2 <= (a.k * a.na) * (a.k * a.na)) {
+      double R[3];
+      rot_col0(a.k, a.mx, a.my, ksx, ksy, a.kx0, a.ky0, R);
+      const double ph = ((ksx + a.kx0) * (ksx + a.kx0) + (ksy + a.ky0) * (ksy + a.ky0)) / 2.0 / a.k * a.z0 +
+                        p2 / 2.0 / a.k * a.z;
+      double sp, cp;
+      sincos(ph, &sp, &cp);
+      for (int c = 0; c < 3; ++c) v[c] = make_double2(cp * R[c], sp * R[c]);
+    }
+    const double w = a.k * a.k / (a.k * a.k - (a.kx0 + sx) * (a.kx0 + sx)) / S;
+    const double wt[3] = {w, sx * w, sy * w};
+    for (int c = 0; c < 3; ++c) {
+      P[(int64_t)i * cols + (int64_t)c * S + s] = v[c];
+      for (int q = 0; q < 3; ++q)
+        Q[(int64_t)q * cols * ncut + ((int64_t)c * S + s) * ncut + i] = make_double2(v[c].x * wt[q], -v[c].y * wt[q]);
+    }
+  }
+}
+
+// Jacobi rotation parameters for the pairs of one round-robin step:
+// block [[a, b], [conj b, d]], b = |b| e^{i phi}: J = diag(1, e^{-i phi}) G,
+// G the real Jacobi rotation of [[a, |b|], [|b|, d]] (c, s with t = tan theta).
+// rot[b][k] = (c, s, cos phi, sin phi)
+__global__ void jacobi_params_kernel(int n, int npair, const int* pairs, const double2* A, double4_t* rot,
+                                     int batch) {
+  const int64_t total = (int64_t)npair * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / npair), k = (int)(e % npair);
+    const int p = pairs[2 * k], q = pairs[2 * k + 1];
+    double4_t r = {1.0, 0.0, 1.0, 0.0};
+    if (p >= 0 && q >= 0) {
+      const double2* M = A + (int64_t)b * n * n;
+      const double a = M[(int64_t)p * n + p].x, d = M[(int64_t)q * n + q].x;
+      const double2 bq = M[(int64_t)p * n + q];
+      const double ab = hypot(bq.x, bq.y);
+      if (ab > 1e-300 && ab > 1e-18 * (fabs(a) + fabs(d))) {
+        const double tau = (d - a) / (2.0 * ab);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t);
+        r = (double4_t){c, t * c, bq.x / ab, bq.y / ab};
+      }
+    }
+    rot[e] = r;
+  }
+}
+
+// rows p, q <- J^H rows: (x, y) -> (c x - s e^{i phi} y, s x + c e^{i phi} y)
+__global__ void jacobi_rows_kernel(int n, int npair, const int* pairs, double2* A, const double4_t* rot, int batch) {
+  const int64_t total = (int64_t)npair * n * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / ((int64_t)npair * n));
+    const int64_t r = e - (int64_t)b * npair * n;
+    const int k = (int)(r / n), j = (int)(r % n);
+    const int p = pairs[2 * k], q = pairs[2 * k + 1];
+    if (p < 0 || q < 0) continue;
+    const double4_t R = rot[(int64_t)b * npair + k];
+    if (R[1] == 0.0) continue;
+    double2* M = A + (int64_t)b * n * n;
+    const double2 x = M[(int64_t)p * n + j], y = M[(int64_t)q * n + j];
+    const double2 ey = cmul(make_double2(R[2], R[3]), y);  // e^{i phi} y
+    M[(int64_t)p * n + j] = csub(cscale(x, R[0]), cscale(ey, R[1]));
+    M[(int64_t)q * n + j] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
+  }
+}
+
+// columns p, q <- columns J: (x, y) -> (c x - s e^{-i phi} y, s x + c e^{-i phi} y), for A and V
+__global__ void jacobi_cols_kernel(int n, int npair, const int* pairs, double2* A, double2* V, const double4_t* rot,
+                                   int batch) {
+  const int64_t total = (int64_t)npair * n * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / ((int64_t)npair * n));
+    const int64_t r = e - (int64_t)b * npair * n;
+    const int i = (int)(r / npair), k = (int)(r % npair);
+    const int p = pairs[2 * k], q = pairs[2 * k + 1];
+    if (p < 0 || q < 0) continue;
+    const double4_t R = rot[(int64_t)b * npair + k];
+    if (R[1] == 0.0) continue;
+    const double2 em = make_double2(R[2], -R[3]);  // e^{-i phi}
+    for (int w = 0; w < 2; ++w) {
+      double2* M = (w == 0 ? A : V) + (int64_t)b * n * n + (int64_t)i * n;
+      const double2 x = M[p], y = M[q];
+      const double2 ey = cmul(em, y);
+      M[p] = csub(cscale(x, R[0]), cscale(ey, R[1]));
+      M[q] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
+    }
+  }
+}
+
+// off-diagonal and total squared Frobenius norms per batch entry (atomics)
+__global__ void offdiag_kernel(int n, const double2* A, double* out, int batch) {
+  const int b = blockIdx.y;
+  const double2* M = A + (int64_t)b * n * n;
+  double off = 0.0, tot = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < (int64_t)n * n;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = M[e];
+    const double a = v.x * v.x + v.y * v.y;
+    tot += a;
+    if (e / n != e % n) off += a;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    off += __shfl_xor(off, o);
+    tot += __shfl_xor(tot, o);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(out + 2 * b, off);
+    atomicAdd(out + 2 * b + 1, tot);
+  }
+}
+
+__global__ void eye_kernel(int n, double2* V, int batch) {
+  const int64_t total = (int64_t)n * n * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = e % ((int64_t)n * n);
+    V[e] = (r / n == r % n) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
+  }
+}
+
+__global__ void zero_kernel(double* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+__global__ void diag_kernel(int n, const double2* A, double* w, int batch) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * batch) return;
+  const int b = (int)(e / n), i = (int)(e % n);
+  w[e] = A[(int64_t)b * n * n + (int64_t)i * n + i].x;
+}
+
+// phi[b][m][j] = V[b][j][order[b][m]], alpha[b][m] = w[b][order[b][m]]
+__global__ void gather_kernel(int n, int nk, const double2* V, const int* order, double2* phi, int batch) {
+  const int64_t total = (int64_t)nk * n * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / ((int64_t)nk * n));
+    const int64_t r = e - (int64_t)b * nk * n;
+    const int m = (int)(r / n), j = (int)(r % n);
+    phi[e] = V[(int64_t)b * n * n + (int64_t)j * n + order[b * nk + m]];
+  }
+}
+
+// ---------------------------------------------------------------- SOCS image
+struct SocsSetup {
+  double2 ampvc, ampab, phase0;
+  double k, dx, dy, kx0, ky0, z0;
+  int lpmx, lpmy;  // thin spectrum window
+  int lpx, lpy;    // pupil half widths
+};
+
+// the four amplitude vectors of socs.py:83-98 (thin amplitude from the indicator spectrum)
+__global__ void socs_amps_kernel(SocsSetup a, const double2* fmask, const double2* a0, const double2* ax,
+                                 const double2* ay, const int* pupil, int ncut, double2* amps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncut) return;
+  const int lp = pupil[2 * i], mp = pupil[2 * i + 1];
+  double2 fm = cmul(csub(a.ampvc, a.ampab), fmask[(int64_t)(lp + a.lpmx) * (2 * a.lpmy + 1) + (mp + a.lpmy)]);
+  if (lp == 0 && mp == 0) fm = cadd(fm, a.ampab);
+  const double kxp = 2.0 * PI * lp / a.dx, kyp = 2.0 * PI * mp / a.dy;
+  const double ph = -(a.kx0 * kxp + kxp * kxp / 2.0) / a.k * a.z0 - (a.ky0 * kyp + kyp * kyp / 2.0) / a.k * a.z0;
+  double sp, cp;
+  sincos(ph, &sp, &cp);
+  const double2 famp = cdiv(cmul(fm, make_double2(cp, sp)), a.phase0);
+  amps[i] = cadd(famp, a0[i]);                                                // Axs0
+  amps[ncut + i] = cscale(ax[i], a.dx / 2.0 / PI);                            // Axsx
+  amps[2 * ncut + i] = cscale(ay[i], a.dy / 2.0 / PI);                        // Axsy
+  amps[3 * ncut + i] = cadd(cscale(ax[i], lp / 2.0), cscale(ay[i], mp / 2.0));  // Axsxy
+}
+
+// field spectra: field f = (amplitude vector va[f], kernel set vk[f], kernel m[f])
+__global__ void socs_spectra_kernel(const SocsSetup a, int nf, const int* fdesc, const double2* amps,
+                                    const double2* phi, int nk, const int* pupil, int ncut, double2* F) {
+  const int64_t total = (int64_t)nf * ncut;
+  const int nl = 2 * a.lpx + 1, nm = 2 * a.lpy + 1;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int f = (int)(e / ncut), i = (int)(e % ncut);
+    const int va = fdesc[3 * f], vk = fdesc[3 * f + 1], m = fdesc[3 * f + 2];
+    const double2 v = cmul(amps[(int64_t)va * ncut + i], phi[((int64_t)vk * nk + m) * ncut + i]);
+    F[(int64_t)f * nl * nm + (int64_t)(pupil[2 * i] + a.lpx) * nm + (pupil[2 * i + 1] + a.lpy)] = v;
+  }
+}
+
+// G[f][l][y] = sum_m F[f][l][m] e^{2 pi i m y / nout}
+__global__ void socs_ypass_kernel(int nf, int nl, int nm, int lpy, int nout, const double2* F, const double2* tw,
+                                  double2* G) {
+  const int64_t total = (int64_t)nf * nl * nout;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(e % nout);
+    const int64_t fl = e / nout;
+    const double2* f = F + fl * nm;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int mm = 0; mm < nm; ++mm) {
+      const int m = mm - lpy;
+      acc = cadd(acc, cmul(f[mm], tw[(((int64_t)m * y) % nout + nout) % nout]));
+    }
+    G[e] = acc;
+  }
+}
+
+// fields[f][x][y] = sum_l e^{2 pi i l x / nout} G[f][l][y]
+__global__ void socs_xpass_kernel(int nf, int nl, int lpx, int nout, const double2* G, const double2* tw,
+                                  double2* fld) {
+  const int64_t total = (int64_t)nf * nout * nout;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int f = (int)(e / ((int64_t)nout * nout));
+    const int64_t r = e - (int64_t)f * nout * nout;
+    const int x = (int)(r / nout), y = (int)(r % nout);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int l = 0; l < nl; ++l)
+      acc = cadd(acc, cmul(tw[(((int64_t)(l - lpx) * x) % nout + nout) % nout], G[((int64_t)f * nl + l) * nout + y]));
+    fld[e] = acc;
+  }
+}
+
+__device__ inline double re_dot(double2 a, double2 b) { return a.x * b.x + a.y * b.y; }
+
+// small image (socs.py:100-142): fields ordered [f0_0..f0_{n0-1}] then, per m < nxy,
+// (f0x, f0y, fx, fy, fxy)
+__global__ void socs_combine_kernel(int nout, int n0, int nxy, const double2* fld, const double* alpha, int nk,
+                                    double* small) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= nout * nout) return;
+  const int64_t np = (int64_t)nout * nout;
+  double acc = 0.0;
+  for (int m = 0; m < n0; ++m) {
+    const double2 f0 = fld[m * np + pix];
+    acc += alpha[m] * (f0.x * f0.x + f0.y * f0.y);
+    if (m < nxy) {
+      const double2* g = fld + (n0 + 5 * (int64_t)m) * np + pix;
+      const double2 f0x = g[0], f0y = g[np], fx = g[2 * np], fy = g[3 * np], fxy = g[4 * np];
+      acc += 2.0 * alpha[nk + m] * re_dot(f0x, fx) + 2.0 * alpha[2 * nk + m] * re_dot(f0y, fy) +
+             2.0 * alpha[m] * re_dot(f0, fxy);
+    }
+  }
+  small[pix] = acc;
+}
+
+// S[u][v] = (1/nout^2) sum_{x,y} small[x][y] e^{-2 pi i (u x + v y)/nout}, u, v in [-nout/2, nout/2)
+__global__ void socs_spec_kernel(int nout, const double* small, const double2* twm, double2* S) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nout * nout) return;
+  const int ui = e / nout, vi = e % nout;
+  const int u = ui - nout / 2, v = vi - nout / 2;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int x = 0; x < nout; ++x) {
+    double2 row = make_double2(0.0, 0.0);
+    for (int y = 0; y < nout; ++y)
+      row = cadd(row, cscale(twm[(((int64_t)v * y) % nout + nout) % nout], small[x * nout + y]));
+    acc = cadd(acc, cmul(twm[(((int64_t)u * x) % nout + nout) % nout], row));
+  }
+  S[e] = cscale(acc, 1.0 / ((double)nout * nout));
+}
+
+// T[u][y] = sum_v S[u][v] e^{2 pi i v y / ny}
+__global__ void socs_up_y_kernel(int nout, int ny, const double2* S, const double2* twy, double2* T) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nout * ny) return;
+  const int ui = (int)(e / ny), y = (int)(e % ny);
+  double2 acc = make_double2(0.0, 0.0);
+  for (int vi = 0; vi < nout; ++vi) {
+    const int v = vi - nout / 2;
+    acc = cadd(acc, cmul(S[ui * nout + vi], twy[(((int64_t)v * y) % ny + ny) % ny]));
+  }
+  T[e] = acc;
+}
+
+// img[x][y] = Re sum_u T[u][y] e^{2 pi i u x / nx} * (nx ny) / (nx ny)
+__global__ void socs_up_x_kernel(int nout, int nx, int ny, const double2* T, const double2* twx, double* img) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nx * ny) return;
+  const int x = (int)(e / ny), y = (int)(e % ny);
+  double acc = 0.0;
+  for (int ui = 0; ui < nout; ++ui) {
+    const int u = ui - nout / 2;
+    const double2 t = twx[(((int64_t)u * x) % nx + nx) % nx];
+    const double2 v = T[(int64_t)ui * ny + y];
+    acc += t.x * v.x - t.y * v.y;
+  }
+  img[e] = acc;
+}
+
+std::vector<std::complex<double>> twiddles(int n, double sign) {
+  std::vector<std::complex<double>> t(n);
+  const std::complex<double> zi(0.0, 1.0);
+  for (int i = 0; i < n; ++i) t[i] = std::exp(sign * 2.0 * PI * zi * double(i) / double(n));
+  return t;
+}
+
+}  // namespace
+
+namespace euvl {
+int fill(euvl_handle h, int64_t count, double2* out, double2 v);
+}
+
+extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
+                                const double* h_src, double kx0, double ky0, double z0, double z, int nk,
+                                double* d_alpha, double* d_phi, int* h_sweeps) {
+  using namespace euvl;
+  if (!h || !p || ncut <= 1 || !h_pupil || n_src <= 0 || !h_src || nk <= 0 || nk > ncut || !d_alpha || !d_phi) {
+    set_error("euvl_tcc_kernels: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  const int n = ncut, B = 3, S = n_src;
+  // round-robin schedule over n' = n rounded up to even players (index -1 = bye)
+  const int np = n + (n & 1), npair = np / 2, nsteps = np - 1;
+  std::vector<int> pairs((size_t)nsteps * npair * 2);
+  {
+    std::vector<int> ring(np);
+    std::iota(ring.begin(), ring.end(), 0);
+    for (int st = 0; st < nsteps; ++st) {
+      for (int k = 0; k < npair; ++k) {
+        int a = ring[k], b = ring[np - 1 - k];
+        if (a >= n) a = -1;
+        if (b >= n) b = -1;
+        pairs[((size_t)st * npair + k) * 2] = a;
+        pairs[((size_t)st * npair + k) * 2 + 1] = b;
+      }
+      std::rotate(ring.begin() + 1, ring.end() - 1, ring.end());
+    }
+  }
+  Upload up;
+  const size_t o_pairs = up.add(pairs.data(), sizeof(int) * pairs.size());
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * n);
+  const size_t o_src = up.add(h_src, sizeof(double) * 2 * S);
+  const size_t mat = sizeof(double2) * (size_t)n * n * B;
+  const size_t pb = sizeof(double2) * (size_t)n * 3 * S;
+  EUVL_TRY(ws_reserve(h, up.host.size() + 2 * mat + 4 * pb + sizeof(double4_t) * npair * B + sizeof(double) * n * B +
+                             sizeof(int) * nk * B + 8192));
+  char* blk = (char*)ws_take(h, up.host.size());
+  double2* A = (double2*)ws_take(h, mat);
+  double2* V = (double2*)ws_take(h, mat);
+  double2* P = (double2*)ws_take(h, pb);
+  double2* Q = (double2*)ws_take(h, 3 * pb);
+  double4_t* rot = (double4_t*)ws_take(h, sizeof(double4_t) * npair * B);
+  double* w = (double*)ws_take(h, sizeof(double) * n * B);
+  int* order_d = (int*)ws_take(h, sizeof(int) * nk * B);
+  if (!A || !V || !P || !Q || !rot || !w || !order_d) {
+    set_error("euvl_tcc_kernels: workspace carve failed");
+    return EUVL_ERR_NOMEM;
+  }
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  TccSetup a{p->k, p->dx, p->dy, p->na, kx0, ky0, z0, z, p->mx, p->my};
+  hipLaunchKernelGGL(tcc_phase_kernel, grid1((int64_t)n * S), dim3(256), 0, h->stream, a, (const int*)(blk + o_pup),
+                     n, (const double*)(blk + o_src), S, P, Q);
+  EUVL_LAUNCH_CHECK();
+  // TCC_w = P Q_w  (n x 3S x n), the three weights as a batch sharing P
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  EUVL_TRY(zgemm(h, n, n, 3 * S, one, P, 3 * S, 0, Q, n, (int64_t)3 * S * n, zero, A, n, (int64_t)n * n, zero, B));
+  hipLaunchKernelGGL(eye_kernel, grid1((int64_t)n * n * B), dim3(256), 0, h->stream, n, V, B);
+  EUVL_LAUNCH_CHECK();
+  const int* d_pairs = (const int*)(blk + o_pairs);
+  int sweep = 0;
+  double worst = INFINITY;
+  for (sweep = 0; sweep < 30; ++sweep) {
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, h->stream, h->d_red, 2 * B);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(offdiag_kernel, dim3(256, B), dim3(256), 0, h->stream, n, A, h->d_red, B);
+    EUVL_LAUNCH_CHECK();
+    EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 2 * B, hipMemcpyDeviceToHost, h->stream));
+    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+    worst = 0.0;
+    for (int b = 0; b < B; ++b) {
+      const double off = h->h_pinned[2 * b], tot = h->h_pinned[2 * b + 1];
+      const double r = (tot > 0.0) ? off / tot : 0.0;
+      worst = std::max(worst, std::isnan(r) ? INFINITY : r);
+    }
+    if (worst < 1e-30) break;
+    for (int st = 0; st < nsteps; ++st) {
+      const int* pr = d_pairs + (size_t)st * npair * 2;
+      hipLaunchKernelGGL(jacobi_params_kernel, grid1((int64_t)npair * B), dim3(256), 0, h->stream, n, npair, pr, A,
+                         rot, B);
+      hipLaunchKernelGGL(jacobi_rows_kernel, grid1((int64_t)npair * n * B), dim3(256), 0, h->stream, n, npair, pr, A,
+                         rot, B);
+      hipLaunchKernelGGL(jacobi_cols_kernel, grid1((int64_t)npair * n * B), dim3(256), 0, h->stream, n, npair, pr, A,
+                         V, rot, B);
+    }
+    EUVL_LAUNCH_CHECK();
+  }
+  if (h_sweeps) *h_sweeps = sweep;
+  if (!(worst < 1e-26)) {
+    set_error("euvl_tcc_kernels: Jacobi off-diagonal ratio %.3e after %d sweeps", worst, sweep);
+    return EUVL_ERR_NOT_CONVERGED;
+  }
+  // eigenvalues, stable sort by |alpha| descending (the reference's bubble sort order), top nk
+  hipLaunchKernelGGL(diag_kernel, grid1((int64_t)n * B), dim3(256), 0, h->stream, n, A, w, B);
+  EUVL_LAUNCH_CHECK();
+  std::vector<double> hw((size_t)n * B);
+  EUVL_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * n * B, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  std::vector<int> order((size_t)nk * B);
+  std::vector<double> halpha((size_t)nk * B);
+  for (int b = 0; b < B; ++b) {
+    std::vector<int> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    // ties keep ascending eigenvalue order, as after the reference's ascending eigh
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return hw[(size_t)b * n + x] < hw[(size_t)b * n + y]; });
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int x, int y) { return std::fabs(hw[(size_t)b * n + x]) > std::fabs(hw[(size_t)b * n + y]); });
+    for (int m = 0; m < nk; ++m) {
+      order[(size_t)b * nk + m] = idx[m];
+      halpha[(size_t)b * nk + m] = hw[(size_t)b * n + idx[m]];
+    }
+  }
+  EUVL_HIP_CHECK(hipMemcpyAsync(order_d, order.data(), sizeof(int) * nk * B, hipMemcpyHostToDevice, h->stream));
+  EUVL_HIP_CHECK(hipMemcpyAsync(d_alpha, halpha.data(), sizeof(double) * nk * B, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(gather_kernel, grid1((int64_t)nk * n * B), dim3(256), 0, h->stream, n, nk, V, order_d,
+                     (double2*)d_phi, B);
+  EUVL_LAUNCH_CHECK();
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}
+
+extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x,
+                               int lpmax_y, const double* ampvc, const double* ampab, const double* phase0,
+                               const double* d_a0, const double* d_ax, const double* d_ay, int ncut,
+                               const int* h_pupil, const double* d_alpha, const double* d_phi, int nk, int nsocs,
+                               int nsocsxy, int nout, double kx0, double ky0, double z0, int nx, int ny,
+                               double* d_img) {
+  using namespace euvl;
+  if (!h || !p || !d_fmask || !ampvc || !ampab || !phase0 || !d_a0 || !d_ax || !d_ay || ncut <= 0 || !h_pupil ||
+      !d_alpha || !d_phi || nk <= 0 || nsocs < 0 || nsocsxy < 0 || nsocs > nk || nsocsxy > nsocs || nout < 2 ||
+      (nout & 1) || nx <= 0 || ny <= 0 || !d_img) {
+    set_error("euvl_socs_image: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  int lpx = 0, lpy = 0;
+  std::vector<char> seen((size_t)nout * nout, 0);
+  for (int i = 0; i < ncut; ++i) {
+    lpx = std::max(lpx, std::abs(h_pupil[2 * i]));
+    lpy = std::max(lpy, std::abs(h_pupil[2 * i + 1]));
+    const int px = ((h_pupil[2 * i] % nout) + nout) % nout, py = ((h_pupil[2 * i + 1] % nout) + nout) % nout;
+    if (seen[(size_t)px * nout + py]++) {
+      set_error("euvl_socs_image: pupil points collide on the %d x %d grid", nout, nout);
+      return EUVL_ERR_ARG;
+    }
+  }
+  if (lpx > lpmax_x || lpy > lpmax_y) {
+    set_error("euvl_socs_image: pupil outside the thin-mask window");
+    return EUVL_ERR_ARG;
+  }
+  const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
+  const int nf = nsocs + 5 * nsocsxy;
+  std::vector<int> fdesc;
+  for (int m = 0; m < nsocs; ++m) fdesc.insert(fdesc.end(), {0, 0, m});
+  for (int m = 0; m < nsocsxy; ++m) {
+    fdesc.insert(fdesc.end(), {0, 1, m});  // f0x = A0 phix
+    fdesc.insert(fdesc.end(), {0, 2, m});  // f0y = A0 phiy
+    fdesc.insert(fdesc.end(), {1, 1, m});  // fx  = Axsx phix
+    fdesc.insert(fdesc.end(), {2, 2, m});  // fy  = Axsy phiy
+    fdesc.insert(fdesc.end(), {3, 0, m});  // fxy = Axsxy phi0
+  }
+  auto tw_out = twiddles(nout, 1.0), tw_outm = twiddles(nout, -1.0), twx = twiddles(nx, 1.0), twy = twiddles(ny, 1.0);
+  Upload up;
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * ncut);
+  const size_t o_fd = up.add(fdesc.data(), sizeof(int) * fdesc.size());
+  const size_t o_two = up.add(tw_out.data(), sizeof(double2) * nout);
+  const size_t o_twm = up.add(tw_outm.data(), sizeof(double2) * nout);
+  const size_t o_twx = up.add(twx.data(), sizeof(double2) * nx);
+  const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
+  const size_t b_amps = sizeof(double2) * 4 * ncut;
+  const size_t b_F = sizeof(double2) * (size_t)std::max(nf, 1) * nl * nm;
+  const size_t b_G = sizeof(double2) * (size_t)std::max(nf, 1) * nl * nout;
+  const size_t b_fld = sizeof(double2) * (size_t)std::max(nf, 1) * nout * nout;
+  const size_t b_small = sizeof(double) * nout * nout;
+  const size_t b_S = sizeof(double2) * nout * nout;
+  const size_t b_T = sizeof(double2) * (size_t)nout * ny;
+  EUVL_TRY(ws_reserve(h, up.host.size() + b_amps + b_F + b_G + b_fld + b_small + b_S + b_T + 16384));
+  char* blk = (char*)ws_take(h, up.host.size());
+  double2* amps = (double2*)ws_take(h, b_amps);
+  double2* F = (double2*)ws_take(h, b_F);
+  double2* G = (double2*)ws_take(h, b_G);
+  double2* fld = (double2*)ws_take(h, b_fld);
+  double* small = (double*)ws_take(h, b_small);
+  double2* Sp = (double2*)ws_take(h, b_S);
+  double2* T = (double2*)ws_take(h, b_T);
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  SocsSetup a;
+  a.ampvc = make_double2(ampvc[0], ampvc[1]);
+  a.ampab = make_double2(ampab[0], ampab[1]);
+  a.phase0 = make_double2(phase0[0], phase0[1]);
+  a.k = p->k;
+  a.dx = p->dx;
+  a.dy = p->dy;
+  a.kx0 = kx0;
+  a.ky0 = ky0;
+  a.z0 = z0;
+  a.lpmx = lpmax_x;
+  a.lpmy = lpmax_y;
+  a.lpx = lpx;
+  a.lpy = lpy;
+  const int* pup = (const int*)(blk + o_pup);
+  hipLaunchKernelGGL(socs_amps_kernel, dim3((ncut + 255) / 256), dim3(256), 0, h->stream, a, (const double2*)d_fmask,
+                     (const double2*)d_a0, (const double2*)d_ax, (const double2*)d_ay, pup, ncut, amps);
+  EUVL_LAUNCH_CHECK();
+  if (nf > 0) {
+    EUVL_TRY(fill(h, (int64_t)nf * nl * nm, F, make_double2(0.0, 0.0)));
+    hipLaunchKernelGGL(socs_spectra_kernel, grid1((int64_t)nf * ncut), dim3(256), 0, h->stream, a, nf,
+                       (const int*)(blk + o_fd), amps, (const double2*)d_phi, nk, pup, ncut, F);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(socs_ypass_kernel, grid1((int64_t)nf * nl * nout), dim3(256), 0, h->stream, nf, nl, nm, lpy,
+                       nout, F, (const double2*)(blk + o_two), G);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(socs_xpass_kernel, grid1((int64_t)nf * nout * nout), dim3(256), 0, h->stream, nf, nl, lpx,
+                       nout, G, (const double2*)(blk + o_two), fld);
+    EUVL_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(socs_combine_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, nsocs,
+                     nsocsxy, fld, d_alpha, nk, small);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(socs_spec_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, small,
+                     (const double2*)(blk + o_twm), Sp);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(socs_up_y_kernel, grid1((int64_t)nout * ny), dim3(256), 0, h->stream, nout, ny, Sp,
+                     (const double2*)(blk + o_twy), T);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(socs_up_x_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, nout, nx, ny, T,
+                     (const double2*)(blk + o_twx), d_img);
+  EUVL_LAUNCH_CHECK();
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}

@@ -182,3 +182,94 @@ class Engine:
                                             cfg.ky0, cfg.z0, float(z), nx, ny, 1.0 / max(len(src), 1),
                                             _lib.ptr(img)), "euvl_abbe_image")
         return img
+
+    # ------------------------------------------------ images from M3D params
+    def reflect_amplitude(self):
+        """(ampvc, ampab, phase0): blank amplitudes at the central incidence
+        (reflectivity.py:4-17); device-resident engine cache, no file."""
+        vcxx, abxx = self.blank_amplitudes()
+        lsx, lsy = self.cfg.lsmax
+        ampvc, ampab = complex(vcxx[lsx, lsy]), complex(abxx[lsx, lsy])
+        return ampvc, ampab, ampvc / abs(ampvc)
+
+    def _m3d_inputs(self, a0, ax, ay):
+        ncut = len(self.cfg.pupil[0])
+        out = []
+        for v in (a0, ax, ay):
+            t = torch.as_tensor(np.asarray(v.cpu() if torch.is_tensor(v) else v), dtype=torch.complex128)
+            if t.numel() != ncut:
+                raise ValueError(f"M3D vector has {t.numel()} values, the pupil has {ncut} points")
+            out.append(t.to(self.device).contiguous())
+        return out
+
+    def linear(self, bits, a0, ax, ay, z=0.0, mesh_deg=0.2):
+        """(intlinear, intft) images (linear.py:190-225): Abbe imaging of the
+        M3D-linear model and of the thin mask on the source mesh."""
+        cfg = self.cfg
+        ampvc, ampab, ph0 = self.reflect_amplitude()
+        lpx, lpy = cfg.lpmax
+        fmask = self.mask_spectrum(bits, lpx, lpy)[0]
+        a0, ax, ay = self._m3d_inputs(a0, ax, ay)
+        lin, min_, _ = cfg.pupil
+        pu, pup = _lib.arr_i(np.stack([lin, min_], axis=1))
+        dsx, dsy = cfg.mesh_source(mesh_deg)
+        sr, srp = _lib.arr_d(np.stack([dsx, dsy], axis=1))
+        nx, ny = cfg.image_shape
+        il = torch.empty((nx, ny), dtype=torch.float64, device=self.device)
+        it = torch.empty_like(il)
+        cv = lambda z_: (ctypes.c_double * 2)(z_.real, z_.imag)
+        self._bind_stream()
+        _lib.check(self.lib.euvl_linear_images(self.h, ctypes.byref(self.params), _lib.ptr(fmask), lpx, lpy, cv(ampvc),
+                                               cv(ampab), cv(ph0), _lib.ptr(a0), _lib.ptr(ax), _lib.ptr(ay), len(lin),
+                                               pup, len(dsx), srp, cfg.kx0, cfg.ky0, cfg.z0, float(z),
+                                               float(cfg.obscuration), nx, ny, _lib.ptr(il), _lib.ptr(it)),
+                   "euvl_linear_images")
+        return il, it
+
+    def eigenfunction(self, z=0.0, nk=100, mesh_deg=0.2):
+        """STCC kernels (eigenfunction.py:5-179) kept on the device:
+        (alpha [3, nk], phi [3, nk, ncut]); cached per (z, nk, mesh)."""
+        key = (float(z), int(nk), float(mesh_deg))
+        cache = getattr(self, "_kernels", {})
+        if key in cache:
+            return cache[key]
+        cfg = self.cfg
+        lin, min_, _ = cfg.pupil
+        nk = min(nk, len(lin))
+        pu, pup = _lib.arr_i(np.stack([lin, min_], axis=1))
+        dsx, dsy = cfg.mesh_source(mesh_deg)
+        sr, srp = _lib.arr_d(np.stack([dsx, dsy], axis=1))
+        alpha = torch.empty((3, nk), dtype=torch.float64, device=self.device)
+        phi = torch.empty((3, nk, len(lin)), dtype=torch.complex128, device=self.device)
+        sweeps = ctypes.c_int()
+        self._bind_stream()
+        _lib.check(self.lib.euvl_tcc_kernels(self.h, ctypes.byref(self.params), len(lin), pup, len(dsx), srp, cfg.kx0,
+                                             cfg.ky0, cfg.z0, float(z), nk, _lib.ptr(alpha), _lib.ptr(phi),
+                                             ctypes.byref(sweeps)), "euvl_tcc_kernels")
+        self.last_sweeps = sweeps.value
+        cache[key] = (alpha, phi)
+        self._kernels = cache
+        return alpha, phi
+
+    def socs(self, bits, a0, ax, ay, nsocs=100, nsocsxy=20, nout=64, z=0.0, mesh_deg=0.2):
+        """STCC-SOCS image (socs.py:156-204) on the config's image grid."""
+        cfg = self.cfg
+        alpha, phi = self.eigenfunction(z, max(nsocs, 1), mesh_deg)
+        nk = alpha.shape[1]
+        nsocs = min(nsocs, nk)
+        nsocsxy = min(nsocsxy, nsocs)  # the xy terms ride inside the first nsocs kernels (socs.py:100-140)
+        ampvc, ampab, ph0 = self.reflect_amplitude()
+        lpx, lpy = cfg.lpmax
+        fmask = self.mask_spectrum(bits, lpx, lpy)[0]
+        a0, ax, ay = self._m3d_inputs(a0, ax, ay)
+        lin, min_, _ = cfg.pupil
+        pu, pup = _lib.arr_i(np.stack([lin, min_], axis=1))
+        nx, ny = cfg.image_shape
+        img = torch.empty((nx, ny), dtype=torch.float64, device=self.device)
+        cv = lambda z_: (ctypes.c_double * 2)(z_.real, z_.imag)
+        self._bind_stream()
+        _lib.check(self.lib.euvl_socs_image(self.h, ctypes.byref(self.params), _lib.ptr(fmask), lpx, lpy, cv(ampvc),
+                                            cv(ampab), cv(ph0), _lib.ptr(a0), _lib.ptr(ax), _lib.ptr(ay), len(lin), pup,
+                                            _lib.ptr(alpha), _lib.ptr(phi), nk, nsocs, nsocsxy, nout, cfg.kx0, cfg.ky0,
+                                            cfg.z0, nx, ny, _lib.ptr(img)), "euvl_socs_image")
+        return img

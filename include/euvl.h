@@ -181,6 +181,44 @@ int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, 
                  const double* h_vc, const double* h_ab, double kx0, double ky0, double z0,
                  int ncut, const int* h_pupil, double* d_a0, double* d_ax, double* d_ay);
 
+/* Thin-mask and M3D-linear Abbe images on the source mesh (h_src: n_src x 2
+ * doubles, the wave-number offsets dsx, dsy of linear.py:46-97).  Thin
+ * amplitude from the indicator spectrum d_fmask (noutx x nouty, lw = lpmax_x,
+ * mw = lpmax_y): fmask = ampab delta + (ampvc - ampab) m^, times the z0
+ * phases, over phase0 (linear.py:120-143); pupil fields with
+ * rotate() and the central obscuration co (electro_field.py:162-189); image
+ * = mean over sources of |Ex|^2+|Ey|^2+|Ez|^2 on nx x ny (linear.py:10-43).
+ * ampvc, ampab, phase0: host (re, im).  d_a0, d_ax, d_ay: ncut complex (ax,
+ * ay zero where the pupil point has fewer than 8 sources).  Writes
+ * d_intlinear (linear model) and d_intft (thin mask), nx x ny, index x*ny+y.
+ * Replaces int_linear + int_abbe (cnn/intensity/linear/linear.py:10-188). */
+int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
+                       const double* ampvc, const double* ampab, const double* phase0, const double* d_a0,
+                       const double* d_ax, const double* d_ay, int ncut, const int* h_pupil, int n_src,
+                       const double* h_src, double kx0, double ky0, double z0, double z, double co, int nx, int ny,
+                       double* d_intlinear, double* d_intft);
+
+/* STCC kernels: TCC_{S0,SX,SY} = sum_c P_c diag(w) P_c^H / S over the source
+ * mesh (w = k^2/(k^2-(kx0+sx)^2) times 1, sx, sy), Hermitian eigenpairs by a
+ * parallel Jacobi method, sorted by |alpha| descending; the leading nk of
+ * each: d_alpha (3 x nk doubles), d_phi (3 x nk x ncut complex, kernel m of
+ * set b at [b][m][:]).  *h_sweeps receives the Jacobi sweep count.
+ * Replaces eigenfunction() (cnn/intensity/stccsocs/eigenfunction.py:5-179)
+ * and its cupy.linalg.eigh calls (:130-132). */
+int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
+                     const double* h_src, double kx0, double ky0, double z0, double z, int nk, double* d_alpha,
+                     double* d_phi, int* h_sweeps);
+
+/* STCC-SOCS image (socs.py:12-154): nsocs kernels for a0 and nsocsxy for
+ * ax/ay on an nout x nout grid, Fourier-interpolated to nx x ny (d_img,
+ * index x*ny+y).  Amplitude inputs as euvl_linear_images; kernels from
+ * euvl_tcc_kernels.  Replaces intsocs (cnn/intensity/stccsocs/socs.py:12-154). */
+int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
+                    const double* ampvc, const double* ampab, const double* phase0, const double* d_a0,
+                    const double* d_ax, const double* d_ay, int ncut, const int* h_pupil, const double* d_alpha,
+                    const double* d_phi, int nk, int nsocs, int nsocsxy, int nout, double kx0, double ky0,
+                    double z0, int nx, int ny, double* d_img);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,151 @@
+"""GPU parity of the images computed from M3D parameters: the linear model and
+the thin mask on the source mesh (linear.py), the STCC kernels
+(eigenfunction.py) and the SOCS image (socs.py), each through the C ABI
+against the FP64 oracle on the same inputs.
+
+Tolerances: images 1e-10 of their peak, kernel eigenvalues 1e-10 of the
+largest, the TCC rebuilt from the kernels 1e-10 of its largest entry (the
+kernels themselves are fixed only up to a phase, and inside a degenerate
+eigenspace only up to a unitary mix, so the test compares what is invariant).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+ANNULAR = dict(kind=1, sigma_out=0.8, sigma_in=0.3)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test without a GPU")
+
+
+def problem(ndiv, K, image, ndivs=1):
+    from euvlitho_amd.config import high_na
+    from oracle import geometry as G, problems as OP
+    cfg = high_na(ndiv, K, ANNULAR, ndivs=ndivs, image=image)
+    p = OP.hna(cfg.ndivx, cfg.orders[1], None)
+    return cfg, p, G.derived(p)
+
+
+def oracle_mesh(p, d, mesh_deg):
+    from oracle import geometry as G
+    return G.mesh_source(p, d, 1, ANNULAR["sigma_out"], ANNULAR["sigma_in"], 0.0, mesh_deg)
+
+
+def m3d_vectors(ncut, seed, scale):
+    """Synthetic a0, ax, ay (the images are linear in them, so any values test the path)."""
+    rng = np.random.default_rng(seed)
+    v = [scale * (rng.standard_normal(ncut) + 1j * rng.standard_normal(ncut)) for _ in range(3)]
+    v[1][::5] = 0.0  # pupil points with fewer than 8 sources carry no slope
+    v[2][::7] = 0.0
+    return v
+
+
+def oracle_fampxx(p, d, mask):
+    from oracle import imaging as OI
+    ampvc, ampab, ph0 = OI.reflect_amplitude(p, d)
+    return OI.thin_amplitudes(OI.pattern_spectrum(mask, p, d, ampvc, ampab), p, d, ph0)
+
+
+@pytest.mark.parametrize("z", [0.0, 15.0])
+def test_linear_images_vs_oracle(z):
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import imaging as OI
+    say(f"linear images 256 nm clip, 64x64 image, z={z}")
+    cfg, p, d = problem(256, 4, (64, 64))
+    eng = Engine(cfg)
+    mask = manhattan(256, 256, 0.5, 11)
+    ncut = len(cfg.pupil[0])
+    a0, ax, ay = m3d_vectors(ncut, 5, 0.02)
+    il, it = eng.linear(eng.upload_mask(mask), a0, ax, ay, z=z)
+    il, it = il.cpu().numpy(), it.cpu().numpy()
+    dsx, dsy = oracle_mesh(p, d, 0.2)
+    rl, rt = OI.thin_linear_images(p, d, oracle_fampxx(p, d, mask), a0, ax, ay, dsx, dsy, 64, 64, z=z)
+    for g, r, name in ((il, rl, "intlinear"), (it, rt, "intft")):
+        e = np.abs(g - r).max() / r.max()
+        say(f"  {name}: max err / peak = {e:.3e}")
+        assert e < 1e-10
+
+
+def _tcc(alpha, phi):
+    """sum_m alpha_m phi_m phi_m^H (phi[m] = kernel m)."""
+    return (phi.T * alpha) @ phi.conj()
+
+
+def test_tcc_kernels_vs_oracle():
+    from euvlitho_amd.engine import Engine
+    from oracle import imaging as OI
+    say("TCC kernels 256 nm clip, 0.2 degree mesh")
+    cfg, p, d = problem(256, 4, (64, 64))
+    eng = Engine(cfg)
+    ncut = len(cfg.pupil[0])
+    alpha, phi = eng.eigenfunction(0.0, ncut)
+    alpha, phi = alpha.cpu().numpy(), phi.cpu().numpy()
+    say(f"  Jacobi sweeps {eng.last_sweeps}")
+    dsx, dsy = oracle_mesh(p, d, 0.2)
+    ref = OI.tcc_kernels(p, d, dsx, dsy)
+    for c in range(3):
+        ra, rp = ref[2 * c], ref[2 * c + 1]
+        s = np.abs(ra).max()
+        # the x and y TCCs have +-alpha pairs: equal |alpha| up to rounding, so
+        # their order inside a pair is not fixed; compare the spectra as sets
+        ea = np.abs(np.sort(alpha[c]) - np.sort(ra)).max() / s
+        T, Tr = _tcc(alpha[c], phi[c]), _tcc(ra, rp)
+        et = np.abs(T - Tr).max() / np.abs(Tr).max()
+        say(f"  TCC {c}: eigenvalues {ea:.3e}, rebuilt TCC {et:.3e}")
+        assert ea < 1e-10 and et < 1e-10
+        # kernels are orthonormal
+        assert np.abs(phi[c].conj() @ phi[c].T - np.eye(ncut)).max() < 1e-10
+        # |alpha| descending
+        assert np.all(np.diff(np.abs(alpha[c])) <= 1e-12 * s)
+
+
+def _socs_case(ndiv, K, image, mesh_deg, nsocs, nsocsxy, seed):
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import imaging as OI
+    cfg, p, d = problem(ndiv, K, image)
+    eng = Engine(cfg)
+    mask = manhattan(ndiv, ndiv, 0.5, seed)
+    ncut = len(cfg.pupil[0])
+    a0, ax, ay = m3d_vectors(ncut, seed, 0.02)
+    dsx, dsy = oracle_mesh(p, d, mesh_deg)
+    kern = OI.tcc_kernels(p, d, dsx, dsy)
+    # a truncation inside a (near-)degenerate eigenspace would make the image
+    # depend on the basis chosen there; the cases below cut at clear gaps
+    for c, n in ((0, nsocs), (1, nsocsxy), (2, nsocsxy), (0, nsocsxy)):
+        a = np.abs(kern[2 * c])
+        if n < ncut:
+            gap = (a[n - 1] - a[n]) / a[0]
+            say(f"  kernel set {c}: gap at {n} = {gap:.2e}")
+            assert gap > 1e-6, "truncation at a degenerate eigenvalue"
+    img = eng.socs(eng.upload_mask(mask), a0, ax, ay, nsocs=nsocs, nsocsxy=nsocsxy, mesh_deg=mesh_deg)
+    img = img.cpu().numpy()
+    ref = OI.socs_image(p, d, oracle_fampxx(p, d, mask), a0, ax, ay, kern, image[0], image[1], nsocs=nsocs,
+                        nsocsxy=nsocsxy)
+    e = np.abs(img - ref).max() / np.abs(ref).max()
+    say(f"  SOCS image max err / peak = {e:.3e}")
+    assert e < 1e-10
+
+
+def test_socs_all_kernels_vs_oracle():
+    say("SOCS 256 nm clip, every kernel")
+    _socs_case(256, 4, (64, 64), 0.2, 1000, 1000, 21)
+
+
+def test_socs_truncated_vs_oracle():
+    say("SOCS 512 nm clip, 0.5 degree mesh, 100 / 20 kernels")
+    _socs_case(512, 10, (128, 128), 0.5, 100, 20, 22)

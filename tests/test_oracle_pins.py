@@ -264,3 +264,72 @@ def test_reference_default_end_to_end(golden):
     bars = [1.2e-2, 7e-3, 5e-3, 4e-3, 4.5e-2, 4.5e-2]
     assert all(e < b for e, b in zip(rep["inputxx_rel_per_row"], bars))
     assert rep["emint_max_rel"] < 3e-2
+
+
+# ----------------------------------------------- images from M3D parameters
+def _imaging_setup(ndiv=256, K=3):
+    from oracle import imaging as OI
+    p, d = small(ndiv, K)
+    m = rand_mask(ndiv, ndiv, 21)
+    ampvc, ampab, ph0 = OI.reflect_amplitude(p, d)
+    fmask = OI.pattern_spectrum(m, p, d, ampvc, ampab)
+    famp = OI.thin_amplitudes(fmask, p, d, ph0)
+    dsx, dsy = G.mesh_source(p, d, 1, 0.8, 0.3, 90.0, mesh_deg=1.0)
+    return OI, p, d, m, famp, dsx, dsy
+
+
+def test_pattern_spectrum_is_affine_in_indicator():
+    OI, p, d, m, famp, dsx, dsy = _imaging_setup()
+    ampvc, ampab, _ = OI.reflect_amplitude(p, d)
+    fm = OI.pattern_spectrum(m, p, d, ampvc, ampab)
+    mh = em.maskamp(m, p["NDIVX"], p["NDIVY"], p["NDIVX"], p["NDIVY"], 0, 0, 0.0, 1.0)  # mean only
+    ind = OI.pattern_spectrum(m, p, d, 1.0, 0.0)
+    dl = np.zeros_like(ind)
+    dl[d["lpmaxX"], d["lpmaxY"]] = 1.0
+    assert abs(ind[d["lpmaxX"], d["lpmaxY"]] - mh[0, 0]) < 1e-15
+    assert np.abs(fm - (ampab * dl + (ampvc - ampab) * ind)).max() < 1e-15
+
+
+def test_linear_image_reduces_to_thin_mask():
+    """a0 = ax = ay = 0 => intlinear == intft (linear.py:166-169)."""
+    OI, p, d, m, famp, dsx, dsy = _imaging_setup()
+    lin, _, _ = G.pupil_points(p, d)
+    z = np.zeros(len(lin), complex)
+    il, ift = OI.thin_linear_images(p, d, famp, z, z, z, dsx, dsy, 64, 64)
+    assert np.abs(il - ift).max() == 0.0
+    assert ift.max() > 0
+
+
+def test_socs_all_kernels_equals_abbe():
+    """Hopkins/SOCS identity: with every kernel, ax = ay = 0 and an alias-free
+    grid, the SOCS image equals the Abbe image of the same mesh source."""
+    OI, p, d, m, famp, dsx, dsy = _imaging_setup()
+    lin, _, _ = G.pupil_points(p, d)
+    rng = np.random.default_rng(3)
+    a0 = 0.01 * (rng.normal(size=len(lin)) + 1j * rng.normal(size=len(lin)))
+    z = np.zeros(len(lin), complex)
+    il, _ = OI.thin_linear_images(p, d, famp, a0, z, z, dsx, dsy, 64, 64)
+    kern = OI.tcc_kernels(p, d, dsx, dsy)
+    for T in (kern[0], kern[2], kern[4]):
+        assert np.all(np.diff(np.abs(T)) <= 0)  # sorted by |alpha| descending
+    img = OI.socs_image(p, d, famp, a0, z, z, kern, 64, 64, nsocs=len(lin), nsocsxy=0)
+    assert np.abs(img - il).max() < 1e-12 * il.max()
+
+
+def test_socs_linear_terms_are_first_order_exact():
+    """The STCC terms (ax, ay) reproduce the linear-model Abbe image to first
+    order: with every kernel, SOCS - Abbe scales as eps^2 when ax, ay -> eps ax, eps ay."""
+    OI, p, d, m, famp, dsx, dsy = _imaging_setup()
+    lin, _, snum = G.pupil_points(p, d)
+    rng = np.random.default_rng(4)
+    n = len(lin)
+    a0 = 0.01 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    ax0 = 0.02 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    ay0 = 0.02 * (rng.normal(size=n) + 1j * rng.normal(size=n))
+    kern = OI.tcc_kernels(p, d, dsx, dsy)
+    diffs = []
+    for eps in (0.01, 0.02):
+        il, _ = OI.thin_linear_images(p, d, famp, a0, eps * ax0, eps * ay0, dsx, dsy, 64, 64)
+        img = OI.socs_image(p, d, famp, a0, eps * ax0, eps * ay0, kern, 64, 64, nsocs=n, nsocsxy=n)
+        diffs.append(np.abs(img - il).max())
+    assert diffs[0] > 0 and abs(diffs[1] / diffs[0] - 4.0) < 0.05
