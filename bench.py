@@ -5,12 +5,15 @@ Workload (one step = one clip, FP64): a seeded 1024 x 1024 nm Manhattan clip
 CRAO 5.355 deg, 60 nm absorber n = 0.947+0.031i on 40 x Mo/Si 2.8/4.2 nm with
 a 2.5 nm Ru cap, orders +-20 x +-20 (N = 1681 modes), X-dipole (sigma 0.5-0.9,
 60 deg) + annular (sigma 0.7-0.8) source on 2 x 2 lattice divisions
-(190 source points, 4 EM solves per clip), outputs: M3D parameters and the
-rigorous Abbe image (256 x 256).
+(190 source points, 4 EM solves per clip), outputs: M3D parameters, the
+rigorous Abbe image (256 x 256) and the STCC-SOCS image from the M3D
+parameters (24 kernels, 20 of them with the x/y slope terms, nout 64).  The
+SOCS kernels (TCC eigendecomposition on the 0.2 degree source mesh) are
+mask-independent: built once, resident on the device, timed separately.
 
 Prints one JSON line (metric, value = clips/s, roofline of the dominant
 kernel = the FP64 MFMA complex GEMM, cpu_baseline = the FP64 NumPy/LAPACK
-oracle on a bounded sample of the same clip).
+oracle on one clip of the same workload).
 """
 import argparse
 import json
@@ -24,18 +27,43 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+SOCS_KERNELS, SOCS_XY = 24, 20
+
+
+def copy_calibration(mib=512, reps=3):
+    """Plain device-to-device copies of a known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    x = torch.ones(mib << 17, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps}), flush=True)
+
+
+def measured_traffic():
+    """HBM bytes per GEMM launch of this workload, measured with rocprofv3 PMC
+    counters (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy of known size) and
+    kept as a summary under profiles/; empty when absent."""
+    path = os.path.join(ROOT, "profiles", "zgemm_mfma_c2_fp64_traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
 
 
 def workload_config():
-    from euvlitho_amd.config import high_na
-    src = dict(kind=2, sigma_out=0.9, sigma_in=0.5, open_angle=60.0, extra=((1, 0.8, 0.7, 90.0),))
-    return high_na(1024, 20, src, ndivs=2, image=(256, 256))
+    from euvlitho_amd.config import preset
+    return preset("c2")
 
 
 def cpu_baseline(cfg, masks, threads):
-    """Oracle (NumPy + LAPACK zgeev/zgesv, FP64) on a bounded sample: the EM solve
-    of one source division plus its Abbe accumulation and the clip's M3D fit;
-    the clip time is extrapolated to all divisions."""
+    """Oracle (NumPy + LAPACK, FP64) on one whole clip of the workload: the EM
+    solves of every source division, the Abbe accumulation over every source
+    point and the M3D fit.  The blank solves are done first, untimed, as the
+    GPU path keeps them resident; the SOCS image (24 + 5 x 20 small IFFTs,
+    well under 1 % of the clip on the CPU) is left out."""
     from oracle import abbe as OA, em as OE, geometry as G, problems as OP
     p = OP.hna(cfg.ndivx, cfg.orders[1], None)
     d = G.derived(p)
@@ -44,24 +72,21 @@ def cpu_baseline(cfg, masks, threads):
     m = masks[0]
     off = cfg.offsets()
     divs = cfg.source_points()
-    t0 = time.perf_counter()
-    Ax = OE.amp_s("X", m, p, d, li, mi, L, M, off[0][0], off[0][1])
-    t1 = time.perf_counter()
-    _, _, l0, m0 = divs[0]
     nx, ny = cfg.image_shape
-    OA.abbe_image(p, d, m, li, mi, L, M, [(tuple(off[0]), l0, m0)], nx, ny, amps=[Ax])
-    t2 = time.perf_counter()
     vc, ab = OA.blank_amplitudes(p, d)
+    t0 = time.perf_counter()
+    amps = [OE.amp_s("X", m, p, d, li, mi, L, M, o[0], o[1]) for o in off]
+    t1 = time.perf_counter()
+    OA.abbe_image(p, d, m, li, mi, L, M, [(tuple(o), l0, m0) for o, (_, _, l0, m0) in zip(off, divs)], nx, ny,
+                  amps=amps)
+    t2 = time.perf_counter()
+    OA.m3d_params(p, d, m, li, mi, amps[0], vc, ab)
     t3 = time.perf_counter()
-    OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
-    t4 = time.perf_counter()
-    n_off = len(off)
     n_src = sum(len(x[2]) for x in divs)
-    t_clip = n_off * (t1 - t0) + (t2 - t1) * n_src / max(len(l0), 1) + (t4 - t3)
-    return dict(value=1.0 / t_clip, unit="clips/s", cores=threads, kind="port",
-                sample=f"oracle EM solve of 1 of {n_off} divisions ({t1 - t0:.1f} s, N={len(li)}), Abbe of its "
-                       f"{len(l0)} of {n_src} source points ({t2 - t1:.1f} s), M3D fit ({t4 - t3:.1f} s); "
-                       f"clip time extrapolated to {t_clip:.1f} s; blank solves cached as on the GPU")
+    return dict(value=1.0 / (t3 - t0), unit="clips/s", cores=threads, kind="port",
+                sample=f"one whole clip (seed 0): oracle EM solves of {len(off)} divisions ({t1 - t0:.1f} s, "
+                       f"N={len(li)}), Abbe over {n_src} source points ({t2 - t1:.1f} s), M3D fit "
+                       f"({t3 - t2:.1f} s); {t3 - t0:.1f} s per clip; LAPACK threads = cores")
 
 
 def main():
@@ -70,12 +95,17 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy that scales the HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); clips are independent, so N GPUs "
               f"would each run this workload on their own share of the clip stream.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import torch
     from euvlitho_amd.engine import Engine
     from euvlitho_amd.masks import manhattan
@@ -89,11 +119,17 @@ def main():
     eng.blank_amplitudes()  # mask-independent, resident for every clip
     torch.cuda.synchronize()
 
+    t_k = time.perf_counter()
+    eng.eigenfunction(0.0, SOCS_KERNELS)  # mask-independent, resident for every clip
+    torch.cuda.synchronize()
+    tcc_ms = 1000.0 * (time.perf_counter() - t_k)
+
     def step(b):
         Ax = eng.amp_s(b, "X")
         a0, ax, ay = eng.m3d(b, Ax0=Ax[0])
         img = eng.abbe(b, Ax=Ax)
-        return a0, img
+        socs = eng.socs(b, a0, ax, ay, nsocs=SOCS_KERNELS, nsocsxy=SOCS_XY)
+        return a0, img, socs
 
     for i in range(args.warmup):
         step(bits[i])
@@ -111,12 +147,13 @@ def main():
 
     # GEMM roofline: HIP events around every GEMM launch of one extra clip
     import ctypes
-    stats = (ctypes.c_double * 3)()
+    stats = (ctypes.c_double * 4)()
     _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
     step(bits[0])
     _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
-    launches, gemm_ms, gemm_flops = stats[0], stats[1], stats[2]
+    launches, gemm_ms, gemm_flops, gemm_bytes = stats[0], stats[1], stats[2], stats[3]
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    traffic = measured_traffic()
 
     line = {
         "metric": "EM+Abbe mask clips/s at <=1e-3 intensity err vs FP64; p50/p99 clip latency",
@@ -134,11 +171,16 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Manhattan clips)",
         "config": {"workload": "plain FP64 drop-in, 1024x1024 nm clip, +-20 orders (N=1681), dipole+annular "
-                               "190 source pts on 2x2 divisions (4 EM solves), M3D params + Abbe 256x256",
-                   "form": "fp64", "clips": args.steps},
+                               "190 source pts on 2x2 divisions (4 EM solves), M3D params + Abbe 256x256 + "
+                               "24-kernel SOCS 256x256",
+                   "form": "fp64", "clips": args.steps, "socs_kernel_build_ms": tcc_ms},
         "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": None, "launches_per_clip": int(launches),
+                     "traffic": traffic.get("traffic_bytes_per_launch"),
+                     "traffic_source": traffic.get("source"),
+                     "algorithmic_bytes_per_launch": gemm_bytes / max(launches, 1),
+                     "launches_per_clip": int(launches),
+                     "avg_launch_us": 1000.0 * gemm_ms / max(launches, 1),
                      "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms},
     }
     if not args.no_cpu_baseline:

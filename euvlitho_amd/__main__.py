@@ -1,0 +1,60 @@
+"""Command-line drivers that replace the reference programs and scripts,
+reading and writing the same files in the working directory:
+
+  python -m euvlitho_amd m3d     mask.bin -> inputlm.csv, inputxx.csv   (m3d.cpp main)
+  python -m euvlitho_amd abbe    mask.bin -> emint.csv | emintY.csv     (intensity.cpp main)
+  python -m euvlitho_amd linear  mask.bin + inputxx.csv -> intlinear.csv, intft.csv  (linear.py __main__)
+  python -m euvlitho_amd socs    mask.bin + inputxx.csv -> intsocs.csv  (socs.py __main__)
+
+--preset picks the problem (reference: the drivers' own optics; c1, c2: the
+high-NA clips); --format g writes the reference C++ 6-digit numbers.
+"""
+import argparse
+import os
+import sys
+import time
+
+from . import csvio
+from .api import Session
+from .config import preset
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m euvlitho_amd", description=__doc__.split("\n\n")[0])
+    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs"])
+    ap.add_argument("--preset", default="reference")
+    ap.add_argument("--mask", default="mask.bin")
+    ap.add_argument("--inputxx", default="inputxx.csv")
+    ap.add_argument("--ndata", type=int, default=1, help="m3d: clips in mask.bin")
+    ap.add_argument("--polar", default="X", choices=["X", "Y"], help="abbe: polarisation")
+    ap.add_argument("--z", type=float, default=0.0, help="defocus (nm)")
+    ap.add_argument("--format", default="repr", choices=list(csvio.FORMATS))
+    ap.add_argument("--out", default=".", help="output directory")
+    a = ap.parse_args(argv)
+
+    ses = Session(preset(a.preset))
+    out = lambda name: os.path.join(a.out, name)
+    t0 = time.perf_counter()
+    if a.command == "m3d":
+        lin, min_, _ = ses.cfg.pupil
+        csvio.write_inputlm(out("inputlm.csv"), lin, min_, ses.xinput)
+        for s, (a0, ax, ay) in enumerate(ses.m3d(a.mask, a.ndata)):
+            csvio.write_inputxx(out("inputxx.csv"), a0, ax, ay, ses.xinput, a.format, append=s > 0)
+    elif a.command == "abbe":
+        img = ses.abbe(a.mask, a.polar, a.z)
+        csvio.write_image(out("emint.csv" if a.polar == "X" else "emintY.csv"), img, a.format)
+    else:
+        re0, im0, rex, imx, rey, imy = csvio.read_inputxx(a.inputxx)[0]
+        if a.command == "linear":
+            il, it = ses.linear(a.z, a.mask, re0, im0, rex, imx, rey, imy)
+            csvio.write_image(out("intlinear.csv"), il, a.format)
+            csvio.write_image(out("intft.csv"), it, a.format)
+        else:
+            img = ses.socs(a.z, a.mask, re0 + 1j * im0, rex + 1j * imx, rey + 1j * imy)
+            csvio.write_image(out("intsocs.csv"), img, a.format)
+    print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -46,7 +46,7 @@ _SIGS = {
     "euvl_multilayer": [c_void_p, c_char, ctypes.POINTER(EmParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "euvl_amp_s": [c_void_p, c_char, ctypes.POINTER(EmParams), P_int, P_int, c_int, c_int, c_int, c_void_p, c_int,
                    P_double, P_double, c_void_p, P_int],
-    "euvl_abbe_image": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_int, P_int, c_int, P_double,
+    "euvl_abbe_image": [c_void_p, c_char, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_int, P_int, c_int, P_double,
                         c_int, P_int, c_int, P_int, c_double, c_double, c_double, c_double, c_int, c_int, c_double,
                         c_void_p],
     "euvl_m3d_fit": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, c_void_p, c_int, c_int, c_int,

@@ -268,3 +268,21 @@ def high_na(ndiv, K, source, ndivs=0, image=(0, 0)):
     return Config(ndivx=ndiv, ndivy=ndiv, mx=4, my=8, na=0.55, theta0=-5.355, n_abs=0.947 + 0.031j,
                   stack=HIGH_NA_STACK, orders=("rect", K), source=Source(**{**source, "ndivs": ndivs}),
                   image=image)
+
+
+ANNULAR_C1 = dict(kind=1, sigma_out=0.8, sigma_in=0.3)
+DIPOLE_ANNULAR_C2 = dict(kind=2, sigma_out=0.9, sigma_in=0.5, open_angle=60.0, extra=((1, 0.8, 0.7, 90.0),))
+
+
+def preset(name):
+    """Named problems: 'reference' (the reference drivers' own), 'c1' (512 nm
+    clip, +-10 orders, annular 0.3-0.8 on 2 x 2 divisions, 256 x 256 image) and
+    'c2' (1024 nm clip, +-20 orders, X-dipole 0.5-0.9 / 60 deg + annular
+    0.7-0.8 on 2 x 2 divisions, 256 x 256 image)."""
+    if name == "reference":
+        return reference_default()
+    if name == "c1":
+        return high_na(512, 10, ANNULAR_C1, ndivs=2, image=(256, 256))
+    if name == "c2":
+        return high_na(1024, 20, DIPOLE_ANNULAR_C2, ndivs=2, image=(256, 256))
+    raise ValueError(f"unknown preset {name!r} (reference, c1, c2)")

@@ -85,7 +85,12 @@ struct euvl_handle_s {
   // optional per-launch timing of the GEMM kernel (euvl_gemm_stats)
   bool prof = false;
   std::vector<hipEvent_t> ev_pool;
-  std::vector<std::pair<int, double>> ev_used;  // (event index pair start, flops)
+  struct GemmRecord {
+    int ev;        // index of the start event (the end event follows)
+    double flops;  // 8 m n k batch
+    double bytes;  // A, B read once, C written (and read when beta != 0)
+  };
+  std::vector<GemmRecord> ev_used;
   size_t ev_next = 0;
 };
 

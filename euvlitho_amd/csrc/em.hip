@@ -22,10 +22,10 @@
 #include <vector>
 
 #include "ew.h"
+#include "optics.h"
 
 namespace {
 
-const double PI = 3.14159265358979323846;
 const double2 ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
 
 __device__ inline int mask_bit(const uint8_t* bits, int64_t idx) { return (bits[idx >> 3] >> (7 - (idx & 7))) & 1; }
@@ -235,13 +235,6 @@ __global__ void ax_scatter_kernel(int n, int ninc, int batch, const int* idx, co
   }
 }
 
-inline dim3 grid1(int64_t total) {
-  int64_t b = (total + 255) / 256;
-  if (b > 65536) b = 65536;
-  if (b < 1) b = 1;
-  return dim3((unsigned)b);
-}
-
 int check_params(const euvl_em_params* p) {
   if (!p || p->ndivx <= 0 || p->ndivy <= 0 || p->n_layers < 2 || !p->h_layer_eps || !p->h_layer_d || p->k <= 0.0 ||
       p->dx <= 0.0 || p->dy <= 0.0) {
@@ -296,21 +289,6 @@ extern "C" int euvl_mask_spectrum(euvl_handle h, const uint8_t* d_bits, int ndiv
   double2* t = (double2*)euvl::ws_take(h, tw);
   return euvl::mask_spectrum(h, d_bits, ndivx, ndivy, n_clips, lw, mw, (double2*)d_out, s, t);
 }
-
-namespace {
-
-struct Upload {
-  // small host arrays packed into one device block
-  std::vector<char> host;
-  size_t add(const void* p, size_t bytes) {
-    size_t off = (host.size() + 255) & ~size_t(255);
-    host.resize(off + bytes);
-    memcpy(host.data() + off, p, bytes);
-    return off;
-  }
-};
-
-}  // namespace
 
 extern "C" int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n, const double* d_kx,
                                const double* d_ky, double* d_uu, double* d_ub) {

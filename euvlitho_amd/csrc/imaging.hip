@@ -12,56 +12,17 @@
 #include <vector>
 
 #include "ew.h"
+#include "optics.h"
 
 namespace {
 
-const double PI = 3.14159265358979323846;
 
 struct AbbeSetup {
   double k, dx, dy, na, kx0, ky0, z0, z;
+  int pol;  // 0: X (Ax = amp, Ay = 0), 1: Y (Ay = amp, Ax = 0), intensity.cpp:295-305
   int mx, my, lsmx, lsmy, nsx, nsy, lmax, mmax, nrange;
   int lpx, lpy;  // spectrum half-widths
 };
-
-// rotate() of intensity.cpp:423-465 (== electro_field.polarization_rotation):
-// first column R(:,0) only is needed for X polarisation (Ay = 0).
-__device__ void rotate_col0(double k, int MX, int MY, double px, double py, double sx0, double sy0, double R[3]) {
-  double s0[3] = {sx0, sy0, -sqrt(k * k - sx0 * sx0 - sy0 * sy0)};
-  double p[3] = {px, py, 0.0};
-  double pp[3] = {MX * px, MY * py, -sqrt(k * k - MX * px * MX * px - MY * py * MY * py)};
-  const double ez[3] = {0.0, 0.0, 1.0};
-  double es[3], esp[3], em[3], emp[3];
-  auto cross = [](const double* a, const double* b, double* c) {
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
-  };
-  auto norm = [](const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
-  if ((px * px + py * py) > 0.00001) {
-    cross(p, s0, es);
-    double n = norm(es);
-    for (int i = 0; i < 3; ++i) es[i] /= n;
-    cross(pp, ez, esp);
-    n = norm(esp);
-    for (int i = 0; i < 3; ++i) esp[i] /= n;
-    double ps0[3] = {p[0] + s0[0], p[1] + s0[1], p[2] + s0[2]};
-    n = norm(ps0);
-    for (int i = 0; i < 3; ++i) ps0[i] /= n;
-    cross(es, ps0, em);
-    cross(esp, pp, emp);
-    for (int i = 0; i < 3; ++i) emp[i] /= k;
-  } else {
-    cross(ez, s0, es);
-    double n = norm(es);
-    for (int i = 0; i < 3; ++i) es[i] /= n;
-    for (int i = 0; i < 3; ++i) esp[i] = -es[i];
-    cross(es, s0, em);
-    for (int i = 0; i < 3; ++i) em[i] /= k;
-    const double mez[3] = {0.0, 0.0, -1.0};
-    cross(esp, mez, emp);
-  }
-  for (int i = 0; i < 3; ++i) R[i] = esp[i] * es[0] + emp[i] * em[0];
-}
 
 // Pupil fields: F[s][c][l+lpx][m+lpy] (c = x, y, z) for every source point s and pupil point i.
 __global__ void abbe_field_kernel(AbbeSetup a, const double2* Ax, const int* modemap, const double* off,
@@ -72,7 +33,7 @@ __global__ void abbe_field_kernel(AbbeSetup a, const double2* Ax, const int* mod
     const int s = (int)(e / ncut), i = (int)(e % ncut);
     const int o = src[3 * s], l0 = src[3 * s + 1], m0 = src[3 * s + 2];
     const int lin = pupil[2 * i], min_ = pupil[2 * i + 1];
-    const double sx0 = off[4 * o], fx = off[4 * o + 2], fy = off[4 * o + 3];
+    const double sx0 = off[4 * o], sy0 = off[4 * o + 1], fx = off[4 * o + 2], fy = off[4 * o + 3];
     const double kxn = 2.0 * PI / a.dx * fx + 2.0 * PI / a.dx * l0 + 2.0 * PI * lin / a.dx;
     const double kyn = 2.0 * PI / a.dy * fy + 2.0 * PI / a.dy * m0 + 2.0 * PI * min_ / a.dy;
     const double p2 = a.mx * a.mx * kxn * kxn + a.my * a.my * kyn * kyn;
@@ -85,15 +46,16 @@ __global__ void abbe_field_kernel(AbbeSetup a, const double2* Ax, const int* mod
       const int n = modemap[(l + a.lmax) * (2 * a.mmax + 1) + (m + a.mmax)];
       if (n >= 0) amp = Ax[(((int64_t)o * a.nsx + ls) * a.nsy + ms) * a.nrange + n];
     }
-    const double kx = sx0 + 2.0 * PI / a.dx * l0;
-    const double2 A = cscale(amp, 1.0 / sqrt(a.k * a.k - kx * kx));
+    // X: Ax = amp / sqrt(k^2 - kx^2) with kx = sx0 + 2 pi l0 / dx; Y: Ay = amp / sqrt(k^2 - ky^2)
+    const double kin = a.pol ? sy0 + 2.0 * PI / a.dy * m0 : sx0 + 2.0 * PI / a.dx * l0;
+    const double2 A = cscale(amp, 1.0 / sqrt(a.k * a.k - kin * kin));
     double R[3];
-    rotate_col0(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, R);
+    rotate_col(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, a.pol, R);
     const double ph = ((kxn + a.kx0) * (kxn + a.kx0) + (kyn + a.ky0) * (kyn + a.ky0)) / 2.0 / a.k * a.z0 +
                       p2 / 2.0 / a.k * a.z;
     double sp, cp;
     sincos(ph, &sp, &cp);
-    // E_c = i k R(c,0) A, times the phase
+    // E_c = i k R(c, pol) A, times the phase
     const double2 ikA = make_double2(-a.k * A.y, a.k * A.x);
     const double2 t = cmul(ikA, make_double2(cp, sp));
     const int64_t base = (int64_t)s * 3 * nl * nm + (int64_t)(lin + a.lpx) * nm + (min_ + a.lpy);
@@ -255,7 +217,7 @@ __global__ void linear_field_kernel(AbbeSetup a, LinearSetup L, const double2* f
     Al = cadd(Al, cscale(ay[i], ms + mp / 2.0));
     Al = cscale(Al, inv);
     double R[3];
-    rotate_col0(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, R);
+    rotate_col(a.k, a.mx, a.my, kxn, kyn, a.kx0, a.ky0, 0, R);
     const double phz = ((kxn + a.kx0) * (kxn + a.kx0) + (kyn + a.ky0) * (kyn + a.ky0)) / (2.0 * a.k) * a.z0 +
                        p2 / (2.0 * a.k) * a.z;
     double sz, cz;
@@ -276,22 +238,6 @@ __global__ void zero_d_kernel(int64_t count, double* out) {
     out[e] = 0.0;
 }
 
-inline dim3 grid1(int64_t total) {
-  int64_t b = (total + 255) / 256;
-  if (b > 65536) b = 65536;
-  if (b < 1) b = 1;
-  return dim3((unsigned)b);
-}
-
-struct Upload {
-  std::vector<char> host;
-  size_t add(const void* p, size_t bytes) {
-    size_t off = (host.size() + 255) & ~size_t(255);
-    host.resize(off + bytes);
-    if (bytes) memcpy(host.data() + off, p, bytes);
-    return off;
-  }
-};
 
 }  // namespace
 
@@ -299,13 +245,14 @@ namespace euvl {
 int fill(euvl_handle h, int64_t count, double2* out, double2 v);
 }
 
-extern "C" int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const double* d_Ax, int nrange, int lmax,
+extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* p, const double* d_Ax, int nrange,
+                               int lmax,
                                int mmax, const int* h_modemap, int n_off, const double* h_off, int n_src,
                                const int* h_src, int ncut, const int* h_pupil, double kx0, double ky0, double z0,
                                double z, int nx, int ny, double weight, double* d_img) {
   using namespace euvl;
   if (!h || !p || !d_Ax || !h_modemap || n_off <= 0 || !h_off || n_src < 0 || ncut <= 0 || !h_pupil || nx <= 0 ||
-      ny <= 0 || !d_img) {
+      ny <= 0 || !d_img || (polar != 'X' && polar != 'Y')) {
     set_error("euvl_abbe_image: bad argument");
     return EUVL_ERR_ARG;
   }
@@ -321,7 +268,7 @@ extern "C" int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const dou
     lpy = std::max(lpy, std::abs(h_pupil[2 * i + 1]));
   }
   const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
-  AbbeSetup a;
+  AbbeSetup a{};
   a.k = p->k;
   a.dx = p->dx;
   a.dy = p->dy;
@@ -330,6 +277,7 @@ extern "C" int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const dou
   a.ky0 = ky0;
   a.z0 = z0;
   a.z = z;
+  a.pol = polar == 'Y' ? 1 : 0;
   a.mx = p->mx;
   a.my = p->my;
   a.lsmx = p->lsmax_x;

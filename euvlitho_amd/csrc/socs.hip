@@ -14,66 +14,9 @@
 #include <vector>
 
 #include "ew.h"
+#include "optics.h"
 
 namespace {
-
-const double PI = 3.14159265358979323846;
-
-struct Upload {
-  std::vector<char> host;
-  size_t add(const void* p, size_t bytes) {
-    size_t off = (host.size() + 255) & ~size_t(255);
-    host.resize(off + bytes);
-    if (bytes) memcpy(host.data() + off, p, bytes);
-    return off;
-  }
-};
-
-inline dim3 grid1(int64_t total) {
-  int64_t b = (total + 255) / 256;
-  if (b > 65536) b = 65536;
-  if (b < 1) b = 1;
-  return dim3((unsigned)b);
-}
-
-// rotate() first column (intensity.cpp:423-465 / electro_field.py:5-40)
-__device__ void rot_col0(double k, int MX, int MY, double px, double py, double sx0, double sy0, double R[3]) {
-  double s0[3] = {sx0, sy0, -sqrt(k * k - sx0 * sx0 - sy0 * sy0)};
-  double p[3] = {px, py, 0.0};
-  double pp[3] = {MX * px, MY * py, -sqrt(k * k - MX * px * MX * px - MY * py * MY * py)};
-  const double ez[3] = {0.0, 0.0, 1.0};
-  double es[3], esp[3], em[3], emp[3];
-  auto cross = [](const double* a, const double* b, double* c) {
-    c[0] = a[1] * b[2] - a[2] * b[1];
-    c[1] = a[2] * b[0] - a[0] * b[2];
-    c[2] = a[0] * b[1] - a[1] * b[0];
-  };
-  auto norm = [](const double* a) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); };
-  if ((px * px + py * py) > 0.00001) {
-    cross(p, s0, es);
-    double n = norm(es);
-    for (int i = 0; i < 3; ++i) es[i] /= n;
-    cross(pp, ez, esp);
-    n = norm(esp);
-    for (int i = 0; i < 3; ++i) esp[i] /= n;
-    double ps0[3] = {p[0] + s0[0], p[1] + s0[1], p[2] + s0[2]};
-    n = norm(ps0);
-    for (int i = 0; i < 3; ++i) ps0[i] /= n;
-    cross(es, ps0, em);
-    cross(esp, pp, emp);
-    for (int i = 0; i < 3; ++i) emp[i] /= k;
-  } else {
-    cross(ez, s0, es);
-    double n = norm(es);
-    for (int i = 0; i < 3; ++i) es[i] /= n;
-    for (int i = 0; i < 3; ++i) esp[i] = -es[i];
-    cross(es, s0, em);
-    for (int i = 0; i < 3; ++i) em[i] /= k;
-    const double mez[3] = {0.0, 0.0, -1.0};
-    cross(esp, mez, emp);
-  }
-  for (int i = 0; i < 3; ++i) R[i] = esp[i] * es[0] + emp[i] * em[0];
-}
 
 struct TccSetup {
   double k, dx, dy, na, kx0, ky0, z0, z;
@@ -96,7 +39,7 @@ __global__ void tcc_phase_kernel(TccSetup a, const int* pupil, int ncut, const d
     double2 v[3] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
     if (p2 <= (a.k * a.na) * (a.k * a.na)) {
       double R[3];
-      rot_col0(a.k, a.mx, a.my, ksx, ksy, a.kx0, a.ky0, R);
+      rotate_col(a.k, a.mx, a.my, ksx, ksy, a.kx0, a.ky0, 0, R);
       const double ph = ((ksx + a.kx0) * (ksx + a.kx0) + (ksy + a.ky0) * (ksy + a.ky0)) / 2.0 / a.k * a.z0 +
                         p2 / 2.0 / a.k * a.z;
       double sp, cp;

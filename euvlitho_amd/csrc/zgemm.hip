@@ -201,7 +201,10 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev + 1], h->stream));
-    h->ev_used.push_back({ev, 8.0 * (double)m * (double)n * (double)(k > 0 ? k : 0) * batch});
+    const double kk = k > 0 ? k : 0, nb = batch;
+    const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
+    const double bytes = 16.0 * (m * kk * (sa ? nb : 1.0) + kk * n * (sb ? nb : 1.0) + (double)m * n * nb * (rd_c ? 2 : 1));
+    h->ev_used.push_back({ev, 8.0 * (double)m * (double)n * kk * nb, bytes});
   }
   return EUVL_OK;
 }
@@ -230,16 +233,18 @@ extern "C" int euvl_gemm_stats(euvl_handle h, int enable, double* out) {
   if (!h) return EUVL_ERR_ARG;
   if (out) {
     EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-    double ms = 0.0, flops = 0.0;
+    double ms = 0.0, flops = 0.0, bytes = 0.0;
     for (auto& u : h->ev_used) {
       float t = 0.0f;
-      EUVL_HIP_CHECK(hipEventElapsedTime(&t, h->ev_pool[u.first], h->ev_pool[u.first + 1]));
+      EUVL_HIP_CHECK(hipEventElapsedTime(&t, h->ev_pool[u.ev], h->ev_pool[u.ev + 1]));
       ms += t;
-      flops += u.second;
+      flops += u.flops;
+      bytes += u.bytes;
     }
     out[0] = (double)h->ev_used.size();
     out[1] = ms;
     out[2] = flops;
+    out[3] = bytes;
   }
   h->ev_used.clear();
   h->ev_next = 0;

@@ -4,6 +4,10 @@ of one clip, mirroring the reference drivers
   ampS            cnn/include/header.h:1208-1466       -> Engine.amp_s
   m3d.cpp main    cnn/intensity/m3d/m3d.cpp:28-569      -> Engine.m3d
   intensity.cpp   cnn/intensity/abbe/intensity.cpp:35-421 -> Engine.abbe
+  reflectivity.py cnn/intensity/linear/reflectivity.py:4-17 -> Engine.reflect_amplitude
+  linear.py       cnn/intensity/linear/linear.py:10-225   -> Engine.linear
+  eigenfunction.py cnn/intensity/stccsocs/eigenfunction.py:5-179 -> Engine.eigenfunction
+  socs.py         cnn/intensity/stccsocs/socs.py:12-204   -> Engine.socs
 
 Device memory and streams come from PyTorch (ROCm); all arithmetic runs in
 libeuvl's HIP kernels.
@@ -41,7 +45,7 @@ class Engine:
             eps.ctypes.data_as(_lib.P_double), thk.ctypes.data_as(_lib.P_double))
         self.lidx, self.midx, self.L, self.M = cfg.order_set()
         self.modemap = Config.mode_map(self.lidx, self.midx, self.L, self.M)
-        self._blank = None
+        self._blank = {}
         self.last_info = None
 
     def close(self):
@@ -103,9 +107,9 @@ class Engine:
 
     def blank_amplitudes(self, polar="X"):
         """vcxx, abxx: blank vacuum / absorber amplitudes on the narrow order set at
-        the central offset (m3d.cpp:211-258); cached per engine."""
-        if self._blank is not None:
-            return self._blank
+        the central offset (m3d.cpp:211-258); cached per engine and polarisation."""
+        if polar in self._blank:
+            return self._blank[polar]
         cfg = self.cfg
         orders = cfg.order_set(cfg.narrow_orders)
         n = cfg.ndivx * cfg.ndivy
@@ -126,8 +130,8 @@ class Engine:
                     if n_ >= 0:
                         vcxx[is_, js] = vch[is_, js, n_]
                         abxx[is_, js] = abh[is_, js, n_]
-        self._blank = (vcxx, abxx)
-        return self._blank
+        self._blank[polar] = (vcxx, abxx)
+        return self._blank[polar]
 
     # --------------------------------------------------------------- M3D params
     def m3d(self, bits, Ax0=None):
@@ -157,11 +161,18 @@ class Engine:
         return a0, ax, ay
 
     # ----------------------------------------------------------- Abbe (EM amps)
-    def abbe(self, bits, z=0.0, Ax=None):
-        """Rigorous Abbe image (nx, ny) of one clip, X polarisation (intensity.cpp main)."""
+    def abbe(self, bits, z=0.0, Ax=None, polar="X"):
+        """Rigorous Abbe image (nx, ny) of one clip (intensity.cpp main): polar
+        'X' gives emint.csv, 'Y' emintY.csv.  z: defocus, or a sequence of
+        defocus values (the z loop of intensity.cpp:219) -> (nz, nx, ny); the
+        EM amplitudes do not depend on z and are solved once."""
         cfg = self.cfg
+        if polar not in ("X", "Y"):
+            raise ValueError("polar must be 'X' or 'Y'")
         if Ax is None:
-            Ax = self.amp_s(bits, "X")
+            Ax = self.amp_s(bits, polar)
+        if np.ndim(z) > 0:
+            return torch.stack([self.abbe(bits, float(zz), Ax, polar) for zz in z])
         offs = cfg.offsets()
         src = []
         divs = cfg.source_points()
@@ -177,7 +188,8 @@ class Engine:
         sr, srp = _lib.arr_i(src)
         pu, pup_p = _lib.arr_i(pup)
         self._bind_stream()
-        _lib.check(self.lib.euvl_abbe_image(self.h, ctypes.byref(self.params), _lib.ptr(Ax), len(self.lidx), self.L,
+        _lib.check(self.lib.euvl_abbe_image(self.h, polar.encode(), ctypes.byref(self.params), _lib.ptr(Ax),
+                                            len(self.lidx), self.L,
                                             self.M, mmp, len(offs), ofp, len(src), srp, len(lin), pup_p, cfg.kx0,
                                             cfg.ky0, cfg.z0, float(z), nx, ny, 1.0 / max(len(src), 1),
                                             _lib.ptr(img)), "euvl_abbe_image")

@@ -47,9 +47,11 @@ const char* euvl_last_error(void);
 int64_t euvl_workspace_bytes(euvl_handle h);
 /* Per-launch timing of the GEMM kernel with HIP events on the handle's
  * stream.  If out is non-null, synchronises and writes {launches, total
- * milliseconds, algorithmic flops (8 m n k per complex GEMM)} of the launches
- * recorded since the last call; then clears them and enables (1) or disables
- * (0) recording.  No reference counterpart (instrumentation). */
+ * milliseconds, algorithmic flops (8 m n k per complex GEMM), algorithmic
+ * bytes (A and B read once, C written, and read too when beta != 0)} of the
+ * launches recorded since the last call (out holds 4 doubles); then clears
+ * them and enables (1) or disables (0) recording.  No reference counterpart
+ * (instrumentation). */
 int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 
 /* ---------------------------------------------------------------------------
@@ -154,17 +156,19 @@ int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p,
  * ------------------------------------------------------------------------- */
 
 /* Rigorous Abbe image accumulation for n_src source points of one
- * polarisation (X) from EM amplitudes, the per-source-point loop of
- * intensity.cpp:276-388 (field, rotate(), defocus phase, unnormalised inverse
- * DFT, |Ex|^2+|Ey|^2+|Ez|^2).  Adds weight * intensity into d_img (nx x ny
- * doubles, index x*ny + y).
+ * polarisation from EM amplitudes of that polarisation, the per-source-point
+ * loop of intensity.cpp:276-388 (field, rotate(), defocus phase, unnormalised
+ * inverse DFT, |Ex|^2+|Ey|^2+|Ez|^2).  Adds weight * intensity into d_img
+ * (nx x ny doubles, index x*ny + y).
+ *   polar: 'X' (emint.csv: Ax = amp/sqrt(k^2-kx^2), column 0 of rotate()) or
+ *          'Y' (emintY.csv: Ay = amp/sqrt(k^2-ky^2), column 1)
  *   d_Ax: amplitudes, n_off x nsx x nsy x nrange complex (euvl_amp_s layout)
  *   h_src: n_src x 3 ints (offset index o, l0s, m0s); h_off: per offset
  *          (sx0, sy0, fx, fy) with fx = nsx/ndivs, fy = nsy/ndivs
  *   h_pupil: ncut x 2 ints (linput, minput)
  *   h_modemap: (2 lmax+1)(2 mmax+1) ints, mode index of (l, m) or -1
  *   kx0, ky0: incidence of rotate(); z0, z: defocus terms of intensity.cpp:353 */
-int euvl_abbe_image(euvl_handle h, const euvl_em_params* p, const double* d_Ax, int nrange,
+int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* p, const double* d_Ax, int nrange,
                     int lmax, int mmax, const int* h_modemap,
                     int n_off, const double* h_off, int n_src, const int* h_src,
                     int ncut, const int* h_pupil, double kx0, double ky0, double z0, double z,

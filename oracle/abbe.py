@@ -32,9 +32,11 @@ def rotate(k, MX, MY, px, py, sx0, sy0):
     return np.outer(esp, es[:2]) + np.outer(emp, em_[:2])
 
 
-def division_fields(p, d, Ax, divoff, l0s, m0s, lin, min_, lidx, midx, z):
+def division_fields(p, d, Ax, divoff, l0s, m0s, lin, min_, lidx, midx, z, polar="X"):
     """Pupil spectra of one division: returns list over sources of (px, py, fx, fy, fz)
-    entries restating intensity.cpp:248-368 (X polarisation)."""
+    entries restating intensity.cpp:248-368 (X: Ax = amp/sqrt(k^2-kx^2) on column 0
+    of rotate(); Y: Ay = amp/sqrt(k^2-ky^2) on column 1, intensity.cpp:295-305)."""
+    col = 0 if polar == "X" else 1
     k, dx, dy = d["k"], d["dx"], d["dy"]
     MX, MY = p["MX"], p["MY"]
     sx0, sy0, fxo, fyo = divoff
@@ -42,7 +44,7 @@ def division_fields(p, d, Ax, divoff, l0s, m0s, lin, min_, lidx, midx, z):
     pos = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(lidx, midx))}
     out = []
     for l0, m0 in zip(l0s, m0s):
-        kx = sx0 + 2.0 * PI / dx * l0
+        kin = sx0 + 2.0 * PI / dx * l0 if col == 0 else sy0 + 2.0 * PI / dy * m0
         ls, ms = l0 + d["lsmaxX"], m0 + d["lsmaxY"]
         rows = []
         for li, mi in zip(lin, min_):
@@ -54,9 +56,9 @@ def division_fields(p, d, Ax, divoff, l0s, m0s, lin, min_, lidx, midx, z):
             n = pos.get((int(l0 + li), int(m0 + mi)))
             if n is not None and 0 <= ls < d["nsourceX"] and 0 <= ms < d["nsourceY"]:
                 amp = Ax[ls, ms, n]
-            A = amp / np.sqrt(k * k - kx * kx)
+            A = amp / np.sqrt(k * k - kin * kin)
             R = rotate(k, MX, MY, kxn, kyn, kx0, ky0)
-            E = 1j * k * R[:, 0] * A
+            E = 1j * k * R[:, col] * A
             phase = np.exp(1j * ((kxn + kx0) ** 2 + (kyn + ky0) ** 2) / 2.0 / k * p["z0"]
                            + 1j * (MX * MX * kxn * kxn + MY * MY * kyn * kyn) / 2.0 / k * z)
             rows.append((int(li), int(mi), E * phase))
@@ -64,7 +66,7 @@ def division_fields(p, d, Ax, divoff, l0s, m0s, lin, min_, lidx, midx, z):
     return out
 
 
-def abbe_image(p, d, mask2d, lidx, midx, LMAX, MMAX, divisions, nx, ny, z=0.0, amps=None):
+def abbe_image(p, d, mask2d, lidx, midx, LMAX, MMAX, divisions, nx, ny, z=0.0, amps=None, polar="X"):
     """Mean over all source points of |Ex|^2+|Ey|^2+|Ez|^2 on an nx x ny grid
     (intensity.cpp:219-408).  divisions: list of ((sx0, sy0, fx, fy), l0s, m0s).
     amps: optional precomputed Ax per division."""
@@ -72,8 +74,8 @@ def abbe_image(p, d, mask2d, lidx, midx, LMAX, MMAX, divisions, nx, ny, z=0.0, a
     img = np.zeros((nx, ny))
     total = 0
     for q, (off, l0s, m0s) in enumerate(divisions):
-        Ax = amps[q] if amps is not None else em.amp_s("X", mask2d, p, d, lidx, midx, LMAX, MMAX, off[0], off[1])
-        for rows in division_fields(p, d, Ax, off, l0s, m0s, lin, min_, lidx, midx, z):
+        Ax = amps[q] if amps is not None else em.amp_s(polar, mask2d, p, d, lidx, midx, LMAX, MMAX, off[0], off[1])
+        for rows in division_fields(p, d, Ax, off, l0s, m0s, lin, min_, lidx, midx, z, polar):
             F = np.zeros((3, nx, ny), dtype=np.complex128)
             for li, mi, f in rows:
                 F[:, li % nx, mi % ny] = f
