@@ -142,8 +142,8 @@ int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n,
  * Output d_Ax: n_off x (2 lsmax_x+1) x (2 lsmax_y+1) x nrange complex;
  * the entry [o][ls+lsmax_x][ms+lsmax_y][:] is Ax of header.h:1394-1465 for
  * the lattice incidence (ls, ms) inside the NA and 0 elsewhere.
- * h_info (optional, 4 ints per offset): Newton-Schulz iterations, expm
- * squarings, Neumann factors, 0.
+ * h_info (optional, 4 ints per offset): 0, cos/sinc doublings, Neumann
+ * factors applied, incidence columns solved.
  * Replaces ampS (cnn/include/header.h:1208-1466) with absorberS0
  * (header.h:754-1145) and the top solve (header.h:1355-1391). */
 int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p,
