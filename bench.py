@@ -11,9 +11,11 @@ parameters (24 kernels, 20 of them with the x/y slope terms, nout 64).  The
 SOCS kernels (TCC eigendecomposition on the 0.2 degree source mesh) are
 mask-independent: built once, resident on the device, timed separately.
 
-Prints one JSON line (metric, value = clips/s, roofline of the dominant
-kernel = the FP64 MFMA complex GEMM, cpu_baseline = the FP64 NumPy/LAPACK
-oracle on one clip of the same workload).
+Prints one JSON line: metric, value = clips/s of the plain FP64 form,
+roofline of the dominant kernel (the FP64 MFMA complex GEMM), cpu_baseline =
+the FP64 NumPy/LAPACK oracle on one clip of the same workload, and under
+"forms" the precision form (FP32 MFMA products, FP64 storage and the rest)
+timed on the same clips with its error against the FP64 form.
 """
 import argparse
 import json
@@ -27,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SOCS_KERNELS, SOCS_XY = 24, 20
 
 
@@ -95,6 +98,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only (profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy that scales the HBM byte counters")
     args = ap.parse_args()
@@ -106,6 +110,7 @@ def main():
     if args.copy_calibration:
         copy_calibration()
         return
+    import ctypes
     import torch
     from euvlitho_amd.engine import Engine
     from euvlitho_amd.masks import manhattan
@@ -114,50 +119,78 @@ def main():
     cfg = workload_config()
     n = args.warmup + args.steps
     masks = [manhattan(cfg.ndivx, cfg.ndivy, 0.5, seed) for seed in range(n)]
-    eng = Engine(cfg, "cuda:0")
-    bits = [eng.upload_mask(m) for m in masks]
-    eng.blank_amplitudes()  # mask-independent, resident for every clip
-    torch.cuda.synchronize()
 
-    t_k = time.perf_counter()
-    eng.eigenfunction(0.0, SOCS_KERNELS)  # mask-independent, resident for every clip
-    torch.cuda.synchronize()
-    tcc_ms = 1000.0 * (time.perf_counter() - t_k)
-
-    def step(b):
-        Ax = eng.amp_s(b, "X")
-        a0, ax, ay = eng.m3d(b, Ax0=Ax[0])
-        img = eng.abbe(b, Ax=Ax)
-        socs = eng.socs(b, a0, ax, ay, nsocs=SOCS_KERNELS, nsocsxy=SOCS_XY)
-        return a0, img, socs
-
-    for i in range(args.warmup):
-        step(bits[i])
-    torch.cuda.synchronize()
-    lat = []
-    t0 = time.perf_counter()
-    for i in range(args.warmup, n):
-        ts = time.perf_counter()
-        step(bits[i])
+    def run_form(precision):
+        """Time one form: warmup, then `steps` clips each bracketed by a device
+        synchronisation; then one more clip with per-GEMM HIP events."""
+        eng = Engine(cfg, "cuda:0", precision=precision)
+        bits = [eng.upload_mask(m) for m in masks]
+        eng.blank_amplitudes()  # mask-independent, resident for every clip
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - ts)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = 1000.0 * wall / args.steps
+        t_k = time.perf_counter()
+        eng.eigenfunction(0.0, SOCS_KERNELS)  # mask-independent, resident for every clip
+        torch.cuda.synchronize()
+        tcc_ms = 1000.0 * (time.perf_counter() - t_k)
 
-    # GEMM roofline: HIP events around every GEMM launch of one extra clip
-    import ctypes
-    stats = (ctypes.c_double * 4)()
-    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
-    step(bits[0])
-    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
-    launches, gemm_ms, gemm_flops, gemm_bytes = stats[0], stats[1], stats[2], stats[3]
+        def step(b):
+            Ax = eng.amp_s(b, "X")
+            a0, ax, ay = eng.m3d(b, Ax0=Ax[0])
+            img = eng.abbe(b, Ax=Ax)
+            socs = eng.socs(b, a0, ax, ay, nsocs=SOCS_KERNELS, nsocsxy=SOCS_XY)
+            return Ax, a0, img, socs
+
+        for i in range(args.warmup):
+            step(bits[i])
+        torch.cuda.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for i in range(args.warmup, n):
+            ts = time.perf_counter()
+            step(bits[i])
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - ts)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        # GEMM roofline: HIP events on the library's stream around every GEMM launch of one extra clip
+        stats = (ctypes.c_double * 4)()
+        _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
+        outs = [t.cpu().numpy() for t in step(bits[args.warmup])]
+        _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
+        eng.close()
+        return dict(lat=lat, wall=wall, tcc_ms=tcc_ms, outs=outs, launches=stats[0], gemm_ms=stats[1],
+                    gemm_flops=stats[2], gemm_bytes=stats[3])
+
+    f64 = run_form(_lib.PREC_FP64)
+    mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
+    ms = 1000.0 * f64["wall"] / args.steps
+    launches, gemm_ms, gemm_flops, gemm_bytes = f64["launches"], f64["gemm_ms"], f64["gemm_flops"], f64["gemm_bytes"]
     achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     traffic = measured_traffic()
+    lat = f64["lat"]
+    mixed = None
+    if mix is not None:
+        # mixed form: its error against the FP64 form on the same clip (amplitudes, a0, Abbe, SOCS)
+        (A64, a064, I64, S64), (A32, a032, I32, S32) = f64["outs"], mix["outs"]
+        mix_ach = mix["gemm_flops"] / (mix["gemm_ms"] * 1e-3) / 1e12 if mix["gemm_ms"] > 0 else 0.0
+        mixed = {
+            "value": args.steps / mix["wall"], "unit": "clips/s", "ms_per_step": 1000.0 * mix["wall"] / args.steps,
+            "p50_ms": 1000.0 * float(np.percentile(mix["lat"], 50)),
+            "p99_ms": 1000.0 * float(np.percentile(mix["lat"], 99)),
+            "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
+            "err_vs_fp64": {"abbe_image_of_peak": float(np.abs(I32 - I64).max() / I64.max()),
+                            "socs_image_of_peak": float(np.abs(S32 - S64).max() / np.abs(S64).max()),
+                            "amplitudes_of_max": float(np.abs(A32 - A64).max() / np.abs(A64).max()),
+                            "a0_of_max": float(np.abs(a032 - a064).max() / np.abs(a064).max())},
+            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel<F32>", "achieved": mix_ach,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": mix_ach / FP32_PEAK_TFLOPS,
+                         "avg_launch_us": 1000.0 * mix["gemm_ms"] / max(mix["launches"], 1),
+                         "gemm_ms_per_clip": mix["gemm_ms"]},
+        }
+    tcc_ms = f64["tcc_ms"]
 
     line = {
         "metric": "EM+Abbe mask clips/s at <=1e-3 intensity err vs FP64; p50/p99 clip latency",
-        "value": args.steps / wall,
+        "value": args.steps / f64["wall"],
         "unit": "clips/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -182,6 +215,7 @@ def main():
                      "launches_per_clip": int(launches),
                      "avg_launch_us": 1000.0 * gemm_ms / max(launches, 1),
                      "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms},
+        "forms": {"mixed": mixed} if mixed else {},
     }
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))

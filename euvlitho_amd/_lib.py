@@ -18,6 +18,12 @@ class EuvlError(RuntimeError):
     pass
 
 
+# euvl_set_precision bits (include/euvl.h)
+PREC_FP32_MATFUN, PREC_FP32_COUPLING, PREC_FP32_LU, PREC_FP32_GEMM = 1, 2, 4, 8
+PREC_FP64 = 0
+PREC_MIXED = PREC_FP32_MATFUN | PREC_FP32_COUPLING | PREC_FP32_LU
+
+
 class EmParams(ctypes.Structure):
     _fields_ = [
         ("ndivx", c_int), ("ndivy", c_int), ("mx", c_int), ("my", c_int),
@@ -36,6 +42,7 @@ _SIGS = {
     "euvl_set_stream": [c_void_p, c_void_p],
     "euvl_workspace_bytes": [c_void_p],
     "euvl_gemm_stats": [c_void_p, c_int, P_double],
+    "euvl_set_precision": [c_void_p, c_int],
     "euvl_zgemm_batched": [c_void_p, c_int, c_int, c_int, P_double, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                            c_int64, P_double, c_void_p, c_int64, c_int64, P_double, c_int],
     "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],

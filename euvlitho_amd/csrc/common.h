@@ -92,7 +92,22 @@ struct euvl_handle_s {
   };
   std::vector<GemmRecord> ev_used;
   size_t ev_next = 0;
+  // GEMM compute precision: a GEMM runs on FP32 MFMA when the bit of the
+  // current scope is set in prec_mask (euvl_set_precision), else on FP64 MFMA
+  int prec_mask = 0;
+  int prec_scope = 0;
 };
+
+namespace euvl {
+// Marks the GEMMs issued while it lives as belonging to one precision scope
+// (EUVL_PREC_* bit); restores the enclosing scope on exit.
+struct PrecScope {
+  euvl_handle h;
+  int saved;
+  PrecScope(euvl_handle h_, int scope) : h(h_), saved(h_->prec_scope) { h->prec_scope = scope; }
+  ~PrecScope() { h->prec_scope = saved; }
+};
+}  // namespace euvl
 
 namespace euvl {
 // Ensure the workspace holds at least `bytes`; invalidates earlier pointers.

@@ -432,6 +432,8 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   int ndbl = 0;
   EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl));
 
+  // products of the coupling and the top solve (the LU keeps its own scope)
+  PrecScope scope(h, EUVL_PREC_FP32_COUPLING);
   // Minv = (I+G)(I+G^2)(I+G^4)... ~ (I - G)^-1 with M = i k (I - G); ||G|| is ~0.1 for these stacks.
   double2* Gp = Mx[0];  // D no longer needed
   double2* Fm = Mx[5];

@@ -18,6 +18,7 @@ namespace euvl {
 // D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
              int* doublings) {
+  PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
   const size_t mat = sizeof(double2) * (size_t)n * n * batch;
   double2* s[6];
   for (int i = 0; i < 6; ++i) {

@@ -273,18 +273,27 @@ __global__ void socs_combine_kernel(int nout, int n0, int nxy, const double2* fl
 }
 
 // S[u][v] = (1/nout^2) sum_{x,y} small[x][y] e^{-2 pi i (u x + v y)/nout}, u, v in [-nout/2, nout/2)
-__global__ void socs_spec_kernel(int nout, const double* small, const double2* twm, double2* S) {
+// Centred spectrum of the small image, S[u][v] = (1/nout^2) sum_{x,y}
+// small[x][y] e^{-2 pi i (u x + v y)/nout}, u, v in [-nout/2, nout/2) (the
+// fft2 / nout^2 of socs.py:143), in two separable passes:
+// R[x][v] = sum_y small[x][y] e^{-2 pi i v y/nout}
+__global__ void socs_spec_y_kernel(int nout, const double* small, const double2* twm, double2* R) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nout * nout) return;
+  const int x = e / nout, v = e % nout - nout / 2;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int y = 0; y < nout; ++y) acc = cadd(acc, cscale(twm[(((int64_t)v * y) % nout + nout) % nout], small[x * nout + y]));
+  R[e] = acc;
+}
+
+// S[u][v] = (1/nout^2) sum_x e^{-2 pi i u x/nout} R[x][v]
+__global__ void socs_spec_x_kernel(int nout, const double2* R, const double2* twm, double2* S) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nout * nout) return;
   const int ui = e / nout, vi = e % nout;
-  const int u = ui - nout / 2, v = vi - nout / 2;
+  const int u = ui - nout / 2;
   double2 acc = make_double2(0.0, 0.0);
-  for (int x = 0; x < nout; ++x) {
-    double2 row = make_double2(0.0, 0.0);
-    for (int y = 0; y < nout; ++y)
-      row = cadd(row, cscale(twm[(((int64_t)v * y) % nout + nout) % nout], small[x * nout + y]));
-    acc = cadd(acc, cmul(twm[(((int64_t)u * x) % nout + nout) % nout], row));
-  }
+  for (int x = 0; x < nout; ++x) acc = cadd(acc, cmul(twm[(((int64_t)u * x) % nout + nout) % nout], R[x * nout + vi]));
   S[e] = cscale(acc, 1.0 / ((double)nout * nout));
 }
 
@@ -501,13 +510,14 @@ extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const dou
   const size_t b_small = sizeof(double) * nout * nout;
   const size_t b_S = sizeof(double2) * nout * nout;
   const size_t b_T = sizeof(double2) * (size_t)nout * ny;
-  EUVL_TRY(ws_reserve(h, up.host.size() + b_amps + b_F + b_G + b_fld + b_small + b_S + b_T + 16384));
+  EUVL_TRY(ws_reserve(h, up.host.size() + b_amps + b_F + b_G + b_fld + b_small + 2 * b_S + b_T + 16384));
   char* blk = (char*)ws_take(h, up.host.size());
   double2* amps = (double2*)ws_take(h, b_amps);
   double2* F = (double2*)ws_take(h, b_F);
   double2* G = (double2*)ws_take(h, b_G);
   double2* fld = (double2*)ws_take(h, b_fld);
   double* small = (double*)ws_take(h, b_small);
+  double2* Rp = (double2*)ws_take(h, b_S);
   double2* Sp = (double2*)ws_take(h, b_S);
   double2* T = (double2*)ws_take(h, b_T);
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
@@ -544,8 +554,11 @@ extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const dou
   hipLaunchKernelGGL(socs_combine_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, nsocs,
                      nsocsxy, fld, d_alpha, nk, small);
   EUVL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(socs_spec_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, small,
-                     (const double2*)(blk + o_twm), Sp);
+  hipLaunchKernelGGL(socs_spec_y_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, small,
+                     (const double2*)(blk + o_twm), Rp);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(socs_spec_x_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout,
+                     (const double2*)Rp, (const double2*)(blk + o_twm), Sp);
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(socs_up_y_kernel, grid1((int64_t)nout * ny), dim3(256), 0, h->stream, nout, ny, Sp,
                      (const double2*)(blk + o_twy), T);

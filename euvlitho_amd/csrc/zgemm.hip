@@ -1,32 +1,67 @@
-// Batched complex FP64 GEMM on CDNA4 FP64 MFMA (v_mfma_f64_16x16x4_f64).
+// Batched complex GEMM on CDNA4 MFMA, FP64 storage.
 //
-// C = alpha*A*B + beta*C + gamma*I, row-major, interleaved complex.
-// A workgroup of 4 waves (2 x 2) computes a (32 TM) x (32 TN) tile; each wave
-// owns TM x TN MFMA tiles of 16 x 16 with separate real/imag accumulators
+// C = alpha*A*B + beta*C + gamma*I, row-major, interleaved complex double.
+// A workgroup of WM x WN waves computes a (16 TM WM) x (16 TN WN) tile; each
+// wave owns TM x TN MFMA tiles of 16 x 16 with separate real/imag accumulators
 // (4 real MFMAs per complex MFMA step: C_re += A_re B_re - A_im B_im,
-// C_im += A_re B_im + A_im B_re).  K is staged through LDS in slabs of 16 with
-// a register prefetch of the next slab.  Operand layout of the f64 16x16x4
-// MFMA: lane l holds A[l%16][l/16] and B[l/16][l%16]; the 4 accumulator
-// doubles of lane l are D[4*r + l/16][l%16], r = 0..3 (measured on gfx950).
+// C_im += A_re B_im + A_im B_re).  K is staged through LDS in slabs of BK
+// with a register prefetch of the next slab.
+//
+// Two compute precisions share the kernel:
+//   F64: v_mfma_f64_16x16x4_f64, FP64 accumulation (the FP64 drop-in);
+//   F32: operands rounded to FP32 when staged into LDS, v_mfma_f32_16x16x4_f32
+//        with FP32 accumulation over K, FP64 epilogue (the precision form:
+//        twice the MFMA rate of FP64 on MI355X, half the LDS and registers).
+// Operand layout of both 16x16x4 MFMAs: lane l holds A[l%16][l/16] and
+// B[l/16][l%16].  Accumulator element r of lane l is D[4r + l/16][l%16] for
+// f64 and D[4(l/16) + r][l%16] for f32 (both measured on gfx950).
 //
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2);
 // the linear workgroup id is remapped so that every XCD works on one
-// contiguous range of tiles (shared A row panels stay in one L2).
+// contiguous range of tiles, walked in groups of 8 tile rows (shared A row
+// panels and B column panels stay in one L2).
 #include <cstdlib>
 
 #include "common.h"
 
 namespace {
 
-template <int TM, int TN, int BK, int MINB>
-__global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+struct F64 {
+  using T = double;
+  using T2 = double2;
+  using V4 = double4_t;
+  static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int lk) { return 4 * r + lk; }
+};
+
+struct F32 {
+  using T = float;
+  using T2 = float2;
+  using V4 = float4_t;
+  static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int r, int lk) { return 4 * lk + r; }
+};
+
+template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
+__global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     int M, int N, int K, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sa,
     const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* __restrict__ C,
     int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles) {
-  constexpr int BM = 32 * TM, BN = 32 * TN;
-  constexpr int QA = BM * BK / 256, QB = BN * BK / 256;  // complex values per thread per slab
-  __shared__ double2 As[BK][BM + 1];
-  __shared__ double2 Bs[BK][BN + 1];
+  constexpr int NT = 64 * WM * WN;                      // WM x WN waves
+  constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;   // workgroup tile
+  constexpr int QA = BM * BK / NT, QB = BN * BK / NT;   // complex values per thread per slab
+  static_assert(QA >= 1 && QB >= 1 && BK % QA == 0 && BN % QB == 0, "slab split");
+  using T = typename P::T;
+  using T2 = typename P::T2;
+  using V4 = typename P::V4;
+  __shared__ T2 As[BK][BM + 1];
+  __shared__ T2 Bs[BK][BN + 1];
 
   // XCD-aware remap of the linear workgroup id (bijective on [0, total))
   const int total = gridDim.x;
@@ -38,13 +73,23 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
   A += (int64_t)b * sa;
   B += (int64_t)b * sb;
   C += (int64_t)b * sc;
-  const int row0 = (tile / tiles_n) * BM;
-  const int col0 = (tile % tiles_n) * BN;
+  // grouped order: runs of GM tile rows are walked column by column, so the
+  // ~128 workgroups an XCD runs at once cover a GM x 16 block of tiles and
+  // share GM row panels of A and ~16 column panels of B in its L2 (a plain
+  // row-major walk streams every column panel of B once per tile row)
+  constexpr int GM = 8;
+  const int tiles_m = tiles / tiles_n;
+  const int group = tile / (GM * tiles_n);
+  const int first = group * GM;
+  const int gsize = min(tiles_m - first, GM);
+  const int in_group = tile - group * GM * tiles_n;
+  const int row0 = (first + in_group % gsize) * BM;
+  const int col0 = (in_group / gsize) * BN;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / WN, wn = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
 
   // global->LDS assignment: A slab BM x 16 (row-major, QA consecutive k per thread),
@@ -52,29 +97,33 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
   const int a_r = tid / (BK / QA), a_k = (tid % (BK / QA)) * QA;
   const int b_k = tid / (BN / QB), b_c = (tid % (BN / QB)) * QB;
 
-  double2 ra[QA], rb[QB];
+  T2 ra[QA], rb[QB];
   auto load_slab = [&](int k0) {
     const int gr = row0 + a_r;
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int gk = k0 + a_k + q;
-      ra[q] = (gr < M && gk < K) ? A[(int64_t)gr * lda + gk] : make_double2(0.0, 0.0);
+      const double2 v = (gr < M && gk < K) ? A[(int64_t)gr * lda + gk] : make_double2(0.0, 0.0);
+      ra[q].x = (T)v.x;
+      ra[q].y = (T)v.y;
     }
     const int gk = k0 + b_k;
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int gc = col0 + b_c + q;
-      rb[q] = (gk < K && gc < N) ? B[(int64_t)gk * ldb + gc] : make_double2(0.0, 0.0);
+      const double2 v = (gk < K && gc < N) ? B[(int64_t)gk * ldb + gc] : make_double2(0.0, 0.0);
+      rb[q].x = (T)v.x;
+      rb[q].y = (T)v.y;
     }
   };
 
-  double4_t acc_re[TM][TN], acc_im[TM][TN];
+  V4 acc_re[TM][TN], acc_im[TM][TN];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
     for (int u = 0; u < TN; ++u) {
-      acc_re[t][u] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      acc_im[t][u] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      acc_re[t][u] = (V4){0, 0, 0, 0};
+      acc_im[t][u] = (V4){0, 0, 0, 0};
     }
 
   load_slab(0);
@@ -88,8 +137,8 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
     if (k0 + BK < K) load_slab(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
-      double2 fa[TM], fb[TN];
-      double nai[TM];
+      T2 fa[TM], fb[TN];
+      T nai[TM];
 #pragma unroll
       for (int t = 0; t < TM; ++t) {
         fa[t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
@@ -103,22 +152,22 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_re[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].x, fb[u].x, acc_re[t][u], 0, 0, 0);
+          acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
 #pragma unroll
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_im[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].x, fb[u].y, acc_im[t][u], 0, 0, 0);
+          acc_im[t][u] = P::mfma(fa[t].x, fb[u].y, acc_im[t][u]);
 #pragma unroll
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_re[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[t], fb[u].y, acc_re[t][u], 0, 0, 0);
+          acc_re[t][u] = P::mfma(nai[t], fb[u].y, acc_re[t][u]);
 #pragma unroll
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_im[t][u] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[t].y, fb[u].x, acc_im[t][u], 0, 0, 0);
+          acc_im[t][u] = P::mfma(fa[t].y, fb[u].x, acc_im[t][u]);
     }
   }
 
@@ -129,10 +178,10 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
     for (int u = 0; u < TN; ++u)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int gr = row0 + wm * 16 * TM + t * 16 + r * 4 + lk;
+        const int gr = row0 + wm * 16 * TM + t * 16 + P::row(r, lk);
         const int gc = col0 + wn * 16 * TN + u * 16 + li;
         if (gr < M && gc < N) {
-          double2 v = cmul(alpha, make_double2(acc_re[t][u][r], acc_im[t][u][r]));
+          double2 v = cmul(alpha, make_double2((double)acc_re[t][u][r], (double)acc_im[t][u][r]));
           double2* cp = C + (int64_t)gr * ldc + gc;
           if (use_beta) v = cadd(v, cmul(beta, *cp));
           if (gr == gc) v = cadd(v, gamma);
@@ -141,32 +190,54 @@ __global__ __launch_bounds__(256, MINB) void zgemm_mfma_kernel(
       }
 }
 
-// kernel variants (64x64 workgroup tile, K slab 16): 0 = uncapped registers
-// (2 waves/SIMD), 1 = registers capped for >= 3 waves/SIMD (the compiler
-// reaches 4 with 126 VGPRs).  Measured on MI355X (N = 1681 x 4 / 8705):
-// 50 / 59 and 56 / 64 TFLOP/s; 128x64, 64x128 and 128x128 tiles (1 wave/SIMD)
-// reached 32-40, and a K slab of 32 48-55.  1 is the default and
-// EUVL_GEMM_TILE overrides it for tuning runs.
-int g_variant = -1;
+// FP64 kernel variants (K slab 16): 0 = 2x2 waves, 64x64, uncapped registers
+// (2 waves/SIMD); 1 = the same with registers capped for >= 3 waves/SIMD (the
+// compiler reaches 4 with 126 VGPRs); 2 = 2x4 waves (64x128); 3 = 4x4 waves
+// (128x128); 4 = 4x2 waves (128x64).  Measured on MI355X (TFLOP/s at
+// N = 1681 x 4 / 4096 / 8705): 50/59/59, 55/69/67, 54/66/64, 43/68/63,
+// 54/68/66 (earlier 2x2-wave 128x64 and 128x128 tiles with 1 wave/SIMD reached
+// 32-40, and a K slab of 32 48-55).  The FP64 kernel is bound by the MFMA
+// issue rate, not by operand bytes, so 1 stays the default; EUVL_GEMM_TILE
+// overrides it for tuning runs.
+// FP32-compute variants (waves WM x WN, each TM x TN MFMA tiles, K slab 16):
+// 0 = 2x2 waves, 64x64; 1 = 2x4 waves, 64x128; 2 = 4x2 waves, 128x64;
+// 3 = 4x4 waves, 128x128; 4 = 2x2 waves of 4x2 tiles, 128x64.
+// EUVL_GEMM_TILE32 overrides the choice by size below.
+int g_variant = -1, g_variant32 = -2;  // -2: environment not read yet, -1: by size
+
+int env_variant(const char* name, int dflt, int hi) {
+  const char* e = getenv(name);
+  int v = e ? atoi(e) : dflt;
+  return (v < 0 || v > hi) ? dflt : v;
+}
 
 int pick_variant(int, int) {
-  if (g_variant < 0) {
-    const char* e = getenv("EUVL_GEMM_TILE");
-    g_variant = e ? atoi(e) : 1;
-    if (g_variant < 0 || g_variant > 1) g_variant = 1;
-  }
+  if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 4);
   return g_variant;
 }
 
-template <int TM, int TN, int BK, int MINB>
+// FP32 compute: larger tiles halve the operand bytes per flop, which is what
+// limits this kernel once the MFMA rate doubles, but need enough workgroups
+// to fill 256 CUs.  Measured on MI355X (TFLOP/s, N x N x N x batch):
+//   1681 x 4: 64x64 75, 64x128 84, 128x128 74;  4096: 98, 109, 123;
+//   8705: 67, 92, 108.
+int pick_variant32(int m, int n, int batch) {
+  if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 4);
+  if (g_variant32 >= 0) return g_variant32;
+  if (n <= 64) return 0;
+  const int64_t big_tiles = (int64_t)((m + 127) / 128) * ((n + 127) / 128) * batch;
+  return big_tiles >= 1024 ? 3 : 1;
+}
+
+template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
 void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
             const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
             double2 gamma, int batch) {
-  constexpr int BM = 32 * TM, BN = 32 * TN;
+  constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;
   const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
   const int tiles = tiles_m * tiles_n;
-  hipLaunchKernelGGL((zgemm_mfma_kernel<TM, TN, BK, MINB>), dim3(tiles * batch), dim3(256), 0, h->stream, m, n, k,
-                     alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles);
+  hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
+                     h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles);
 }
 
 }  // namespace
@@ -194,9 +265,24 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     h->ev_next += 2;
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev], h->stream));
   }
-  switch (pick_variant(m, n)) {
-    case 0: launch<2, 2, 16, 1>(h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch); break;
-    default: launch<2, 2, 16, 3>(h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch); break;
+  if (h->prec_mask & h->prec_scope) {
+#define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch
+    switch (pick_variant32(m, n, batch)) {
+      case 1: launch<F32, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F32, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F32, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F32, 2, 2, 4, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      default: launch<F32, 2, 2, 2, 2, 16, 4>(EUVL_GEMM_ARGS); break;
+    }
+  } else {
+    switch (pick_variant(m, n)) {
+      case 0: launch<F64, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F64, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F64, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F64, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      default: launch<F64, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
+    }
+#undef EUVL_GEMM_ARGS
   }
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
@@ -224,6 +310,7 @@ extern "C" int euvl_zgemm_batched(euvl_handle h, int m, int n, int k, const doub
     return EUVL_ERR_ARG;
   }
   double2 g = gamma ? make_double2(gamma[0], gamma[1]) : make_double2(0.0, 0.0);
+  euvl::PrecScope scope(h, EUVL_PREC_FP32_GEMM);
   return euvl::zgemm(h, m, n, k, make_double2(alpha[0], alpha[1]), (const double2*)d_A, lda, sa,
                      (const double2*)d_B, ldb, sb, make_double2(beta[0], beta[1]), (double2*)d_C, ldc, sc, g,
                      batch);

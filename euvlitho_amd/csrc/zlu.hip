@@ -382,6 +382,7 @@ __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, in
 namespace euvl {
 
 int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info) {
+  PrecScope scope(h, EUVL_PREC_FP32_LU);
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
   double2* Pt = (double2*)aux_reserve(h, sizeof(double2) * (size_t)batch * NB * n);
   if (!Pt) {
@@ -424,6 +425,7 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
 
 int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64_t sa, const int* ipiv,
            double2* B, int64_t ldb, int64_t sb, int batch) {
+  PrecScope scope(h, EUVL_PREC_FP32_LU);
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
   hipLaunchKernelGGL(laswp_kernel, dim3((nrhs + 255) / 256, batch), dim3(256), 0, h->stream, nrhs, B, ldb, sb,
                      ipiv, (int64_t)n, 0, n, 0, 0);

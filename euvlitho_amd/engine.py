@@ -23,7 +23,9 @@ from .masks import pack_bits
 
 
 class Engine:
-    def __init__(self, cfg: Config, device="cuda:0"):
+    def __init__(self, cfg: Config, device="cuda:0", precision=_lib.PREC_FP64):
+        """precision: euvl_set_precision mask (0 = the FP64 drop-in; _lib.PREC_MIXED =
+        every product of the EM solve on FP32 MFMA, storage and the rest in FP64)."""
         if not torch.cuda.is_available():
             raise _lib.EuvlError("a ROCm GPU is required (torch.cuda.is_available() is False)")
         self.cfg = cfg
@@ -33,6 +35,8 @@ class Engine:
             h = ctypes.c_void_p()
             _lib.check(self.lib.euvl_create(ctypes.byref(h)), "euvl_create")
         self.h = h
+        self.precision = int(precision)
+        _lib.check(self.lib.euvl_set_precision(h, self.precision), "euvl_set_precision")
         self._layers = cfg.stack.layers()
         eps = np.array([[complex(e).real, complex(e).imag] for e, _ in self._layers], dtype=np.float64).ravel()
         thk = np.array([d for _, d in self._layers], dtype=np.float64)

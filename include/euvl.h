@@ -54,6 +54,23 @@ int64_t euvl_workspace_bytes(euvl_handle h);
  * (instrumentation). */
 int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 
+/* Compute precision of the products inside the EM solve (the precision
+ * form).  mask = 0 (default) keeps every product in FP64 MFMA, the plain
+ * drop-in.  Each set bit moves one group of products to FP32 MFMA with FP32
+ * accumulation over K; operands are rounded to FP32 as they are staged, and
+ * storage, the epilogue (alpha, beta, gamma) and every other kernel stay FP64:
+ *   EUVL_PREC_FP32_MATFUN    the cos / sinc polynomial and its doublings
+ *                            (the counterpart of the absorber eigenproblem)
+ *   EUVL_PREC_FP32_COUPLING  the absorber / cap coupling and top-solve products
+ *   EUVL_PREC_FP32_LU        trailing updates of the LU and its solves
+ *   EUVL_PREC_FP32_GEMM      direct euvl_zgemm_batched calls
+ * No reference counterpart (the reference is FP64 throughout). */
+#define EUVL_PREC_FP32_MATFUN 1
+#define EUVL_PREC_FP32_COUPLING 2
+#define EUVL_PREC_FP32_LU 4
+#define EUVL_PREC_FP32_GEMM 8
+int euvl_set_precision(euvl_handle h, int mask);
+
 /* ---------------------------------------------------------------------------
  * Dense complex linear algebra (batched, row-major, interleaved complex).
  * ------------------------------------------------------------------------- */

@@ -1,0 +1,100 @@
+"""GPU: the precision form (FP32 MFMA products, FP64 storage and FP64
+everywhere else) against the FP64 drop-in on the same inputs.
+
+Bars: an FP32-compute GEMM within 2e-5 of the largest entry of the FP64
+product (K up to 2100, FP32 accumulation); for a whole clip, images within
+1e-3 of their peak of the FP64 images (the bar the precision form is held to)
+and amplitudes within 1e-3 of the largest amplitude.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU test without a GPU")
+    from euvlitho_amd import _lib
+    return _lib.load()
+
+
+@pytest.mark.parametrize("m,n,k,batch", [(17, 33, 5, 2), (64, 64, 64, 1), (100, 70, 130, 3), (300, 257, 2100, 1),
+                                         (1681, 12, 1681, 2)])
+def test_zgemm_fp32_compute(lib, m, n, k, batch):
+    from euvlitho_amd import _lib
+    say(f"FP32-compute zgemm m={m} n={n} k={k} batch={batch}")
+    rng = np.random.default_rng(m + n + k)
+    A = rng.standard_normal((batch, m, k)) + 1j * rng.standard_normal((batch, m, k))
+    B = rng.standard_normal((batch, k, n)) + 1j * rng.standard_normal((batch, k, n))
+    C0 = rng.standard_normal((batch, m, n)) + 1j * rng.standard_normal((batch, m, n))
+    dA, dB, dC = (torch.from_numpy(x).cuda() for x in (A, B, C0))
+    h = ctypes.c_void_p()
+    assert lib.euvl_create(ctypes.byref(h)) == 0
+    try:
+        assert lib.euvl_set_precision(h, _lib.PREC_FP32_GEMM) == 0
+        al, be, ga = (ctypes.c_double * 2)(0.5, -1.0), (ctypes.c_double * 2)(0.25, 0.5), (ctypes.c_double * 2)(2, 0)
+        rc = lib.euvl_zgemm_batched(h, m, n, k, al, ctypes.c_void_p(dA.data_ptr()), k, m * k,
+                                    ctypes.c_void_p(dB.data_ptr()), n, k * n, be, ctypes.c_void_p(dC.data_ptr()), n,
+                                    m * n, ga, batch)
+        assert rc == 0, lib.euvl_last_error().decode()
+        torch.cuda.synchronize()
+        ref = (0.5 - 1j) * A @ B + (0.25 + 0.5j) * C0 + 2.0 * np.eye(m, n)[None]
+        e = np.abs(dC.cpu().numpy() - ref).max() / np.abs(ref).max()
+        say(f"  max err / max |C| = {e:.2e}")
+        assert e < 2e-5
+        assert e > 1e-12  # it really ran in FP32
+        # the mask selects: FP64 again after resetting it
+        assert lib.euvl_set_precision(h, 0) == 0
+        dC.copy_(torch.from_numpy(C0))
+        rc = lib.euvl_zgemm_batched(h, m, n, k, al, ctypes.c_void_p(dA.data_ptr()), k, m * k,
+                                    ctypes.c_void_p(dB.data_ptr()), n, k * n, be, ctypes.c_void_p(dC.data_ptr()), n,
+                                    m * n, ga, batch)
+        assert rc == 0
+        torch.cuda.synchronize()
+        assert np.abs(dC.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-13
+        assert lib.euvl_set_precision(h, 64) != 0  # unknown bit
+    finally:
+        lib.euvl_destroy(h)
+
+
+def test_precision_form_clip_vs_fp64(lib):
+    """512 nm clip, +-10 orders, 4 offsets: amplitudes, M3D parameters, Abbe
+    and SOCS images of the mixed form against the FP64 drop-in."""
+    from euvlitho_amd import _lib
+    from euvlitho_amd.config import preset
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    cfg = preset("c1")
+    m = manhattan(cfg.ndivx, cfg.ndivy, 0.5, 41)
+    out = {}
+    for name, prec in (("fp64", _lib.PREC_FP64), ("mixed", _lib.PREC_MIXED)):
+        say(f"clip in form {name}")
+        eng = Engine(cfg, precision=prec)
+        bits = eng.upload_mask(m)
+        Ax = eng.amp_s(bits, "X")
+        a0, ax, ay = eng.m3d(bits, Ax0=Ax[0])
+        img = eng.abbe(bits, Ax=Ax)
+        socs = eng.socs(bits, a0, ax, ay, nsocs=24, nsocsxy=20)
+        out[name] = [t.cpu().numpy() for t in (Ax, a0, img, socs)]
+        eng.close()
+    (A64, a064, I64, S64), (A32, a032, I32, S32) = out["fp64"], out["mixed"]
+    ea = np.abs(A32 - A64).max() / np.abs(A64).max()
+    e0 = np.abs(a032 - a064).max() / np.abs(a064).max()
+    ei = np.abs(I32 - I64).max() / I64.max()
+    es = np.abs(S32 - S64).max() / np.abs(S64).max()
+    say(f"  amplitudes {ea:.2e}, a0 {e0:.2e}, Abbe image {ei:.2e}, SOCS image {es:.2e} (of max / peak)")
+    assert ea < 1e-3 and e0 < 1e-3
+    assert ei < 1e-3 and es < 1e-3
+    assert ei > 0.0  # the mixed form is a different computation
