@@ -45,6 +45,37 @@ def copy_calibration(mib=512, reps=3):
     print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps}), flush=True)
 
 
+def run_served(n_req, slots):
+    """Served form: requests of six problems (pitch 256-1024 nm, orders +-8..+-20,
+    61-200 source points), mixed precision, `slots` streams with admission by
+    device-memory footprint.  A burst of 24 requests measures the capacity; the
+    Poisson stream is then offered at 80 % of it."""
+    from euvlitho_amd import _lib
+    from euvlitho_amd.serve import Server, poisson_stream, served_configs
+    cfgs = served_configs()
+    srv = Server(n_slots=slots, precision=_lib.PREC_MIXED)
+    try:
+        t = time.perf_counter()
+        srv.warm(cfgs)
+        warm_s = time.perf_counter() - t
+        burst = srv.run(poisson_stream(24, 0.0, seed=1, cfgs=cfgs))
+        cap = len(burst) / srv.wall
+        rate = 0.8 * cap
+        res = srv.run(poisson_stream(n_req, rate, seed=2, cfgs=cfgs))
+        lat = np.array([r.latency for r in res])
+        svc = np.array([r.finish - r.start for r in res])
+        return {"value": len(res) / srv.wall, "unit": "clips/s", "requests": len(res),
+                "offered_rate": rate, "burst_capacity": cap, "slots": slots,
+                "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
+                "service_p50_ms": 1000.0 * float(np.percentile(svc, 50)),
+                "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
+                "resident_build_s": warm_s, "peak_reserved_gib": srv.peak_reserved / 2**30,
+                "problems": ["%dnm K%d %s %dpts" % (c.ndivx, c.orders[1], "ann" if c.source.kind == 1 else "dip+ann",
+                                                       sum(len(x[2]) for x in c.source_points())) for c in cfgs]}
+    finally:
+        srv.close()
+
+
 def measured_traffic():
     """HBM bytes per GEMM launch of this workload, measured with rocprofv3 PMC
     counters (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy of known size) and
@@ -99,6 +130,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only (profiling runs)")
+    ap.add_argument("--served", type=int, default=200,
+                    help="requests of the served Poisson stream (0 = skip; 2000 = the full stream)")
+    ap.add_argument("--slots", type=int, default=4, help="streams of the served form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy that scales the HBM byte counters")
     args = ap.parse_args()
@@ -217,6 +251,8 @@ def main():
                      "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms},
         "forms": {"mixed": mixed} if mixed else {},
     }
+    if args.served > 0:
+        line["forms"]["served"] = run_served(args.served, args.slots)
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cpu_baseline(cfg, masks, threads)

@@ -41,6 +41,8 @@ _SIGS = {
     "euvl_destroy": [c_void_p],
     "euvl_set_stream": [c_void_p, c_void_p],
     "euvl_workspace_bytes": [c_void_p],
+    "euvl_reserve_workspace": [c_void_p, c_int64],
+    "euvl_release_workspace": [c_void_p],
     "euvl_gemm_stats": [c_void_p, c_int, P_double],
     "euvl_set_precision": [c_void_p, c_int],
     "euvl_zgemm_batched": [c_void_p, c_int, c_int, c_int, P_double, c_void_p, c_int64, c_int64, c_void_p, c_int64,

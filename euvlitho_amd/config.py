@@ -164,6 +164,25 @@ class Config:
             raise ValueError(f"unknown order set {kind!r}")
         return l[keep].astype(np.int32), m[keep].astype(np.int32), int(L), int(M)
 
+    def blank_order_set(self):
+        """Order set of the blank-mask solves: the narrow set of m3d.cpp:101-129
+        plus every lattice incidence inside the NA that it misses.  The blank
+        amplitude of incidence (ls, ms) is read at mode (ls, ms)
+        (m3d.cpp:240-258); at small pitches the truncated ellipse can miss an
+        in-NA incidence (256 nm at NA 0.55: (2, 1)), where the reference reads
+        an unset entry.  A uniform mask couples no modes, so the added modes
+        leave every other amplitude unchanged."""
+        li, mi, L, M = self.order_set(self.narrow_orders)
+        have = set(zip(li.tolist(), mi.tolist()))
+        lsx, lsy = self.lsmax
+        extra = [(ls, ms) for ls in range(-lsx, lsx + 1) for ms in range(-lsy, lsy + 1)
+                 if self.in_na(ls, ms) and (ls, ms) not in have]
+        if not extra:
+            return li, mi, L, M
+        el, em = np.array(extra, dtype=np.int32).T
+        return (np.concatenate([li, el]), np.concatenate([mi, em]), max(L, int(np.abs(el).max())),
+                max(M, int(np.abs(em).max())))
+
     @staticmethod
     def mode_map(lidx, midx, L, M):
         mp = np.full((2 * L + 1) * (2 * M + 1), -1, dtype=np.int32)

@@ -107,3 +107,24 @@ extern "C" int euvl_set_precision(euvl_handle h, int mask) {
   h->prec_mask = mask;
   return EUVL_OK;
 }
+
+extern "C" int euvl_reserve_workspace(euvl_handle h, int64_t bytes) {
+  if (!h || bytes < 0) {
+    euvl::set_error("euvl_reserve_workspace: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return euvl::ws_reserve(h, (size_t)bytes);
+}
+
+extern "C" int euvl_release_workspace(euvl_handle h) {
+  if (!h) return EUVL_ERR_ARG;
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (h->ws) EUVL_HIP_CHECK(hipFree(h->ws));
+  if (h->aux) EUVL_HIP_CHECK(hipFree(h->aux));
+  h->ws = nullptr;
+  h->aux = nullptr;
+  h->ws_cap = h->aux_cap = 0;
+  h->ws_top = 0;
+  return EUVL_OK;
+}

@@ -22,21 +22,50 @@ from .config import Config
 from .masks import pack_bits
 
 
+class Resident:
+    """Mask-independent state kept on the device (or host) across clips and
+    shared between engines and streams: blank amplitudes and SOCS kernels per
+    configuration.  Thread-safe; an entry is complete (its stream synchronised)
+    before it is published."""
+
+    def __init__(self):
+        import threading
+        self._d = {}
+        self._lock = threading.Lock()
+
+    def get(self, key):
+        with self._lock:
+            return self._d.get(key)
+
+    def put(self, key, value):
+        with self._lock:
+            return self._d.setdefault(key, value)
+
+    def __len__(self):
+        return len(self._d)
+
+
 class Engine:
-    def __init__(self, cfg: Config, device="cuda:0", precision=_lib.PREC_FP64):
+    def __init__(self, cfg: Config, device="cuda:0", precision=_lib.PREC_FP64, handle=None, resident=None):
         """precision: euvl_set_precision mask (0 = the FP64 drop-in; _lib.PREC_MIXED =
-        every product of the EM solve on FP32 MFMA, storage and the rest in FP64)."""
+        every product of the EM solve on FP32 MFMA, storage and the rest in FP64).
+        handle: an existing euvl handle to run on (not owned: close() leaves it);
+        resident: a Resident store shared between engines for the mask-independent
+        state (blank amplitudes, SOCS kernels)."""
         if not torch.cuda.is_available():
             raise _lib.EuvlError("a ROCm GPU is required (torch.cuda.is_available() is False)")
         self.cfg = cfg
         self.device = torch.device(device)
         self.lib = _lib.load()
-        with torch.cuda.device(self.device):
-            h = ctypes.c_void_p()
-            _lib.check(self.lib.euvl_create(ctypes.byref(h)), "euvl_create")
-        self.h = h
+        self._owns = handle is None
+        if handle is None:
+            with torch.cuda.device(self.device):
+                handle = ctypes.c_void_p()
+                _lib.check(self.lib.euvl_create(ctypes.byref(handle)), "euvl_create")
+        self.h = handle
         self.precision = int(precision)
-        _lib.check(self.lib.euvl_set_precision(h, self.precision), "euvl_set_precision")
+        _lib.check(self.lib.euvl_set_precision(self.h, self.precision), "euvl_set_precision")
+        self.resident = resident if resident is not None else Resident()
         self._layers = cfg.stack.layers()
         eps = np.array([[complex(e).real, complex(e).imag] for e, _ in self._layers], dtype=np.float64).ravel()
         thk = np.array([d for _, d in self._layers], dtype=np.float64)
@@ -49,13 +78,12 @@ class Engine:
             eps.ctypes.data_as(_lib.P_double), thk.ctypes.data_as(_lib.P_double))
         self.lidx, self.midx, self.L, self.M = cfg.order_set()
         self.modemap = Config.mode_map(self.lidx, self.midx, self.L, self.M)
-        self._blank = {}
         self.last_info = None
 
     def close(self):
-        if self.h:
+        if self.h and self._owns:
             self.lib.euvl_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -111,11 +139,14 @@ class Engine:
 
     def blank_amplitudes(self, polar="X"):
         """vcxx, abxx: blank vacuum / absorber amplitudes on the narrow order set at
-        the central offset (m3d.cpp:211-258); cached per engine and polarisation."""
-        if polar in self._blank:
-            return self._blank[polar]
+        the central offset (m3d.cpp:211-258); kept in the resident store per
+        configuration and polarisation."""
+        key = ("blank", self.cfg, polar, self.precision)
+        got = self.resident.get(key)
+        if got is not None:
+            return got
         cfg = self.cfg
-        orders = cfg.order_set(cfg.narrow_orders)
+        orders = cfg.blank_order_set()
         n = cfg.ndivx * cfg.ndivy
         vc = self.amp_s(self.upload_mask(np.ones(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
         ab = self.amp_s(self.upload_mask(np.zeros(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
@@ -134,8 +165,7 @@ class Engine:
                     if n_ >= 0:
                         vcxx[is_, js] = vch[is_, js, n_]
                         abxx[is_, js] = abh[is_, js, n_]
-        self._blank[polar] = (vcxx, abxx)
-        return self._blank[polar]
+        return self.resident.put(key, (vcxx, abxx))
 
     # --------------------------------------------------------------- M3D params
     def m3d(self, bits, Ax0=None):
@@ -245,10 +275,10 @@ class Engine:
     def eigenfunction(self, z=0.0, nk=100, mesh_deg=0.2):
         """STCC kernels (eigenfunction.py:5-179) kept on the device:
         (alpha [3, nk], phi [3, nk, ncut]); cached per (z, nk, mesh)."""
-        key = (float(z), int(nk), float(mesh_deg))
-        cache = getattr(self, "_kernels", {})
-        if key in cache:
-            return cache[key]
+        key = ("kernels", self.cfg, float(z), int(nk), float(mesh_deg))
+        got = self.resident.get(key)
+        if got is not None:
+            return got
         cfg = self.cfg
         lin, min_, _ = cfg.pupil
         nk = min(nk, len(lin))
@@ -263,9 +293,8 @@ class Engine:
                                              cfg.ky0, cfg.z0, float(z), nk, _lib.ptr(alpha), _lib.ptr(phi),
                                              ctypes.byref(sweeps)), "euvl_tcc_kernels")
         self.last_sweeps = sweeps.value
-        cache[key] = (alpha, phi)
-        self._kernels = cache
-        return alpha, phi
+        torch.cuda.current_stream(self.device).synchronize()  # other streams may read them next
+        return self.resident.put(key, (alpha, phi))
 
     def socs(self, bits, a0, ax, ay, nsocs=100, nsocsxy=20, nout=64, z=0.0, mesh_deg=0.2):
         """STCC-SOCS image (socs.py:156-204) on the config's image grid."""

@@ -45,6 +45,12 @@ int euvl_set_stream(euvl_handle h, void* hip_stream);
 const char* euvl_last_error(void);
 /* Device bytes currently held by the handle's workspace. */
 int64_t euvl_workspace_bytes(euvl_handle h);
+/* Grow the handle's workspace to at least `bytes` now (so that later calls
+ * do not reallocate), or free it (and the LU panel buffer) until the next
+ * call needs it.  Both synchronise the handle's stream.  No reference
+ * counterpart: they let a serving loop admit work by device-memory footprint. */
+int euvl_reserve_workspace(euvl_handle h, int64_t bytes);
+int euvl_release_workspace(euvl_handle h);
 /* Per-launch timing of the GEMM kernel with HIP events on the handle's
  * stream.  If out is non-null, synchronises and writes {launches, total
  * milliseconds, algorithmic flops (8 m n k per complex GEMM), algorithmic

@@ -86,8 +86,18 @@ def abbe_image(p, d, mask2d, lidx, midx, LMAX, MMAX, divisions, nx, ny, z=0.0, a
 
 
 def blank_amplitudes(p, d, narrow=("ellipse", 1.5)):
-    """vcxx, abxx on the narrow order set (m3d.cpp:101-129, 211-258)."""
+    """vcxx, abxx on the narrow order set (m3d.cpp:101-129, 211-258).  In-NA
+    lattice incidences that the truncated ellipse misses (small pitches) are
+    appended to the set: the reference reads an unset entry there, and a
+    uniform mask couples no modes, so the other amplitudes are unchanged."""
     li, mi, L, M = order_set(p, narrow[0], cutoff=narrow[1])
+    have = set(zip(li.tolist(), mi.tolist()))
+    extra = [(ls, ms) for ls in range(-d["lsmaxX"], d["lsmaxX"] + 1) for ms in range(-d["lsmaxY"], d["lsmaxY"] + 1)
+             if in_na(ls, ms, p, d) and (ls, ms) not in have]
+    if extra:
+        el, em_ = np.array(extra, dtype=np.int64).T
+        li, mi = np.concatenate([li, el]), np.concatenate([mi, em_])
+        L, M = max(L, int(np.abs(el).max())), max(M, int(np.abs(em_).max()))
     n = p["NDIVX"] * p["NDIVY"]
     Avc = em.amp_s("X", np.ones(n, np.int64), p, d, li, mi, L, M, d["kx0"], d["ky0"])
     Aab = em.amp_s("X", np.zeros(n, np.int64), p, d, li, mi, L, M, d["kx0"], d["ky0"])

@@ -192,14 +192,17 @@ def test_amp_s_c1_vs_oracle(pol):
         assert e < 1e-9
 
 
-def test_m3d_c1_vs_oracle():
+@pytest.mark.parametrize("ndiv,K", [(512, 10), (256, 8)])
+def test_m3d_vs_oracle(ndiv, K):
+    """512 nm (C1) and 256 nm, where the blank-mask order set needs the in-NA
+    incidences the truncated ellipse misses."""
     from euvlitho_amd.engine import Engine
     from euvlitho_amd.masks import manhattan
     from oracle import abbe as OA, em as OE
-    say("m3d C1")
-    cfg = c1_problem()
+    say(f"m3d {ndiv} nm, K={K}")
+    cfg = c1_problem(K, ndiv)
     eng = Engine(cfg)
-    m = manhattan(512, 512, 0.5, 2)
+    m = manhattan(ndiv, ndiv, 0.5, 2)
     bits = eng.upload_mask(m)
     a0, ax, ay = [t.cpu().numpy() for t in eng.m3d(bits)]
     p, d = oracle_problem(cfg)
@@ -207,6 +210,7 @@ def test_m3d_c1_vs_oracle():
     Ax = OE.amp_s("X", m, p, d, li, mi, eng.L, eng.M, d["kx0"], d["ky0"])
     vc, ab = OA.blank_amplitudes(p, d)
     r0, rx, ry = OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
+    assert np.isfinite(r0).all() and np.isfinite(a0).all()
     s = np.abs(r0).max()
     for g, r in ((a0, r0), (ax, rx), (ay, ry)):
         e = np.abs(g - r).max() / s

@@ -88,3 +88,26 @@ def test_engine_requires_gpu():
     from euvlitho_amd.engine import Engine
     with pytest.raises(_lib.EuvlError):
         Engine(reference_default())
+
+
+def test_served_stream_and_footprint():
+    from euvlitho_amd.serve import footprint, poisson_stream, served_configs
+    cfgs = served_configs()
+    a = poisson_stream(50, 10.0, seed=7, cfgs=cfgs)
+    b = poisson_stream(50, 10.0, seed=7, cfgs=cfgs)
+    assert [r.arrival for r in a] == [r.arrival for r in b]
+    assert all(np.array_equal(x.mask, y.mask) for x, y in zip(a, b))
+    assert all(a[i].arrival < a[i + 1].arrival for i in range(49))
+    assert abs(a[-1].arrival / 50 - 0.1) < 0.05  # mean gap 1 / rate
+    assert {r.cfg for r in a} <= set(cfgs)
+    for r in a[:10]:
+        dens = r.mask.mean()
+        assert 0.29 <= dens <= 0.72
+    # the EM workspace dominates the footprint: 13 N x N complex matrices per offset
+    c2 = cfgs[4]
+    n, off = len(c2.order_set()[0]), len(c2.offsets())
+    assert footprint(c2) >= 13 * 16 * n * n * off
+    assert footprint(c2) < 1.2 * 13 * 16 * n * n * off + 2**30
+    # source point counts of the mix are in 61..200
+    pts = [sum(len(x[2]) for x in c.source_points()) for c in cfgs]
+    assert all(61 <= p <= 200 for p in pts)
