@@ -158,6 +158,7 @@ def main():
         """Time one form: warmup, then `steps` clips each bracketed by a device
         synchronisation; then one more clip with per-GEMM HIP events."""
         eng = Engine(cfg, "cuda:0", precision=precision)
+        tuned = eng.autotune()  # GEMM tile choice for this size bucket
         bits = [eng.upload_mask(m) for m in masks]
         eng.blank_amplitudes()  # mask-independent, resident for every clip
         torch.cuda.synchronize()
@@ -192,7 +193,7 @@ def main():
         _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
         eng.close()
         return dict(lat=lat, wall=wall, tcc_ms=tcc_ms, outs=outs, launches=stats[0], gemm_ms=stats[1],
-                    gemm_flops=stats[2], gemm_bytes=stats[3])
+                    gemm_flops=stats[2], gemm_bytes=stats[3], tuned=tuned)
 
     f64 = run_form(_lib.PREC_FP64)
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
@@ -218,7 +219,7 @@ def main():
             "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel<F32>", "achieved": mix_ach,
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": mix_ach / FP32_PEAK_TFLOPS,
                          "avg_launch_us": 1000.0 * mix["gemm_ms"] / max(mix["launches"], 1),
-                         "gemm_ms_per_clip": mix["gemm_ms"]},
+                         "gemm_ms_per_clip": mix["gemm_ms"], "autotuned_variant": mix["tuned"]["fp32"][0]},
         }
     tcc_ms = f64["tcc_ms"]
 
@@ -248,7 +249,8 @@ def main():
                      "algorithmic_bytes_per_launch": gemm_bytes / max(launches, 1),
                      "launches_per_clip": int(launches),
                      "avg_launch_us": 1000.0 * gemm_ms / max(launches, 1),
-                     "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms},
+                     "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms,
+                     "autotuned_variant": f64["tuned"]["fp64"][0]},
         "forms": {"mixed": mixed} if mixed else {},
     }
     if args.served > 0:

@@ -45,9 +45,11 @@ _SIGS = {
     "euvl_release_workspace": [c_void_p],
     "euvl_gemm_stats": [c_void_p, c_int, P_double],
     "euvl_set_precision": [c_void_p, c_int],
+    "euvl_autotune_gemm": [c_void_p, c_int, c_int, c_int, c_int, c_int, P_int, P_double],
     "euvl_zgemm_batched": [c_void_p, c_int, c_int, c_int, P_double, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                            c_int64, P_double, c_void_p, c_int64, c_int64, P_double, c_int],
     "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
+    "euvl_zgetrf_nopiv_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
     "euvl_zgetrs_batched": [c_void_p, c_int, c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
                             c_int64, c_int],
     "euvl_zcos_sinc_batched": [c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_int, P_int],

@@ -472,27 +472,65 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   EUVL_LAUNCH_CHECK();
   double2* R = T2;
   double2* U0 = T1;
-  // LU of U0, solve for the incidence columns
+  double2* U0c = Mx[2];  // C is no longer needed: a copy of U0 for the residual check
+  double2* Bres = Mx[3]; // nor F: the right-hand sides kept for the check
   int* d_info = (int*)h->d_red;
-  EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, d_info));
-  {
-    int* hi = (int*)h->h_pinned;
-    EUVL_HIP_CHECK(hipMemcpyAsync(hi, d_info, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
-    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-    for (int b = 0; b < B; ++b)
-      if (hi[b] != 0) {
-        set_error("euvl_amp_s: U0 singular (offset %d, column %d)", b, hi[b] - 1);
-        return EUVL_ERR_SINGULAR;
-      }
-  }
   const int64_t nsxy = (int64_t)nsx * nsy;
   EUVL_TRY(fill(h, nsxy * N * B, (double2*)d_Ax, ZERO));
+  int lu_path = 0;
   if (ninc > 0) {
     const int64_t sr = (int64_t)N * ninc;
-    hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
-                       (const int*)(blk + o_idx), Brhs);
-    EUVL_LAUNCH_CHECK();
-    EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+    auto singular = [&](bool report) -> int {
+      int* hi = (int*)h->h_pinned;
+      EUVL_HIP_CHECK(hipMemcpyAsync(hi, d_info, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+      for (int b = 0; b < B; ++b)
+        if (hi[b] != 0) {
+          if (report) set_error("euvl_amp_s: U0 singular (offset %d, column %d)", b, hi[b] - 1);
+          return EUVL_ERR_SINGULAR;
+        }
+      return EUVL_OK;
+    };
+    // U0 is well conditioned (cond ~10-20 for these stacks) and its unpivoted
+    // elimination is stable (growth ~1.1), so the LU without row interchanges
+    // (GEMM panels, no pivot searches) comes first; its solve is verified by
+    // the residual ||U0 X - E|| / (||U0|| ||X||) and redone with partial
+    // pivoting if that is not at rounding level.
+    EUVL_TRY(ew_copy(h, sm * B, U0c, U0));
+    EUVL_TRY(zgetrf_nopiv(h, N, U0, N, sm, ipiv, B, d_info));
+    bool ok = singular(false) == EUVL_OK;
+    if (ok) {
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
+                         (const int*)(blk + o_idx), Brhs);
+      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(ew_copy(h, sr * B, Bres, Brhs));
+      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+      {
+        PrecScope fp64(h, 0);  // the check itself always in FP64
+        EUVL_TRY(zgemm(h, N, ninc, N, ONE, U0c, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
+      }
+      double* dr = h->d_red;
+      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Bres, dr));
+      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Brhs, dr + 64));
+      EUVL_TRY(reduce(h, 1, N, B, U0c, dr + 128));
+      EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, dr, sizeof(double) * 192, hipMemcpyDeviceToHost, h->stream));
+      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+      const double tol = (h->prec_mask & EUVL_PREC_FP32_LU) ? 1e-5 : 1e-12;
+      for (int b = 0; b < B; ++b) {
+        const double res = h->h_pinned[b], xm = h->h_pinned[64 + b], nrm = h->h_pinned[128 + b];
+        if (!(res <= tol * nrm * xm)) ok = false;
+      }
+    }
+    if (!ok) {
+      lu_path = 1;
+      EUVL_TRY(ew_copy(h, sm * B, U0, U0c));
+      EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, d_info));
+      EUVL_TRY(singular(true));
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
+                         (const int*)(blk + o_idx), Brhs);
+      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+    }
     EUVL_TRY(zgemm(h, N, ninc, N, ONE, R, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));
     hipLaunchKernelGGL(ax_scatter_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
                        (const int*)(blk + o_idx), (const int*)(blk + o_slot), (const double*)(blk + o_kz), FGc,
@@ -501,7 +539,7 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   }
   if (h_info) {
     for (int b = 0; b < B; ++b) {
-      h_info[4 * b + 0] = 0;
+      h_info[4 * b + 0] = lu_path;
       h_info[4 * b + 1] = ndbl;
       h_info[4 * b + 2] = nfac + 1;
       h_info[4 * b + 3] = ninc;

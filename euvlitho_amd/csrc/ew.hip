@@ -112,6 +112,22 @@ __global__ __launch_bounds__(256) void reduce_meandiag_kernel(int n, const doubl
   if (threadIdx.x == 0) red[b] = sh[0] / n;
 }
 
+// max over an m x ncols block (leading dim ld, batch stride sb) of |re| + |im|; grid (blocks, batch)
+__global__ __launch_bounds__(256) void reduce_maxabs_kernel(int m, int ncols, int64_t ld, int64_t sb,
+                                                            const double2* X, double* red) {
+  const int b = blockIdx.y;
+  X += (int64_t)b * sb;
+  const int64_t per = (int64_t)m * ncols;
+  double acc = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = X[(e / ncols) * ld + e % ncols];
+    const double a = fabs(v.x) + fabs(v.y);
+    acc = (a > acc || a != a) ? a : acc;
+  }
+  acc = wave_max(acc);
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(red + b, acc);
+}
+
 __global__ void zero_kernel(double* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
 }
@@ -182,6 +198,27 @@ int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* 
 
 int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out) {
   EUVL_TRY(reduce(h, kind, n, batch, X, h->d_red));
+  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * batch, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int b = 0; b < batch; ++b) out[b] = h->h_pinned[b];
+  return EUVL_OK;
+}
+
+int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X, double* d_slot) {
+  hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, h->stream, d_slot, batch);
+  EUVL_LAUNCH_CHECK();
+  int64_t blocks = ((int64_t)m * ncols + 255) / 256;
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(reduce_maxabs_kernel, dim3((unsigned)blocks, batch), dim3(256), 0, h->stream, m, ncols, ld, sb, X,
+                     d_slot);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int maxabs_to_host(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X,
+                   double* out) {
+  EUVL_TRY(maxabs(h, m, ncols, ld, sb, batch, X, h->d_red));
   EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * batch, hipMemcpyDeviceToHost, h->stream));
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   for (int b = 0; b < batch; ++b) out[b] = h->h_pinned[b];

@@ -241,10 +241,6 @@ __global__ void zero_d_kernel(int64_t count, double* out) {
 
 }  // namespace
 
-namespace euvl {
-int fill(euvl_handle h, int64_t count, double2* out, double2 v);
-}
-
 extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* p, const double* d_Ax, int nrange,
                                int lmax,
                                int mmax, const int* h_modemap, int n_off, const double* h_off, int n_src,

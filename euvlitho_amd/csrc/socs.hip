@@ -334,10 +334,6 @@ std::vector<std::complex<double>> twiddles(int n, double sign) {
 
 }  // namespace
 
-namespace euvl {
-int fill(euvl_handle h, int64_t count, double2* out, double2 v);
-}
-
 extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
                                 const double* h_src, double kx0, double ky0, double z0, double z, int nk,
                                 double* d_alpha, double* d_phi, int* h_sweeps) {

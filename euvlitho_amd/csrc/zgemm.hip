@@ -21,8 +21,11 @@
 // contiguous range of tiles, walked in groups of 8 tile rows (shared A row
 // panels and B column panels stay in one L2).
 #include <cstdlib>
+#include <map>
+#include <mutex>
+#include <tuple>
 
-#include "common.h"
+#include "ew.h"
 
 namespace {
 
@@ -211,9 +214,32 @@ int env_variant(const char* name, int dflt, int hi) {
   return (v < 0 || v > hi) ? dflt : v;
 }
 
-int pick_variant(int, int) {
+constexpr int N_VARIANTS = 5;  // per precision
+
+// Autotuned choices (euvl_autotune_gemm), per precision and size bucket: m, n
+// and k rounded up to a multiple of 64 below 256 and of 128 above, batch
+// exact.  Shared by every handle of the process.
+std::map<std::tuple<int, int, int, int, int>, int> g_tuned;
+std::mutex g_tuned_mu;
+
+int bucket(int x) { return x < 256 ? ((x + 63) / 64) * 64 : ((x + 127) / 128) * 128; }
+
+std::tuple<int, int, int, int, int> tune_key(int fp32, int m, int n, int k, int batch) {
+  return std::make_tuple(fp32, bucket(m), bucket(n), bucket(k), batch);
+}
+
+int tuned_variant(int fp32, int m, int n, int k, int batch) {
+  std::lock_guard<std::mutex> lock(g_tuned_mu);
+  if (g_tuned.empty()) return -1;
+  auto it = g_tuned.find(tune_key(fp32, m, n, k, batch));
+  return it == g_tuned.end() ? -1 : it->second;
+}
+
+int pick_variant(int m, int n, int k, int batch) {
   if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 4);
-  return g_variant;
+  if (getenv("EUVL_GEMM_TILE")) return g_variant;
+  const int t = tuned_variant(0, m, n, k, batch);
+  return t >= 0 ? t : g_variant;
 }
 
 // FP32 compute: larger tiles halve the operand bytes per flop, which is what
@@ -221,9 +247,11 @@ int pick_variant(int, int) {
 // to fill 256 CUs.  Measured on MI355X (TFLOP/s, N x N x N x batch):
 //   1681 x 4: 64x64 75, 64x128 84, 128x128 74;  4096: 98, 109, 123;
 //   8705: 67, 92, 108.
-int pick_variant32(int m, int n, int batch) {
+int pick_variant32(int m, int n, int k, int batch) {
   if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 4);
   if (g_variant32 >= 0) return g_variant32;
+  const int t = tuned_variant(1, m, n, k, batch);
+  if (t >= 0) return t;
   if (n <= 64) return 0;
   const int64_t big_tiles = (int64_t)((m + 127) / 128) * ((n + 127) / 128) * batch;
   return big_tiles >= 1024 ? 3 : 1;
@@ -238,6 +266,30 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
   const int tiles = tiles_m * tiles_n;
   hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles);
+}
+
+void launch_variant(euvl_handle h, int fp32, int v, int m, int n, int k, double2 alpha, const double2* A,
+                    int64_t lda, int64_t sa, const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C,
+                    int64_t ldc, int64_t sc, double2 gamma, int batch) {
+#define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch
+  if (fp32) {
+    switch (v) {
+      case 1: launch<F32, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F32, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F32, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F32, 2, 2, 4, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      default: launch<F32, 2, 2, 2, 2, 16, 4>(EUVL_GEMM_ARGS); break;
+    }
+  } else {
+    switch (v) {
+      case 0: launch<F64, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F64, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F64, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F64, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      default: launch<F64, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
+    }
+  }
+#undef EUVL_GEMM_ARGS
 }
 
 }  // namespace
@@ -265,25 +317,12 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     h->ev_next += 2;
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev], h->stream));
   }
-  if (h->prec_mask & h->prec_scope) {
-#define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch
-    switch (pick_variant32(m, n, batch)) {
-      case 1: launch<F32, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      case 2: launch<F32, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      case 3: launch<F32, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
-      case 4: launch<F32, 2, 2, 4, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      default: launch<F32, 2, 2, 2, 2, 16, 4>(EUVL_GEMM_ARGS); break;
-    }
-  } else {
-    switch (pick_variant(m, n)) {
-      case 0: launch<F64, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
-      case 2: launch<F64, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      case 3: launch<F64, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
-      case 4: launch<F64, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      default: launch<F64, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
-    }
-#undef EUVL_GEMM_ARGS
-  }
+  if (h->prec_mask & h->prec_scope)
+    launch_variant(h, 1, pick_variant32(m, n, k, batch), m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
+                   gamma, batch);
+  else
+    launch_variant(h, 0, pick_variant(m, n, k, batch), m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
+                   gamma, batch);
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev + 1], h->stream));
@@ -336,5 +375,71 @@ extern "C" int euvl_gemm_stats(euvl_handle h, int enable, double* out) {
   h->ev_used.clear();
   h->ev_next = 0;
   h->prof = enable != 0;
+  return EUVL_OK;
+}
+
+extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch, int fp32, int* out_variant,
+                                  double* out_ms) {
+  if (!h || m <= 0 || n <= 0 || k <= 0 || batch <= 0 || (fp32 != 0 && fp32 != 1)) {
+    euvl::set_error("euvl_autotune_gemm: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  const size_t ea = (size_t)batch * m * k, eb = (size_t)batch * k * n, ec = (size_t)batch * m * n;
+  double2 *A = nullptr, *B = nullptr, *C = nullptr;
+  if (hipMalloc(&A, sizeof(double2) * ea) != hipSuccess || hipMalloc(&B, sizeof(double2) * eb) != hipSuccess ||
+      hipMalloc(&C, sizeof(double2) * ec) != hipSuccess) {
+    (void)hipGetLastError();
+    if (A) (void)hipFree(A);
+    if (B) (void)hipFree(B);
+    euvl::set_error("euvl_autotune_gemm: allocation failed");
+    return EUVL_ERR_NOMEM;
+  }
+  int rc = EUVL_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best_ms = 1e300;
+  int best = -1;
+  do {
+    // operands of ones: the MFMA time does not depend on the values
+    if ((rc = euvl::fill(h, (int64_t)ea, A, make_double2(1.0, 0.0))) != EUVL_OK) break;
+    if ((rc = euvl::fill(h, (int64_t)eb, B, make_double2(1.0, 0.0))) != EUVL_OK) break;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+      euvl::set_error("euvl_autotune_gemm: event creation failed");
+      rc = EUVL_ERR_HIP;
+      break;
+    }
+    const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+    for (int v = 0; v < N_VARIANTS && rc == EUVL_OK; ++v) {
+      launch_variant(h, fp32, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
+                     (int64_t)m * n, zero, batch);
+      const int reps = 3;
+      if (hipEventRecord(e0, h->stream) != hipSuccess) rc = EUVL_ERR_HIP;
+      for (int r = 0; r < reps; ++r)
+        launch_variant(h, fp32, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
+                       (int64_t)m * n, zero, batch);
+      if (hipGetLastError() != hipSuccess || hipEventRecord(e1, h->stream) != hipSuccess ||
+          hipEventSynchronize(e1) != hipSuccess) {
+        euvl::set_error("euvl_autotune_gemm: launch of variant %d failed", v);
+        rc = EUVL_ERR_HIP;
+        break;
+      }
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      const double per = ms / reps;
+      if (out_ms) out_ms[v] = per;
+      if (per < best_ms) { best_ms = per; best = v; }
+    }
+  } while (false);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(h->stream);
+  (void)hipFree(A);
+  (void)hipFree(B);
+  (void)hipFree(C);
+  if (rc != EUVL_OK) return rc;
+  {
+    std::lock_guard<std::mutex> lock(g_tuned_mu);
+    g_tuned[tune_key(fp32, m, n, k, batch)] = best;
+  }
+  if (out_variant) *out_variant = best;
   return EUVL_OK;
 }

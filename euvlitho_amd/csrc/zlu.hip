@@ -1,14 +1,14 @@
-// Batched complex LU with partial pivoting (right-looking, panel width 32)
-// and the matching triangular solves.  Panels are factored by one workgroup
-// per matrix straight in global memory (all waves of a workgroup share one
-// CU, so __syncthreads orders their global reads and writes); the trailing
-// update is the MFMA GEMM of zgemm.hip.
+// Batched complex LU (right-looking, panel width 32) and the matching
+// triangular solves.  Panels are factored by tournament pivoting over
+// several workgroups (below); the trailing update is the MFMA GEMM of
+// zgemm.hip.
+#include <algorithm>
+
 #include "ew.h"
 
 namespace {
 
 constexpr int NB = 32;
-constexpr int PT = 1024;  // panel threads
 
 __device__ inline double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 
@@ -16,299 +16,296 @@ __global__ void zero_int_kernel(int* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
 }
 
-// Copy the panel A[kb:n, kb:kb+nbc] of every batch entry to Pt[b][q][r]
-// (column q contiguous over the m = n-kb panel rows) and back.
-__global__ void panel_in_kernel(int n, const double2* A, int64_t lda, int64_t sa, int kb, int nbc, double2* Pt) {
-  const int b = blockIdx.y;
-  const int m = n - kb;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  const double2* row = A + (int64_t)b * sa + (int64_t)(kb + r) * lda + kb;
-  double2* p = Pt + (int64_t)b * NB * n;
-  for (int q = 0; q < nbc; ++q) p[(int64_t)q * n + r] = row[q];
-}
+// Panel factorisation by tournament pivoting (TSLU, the panel step of
+// communication-avoiding LU).  The m x nbc panel is cut into groups of up to
+// TS rows; every group is factored with partial pivoting in registers (one
+// row per thread, the pivot row broadcast through LDS) only to choose its nbc
+// candidate pivot rows; the candidates of TS / nbc groups form the groups of
+// the next round, until one group is left.  The partial-pivoting LU of that
+// last group gives the panel's pivot rows and its L11\U11 block.  Each group
+// is a workgroup of its own, so a tall panel spreads over ceil(m / TS) CUs
+// with its rows in registers, instead of one CU streaming the panel through
+// global memory once per column block.  A last parallel pass moves the rows
+// and forms L21 = A21 U11^-1.  Every round reads the original panel values:
+// nothing is written to A before that pass.
+// rows (= threads) per group.  A column step costs about the same with one
+// wave (no cross-wave exchange) as with eight, so larger groups (fewer
+// rounds) win: measured per 1681 x 1681 x 4 LU, 24 ms with 512, 31 ms with 64.
+constexpr int TSZ = 512;
 
-__global__ void panel_out_kernel(int n, double2* A, int64_t lda, int64_t sa, int kb, int nbc, const double2* Pt) {
-  const int b = blockIdx.y;
-  const int m = n - kb;
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= m) return;
-  double2* row = A + (int64_t)b * sa + (int64_t)(kb + r) * lda + kb;
-  const double2* p = Pt + (int64_t)b * NB * n;
-  for (int q = 0; q < nbc; ++q) row[q] = p[(int64_t)q * n + r];
-}
+__device__ inline double2 czero() { return make_double2(0.0, 0.0); }
 
-// Factor the transposed panel Pt[b] (nbc columns of m = n-kb rows, column
-// stride n) with partial pivoting; one workgroup per batch entry.  Column
-// accesses are contiguous across threads.  Writes ipiv[kb + j] (absolute rows).
-__global__ __launch_bounds__(PT) void getrf_panel_kernel(int n, double2* Pt, int* ipiv, int kb, int nbc, int* info) {
-  __shared__ double s_val[PT / 64];
-  __shared__ int s_idx[PT / 64];
-  __shared__ int s_piv;
+// Round over the groups of one panel.  Group g of batch entry b factors the
+// panel rows rows_in[b][g*TS ...] (panel positions; the identity when rows_in
+// is null) and writes its min(nbc, rows) pivot rows, in order, to
+// rows_out[b][g*nbc ...].  The last round (one group) instead writes the
+// LU11 block (row j = [L11(j, :j), U11(j, j:)]), the original values of the
+// panel's top nbc rows, the row interchanges ipiv (LAPACK order, absolute
+// rows) and the list of moved rows, and flags the first zero pivot in info.
+//
+// The column loop is not unrolled: a 32-step unrolled body is larger than the
+// instruction cache and every step would refetch it.  Instead each thread
+// keeps a window of its row that shifts left by one column per step, so that
+// the current column is always x[0] and every register index is static; the
+// multipliers that leave the window go to lscr (last round only), from where
+// the pivot rows' L11 parts are gathered at the end.
+template <int TS>
+__global__ __launch_bounds__(TS) void tslu_round_kernel(const double2* __restrict__ A, int64_t lda, int64_t sa,
+                                                        int n, int kb, int nbc, const int* __restrict__ rows_in,
+                                                        int n_in, int* __restrict__ rows_out, int last,
+                                                        double2* __restrict__ lu11, double2* __restrict__ top,
+                                                        double2* __restrict__ lscr, int gstride,
+                                                        int* __restrict__ ipiv, int* __restrict__ moved,
+                                                        int* __restrict__ info) {
+  __shared__ double s_val[TS / 64];
+  __shared__ int s_idx[TS / 64];
   __shared__ double2 s_prow[NB];
-  const int b = blockIdx.x;
-  const int m = n - kb;
-  double2* P = Pt + (int64_t)b * NB * n;
-  ipiv += (int64_t)b * n;
-  const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
-
-  for (int j = 0; j < nbc; ++j) {
-    double2* col = P + (int64_t)j * n;
-    // pivot search: first row of largest |re|+|im| (LAPACK izamax rule)
-    double best = -1.0;
-    int bidx = m;
-    for (int r = j + tid; r < m; r += PT) {
-      const double v = cabs1(col[r]);
-      if (v > best) { best = v; bidx = r; }
+  __shared__ int s_pick[NB], s_pickt[NB];
+  __shared__ int s_tp[2 * NB], s_tr[2 * NB];
+  const int b = blockIdx.y, g = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double2* P = A + (int64_t)b * sa + (int64_t)kb * lda + kb;  // panel origin
+  const int first = g * TS;
+  const int rows = min(TS, n_in - first);
+  const int steps = min(nbc, rows);
+  int pos = -1;
+  if (tid < rows) pos = rows_in ? rows_in[(int64_t)b * n + first + tid] : first + tid;
+  double2 x[NB];
+  {
+    // unconditional loads from a valid row, masked afterwards
+    const double2* row = P + (int64_t)max(pos, 0) * lda;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const double2 v = row[min(q, nbc - 1)];
+      const bool ok = pos >= 0 && q < nbc;
+      x[q] = make_double2(ok ? v.x : 0.0, ok ? v.y : 0.0);  // component selects: no stack temporary
     }
+  }
+  double2* my_l = lscr + (((int64_t)b * gstride + g) * TS + tid) * NB;
+  bool active = pos >= 0;
+#pragma unroll 1
+  for (int j = 0; j < steps; ++j) {
+    // pivot: largest |re|+|im| of the current column over the active rows (first in input order on ties)
+    double best = active ? cabs1(x[0]) : -1.0;
+    int bidx = active ? tid : TS;
     for (int off = 32; off > 0; off >>= 1) {
       const double ov = __shfl_xor(best, off);
       const int oi = __shfl_xor(bidx, off);
       if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
     }
-    if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
-    __syncthreads();
-    if (tid < 64) {
-      double bv = (tid < PT / 64) ? s_val[tid] : -2.0;
-      int bi = (tid < PT / 64) ? s_idx[tid] : m;
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(bv, off);
-        const int oi = __shfl_xor(bi, off);
-        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-      }
-      if (tid == 0) {
-        if (bi >= m) bi = j;
-        s_piv = bi;
-        ipiv[kb + j] = kb + bi;
-        if (bv == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+    int bi = bidx;  // one wave: the butterfly left the winner in every lane
+    if constexpr (TS > 64) {
+      if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
+      __syncthreads();
+      double bv = -2.0;
+      bi = TS;
+#pragma unroll
+      for (int w = 0; w < TS / 64; ++w) {
+        const double v = s_val[w];
+        const int i = s_idx[w];
+        if (v > bv || (v == bv && i < bi)) { bv = v; bi = i; }
       }
     }
-    __syncthreads();
-    const int piv = s_piv;
-    // swap rows j and piv across the panel columns and stage the pivot row
-    if (tid < nbc) {
-      double2* cq = P + (int64_t)tid * n;
-      const double2 t = cq[j];
-      const double2 u = cq[piv];
-      cq[j] = u;
-      if (piv != j) cq[piv] = t;
-      s_prow[tid] = u;
+    if (tid == bi) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) s_prow[q] = x[q];
+      s_pick[j] = pos;
+      s_pickt[j] = tid;
+      active = false;
+      if (last) {
+        double2* out = lu11 + ((int64_t)b * NB + j) * NB + j;  // U11(j, j:)
+#pragma unroll
+        for (int q = 0; q < NB; ++q)
+          if (q < nbc - j) out[q] = x[q];
+        if (x[0].x == 0.0 && x[0].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+      }
     }
     __syncthreads();
-    const double2 d = s_prow[j];
-    const bool nz = (d.x != 0.0 || d.y != 0.0);
-    const double2 dinv = nz ? cdiv(make_double2(1.0, 0.0), d) : make_double2(0.0, 0.0);
-    for (int r = j + 1 + tid; r < m; r += PT) {
-      const double2 l = nz ? cmul(col[r], dinv) : col[r];
-      col[r] = l;
-      if (nz)
-        for (int q = j + 1; q < nbc; ++q) {
-          double2* cq = P + (int64_t)q * n;
-          cq[r] = csub(cq[r], cmul(l, s_prow[q]));
-        }
+    const double2 d = s_prow[0];
+    const bool elim = active && (d.x != 0.0 || d.y != 0.0);
+    const double2 x0 = x[0];
+    double2 l = czero();
+    if (elim) l = cdiv(x0, d);
+    // shift the window left by one column while eliminating
+#pragma unroll
+    for (int q = 0; q < NB - 1; ++q) x[q] = csub(x[q + 1], cmul(l, s_prow[q + 1]));
+    x[NB - 1] = czero();
+    // the multiplier leaving the window (read back for the pivot rows of the last round)
+    my_l[j] = make_double2(elim ? l.x : x0.x, elim ? l.y : x0.y);
+  }
+  __syncthreads();
+  if (!last) {
+    if (tid < steps) rows_out[(int64_t)b * n + g * nbc + tid] = s_pick[tid];
+    return;
+  }
+  // L11(j, k), k < j: the multipliers the j-th pivot row got before it was picked
+  for (int e = tid; e < NB * NB; e += TS) {
+    const int jr = e / NB, k = e % NB;
+    if (jr < nbc && k < jr)
+      lu11[((int64_t)b * NB + jr) * NB + k] = lscr[((int64_t)b * gstride * TS + s_pickt[jr]) * NB + k];
+  }
+  // original values of the panel's top nbc rows (they move to the pivots' places)
+  if (tid < nbc) {
+    const double2* row = P + (int64_t)tid * lda;
+    double2* out = top + ((int64_t)b * NB + tid) * NB;
+    for (int q = 0; q < nbc; ++q) out[q] = row[q];
+  }
+  // the pivot rows as LAPACK's sequence of interchanges: position j <-> the
+  // position the j-th pivot row holds at that point (positions >= nbc that
+  // are touched end up holding rows that started above nbc)
+  if (tid == 0) {
+    int nt = 0;
+    for (int j = 0; j < nbc; ++j) {
+      const int r = s_pick[j];
+      int p = r;  // where row r is now
+      for (int t = 0; t < nt; ++t)
+        if (s_tr[t] == r) p = s_tp[t];
+      int rj = j;  // which row is at position j now
+      for (int t = 0; t < nt; ++t)
+        if (s_tp[t] == j) rj = s_tr[t];
+      ipiv[(int64_t)b * n + kb + j] = kb + p;
+      bool fj = false, fp = false;
+      for (int t = 0; t < nt; ++t) {
+        if (s_tp[t] == j) { s_tr[t] = r; fj = true; }
+        else if (s_tp[t] == p) { s_tr[t] = rj; fp = true; }
+      }
+      if (!fj) { s_tp[nt] = j; s_tr[nt] = r; ++nt; }
+      if (!fp && p != j) { s_tp[nt] = p; s_tr[nt] = rj; ++nt; }
     }
-    __syncthreads();
+    int* mv = moved + (int64_t)b * (1 + 4 * NB);
+    mv[0] = nt;
+    for (int t = 0; t < nt; ++t) {
+      mv[1 + 2 * t] = s_tp[t];
+      mv[2 + 2 * t] = s_tr[t];
+    }
   }
 }
 
-// Register-resident panel factorisation for panels of at most RPT * RT rows.
-// The nbc <= 32 columns are processed in leaves of LW: a leaf lives in
-// registers (row r of the panel is held by thread r % 512, slot r / 512), is
-// factored column by column with partial pivoting (pivot rows exchanged
-// through LDS, the swap mirrored on the panel's other columns in global
-// memory), and then the columns to its right are updated in one pass:
-// U12 = L11^-1 A12 on the 8 pivot rows and A22 -= L21 U12 on the rest.
-constexpr int RT = 512;  // threads of the register-resident panel kernel
-constexpr int RPT = 4;   // rows per thread (panel height <= 2048)
-constexpr int LW = 4;    // leaf width (RPT x LW complex per thread stay in registers)
-
-__global__ __launch_bounds__(RT) void getrf_panel_reg_kernel(int n, double2* Pt, int* ipiv, int kb, int nbc,
-                                                             int* info) {
-  __shared__ double s_val[RT / 64];
-  __shared__ int s_idx[RT / 64];
-  __shared__ int s_piv;
-  __shared__ double2 s_row[2][LW];
-  __shared__ double2 s_l11[LW][LW];
-  __shared__ double2 s_u[LW][NB];
-  __shared__ int s_pivs[LW];
-  const int b = blockIdx.x;
-  const int m = n - kb;
-  double2* P = Pt + (int64_t)b * NB * n;
-  ipiv += (int64_t)b * n;
+// Final pass over the panel rows: position q receives L11\U11 row q (q < nbc)
+// or the row that ends up there, turned into L21 by solving l U11 = a column
+// by column (l_c = a_c / U(c,c), then a -= l_c U(c, :)) with the same
+// shifting window as the rounds.
+__global__ __launch_bounds__(256) void tslu_apply_kernel(double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
+                                                         int m, int nbc, const double2* __restrict__ lu11,
+                                                         const double2* __restrict__ top,
+                                                         const int* __restrict__ moved) {
+  __shared__ double2 s_u[NB][NB + 1];
+  __shared__ int s_tp[2 * NB], s_tr[2 * NB];
+  __shared__ int s_nt;
+  const int b = blockIdx.y;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < NB * NB; e += blockDim.x) {
+    const int r = e / NB, c = e % NB;
+    s_u[r][c] = lu11[((int64_t)b * NB + r) * NB + c];
+  }
+  const int* mv = moved + (int64_t)b * (1 + 4 * NB);
+  if (tid == 0) s_nt = mv[0];
+  if (tid < 2 * NB) {
+    s_tp[tid] = mv[1 + 2 * tid];
+    s_tr[tid] = mv[2 + 2 * tid];
+  }
+  __syncthreads();
+  const int q = blockIdx.x * blockDim.x + tid;
+  if (q >= m) return;
+  double2* Pq = A + (int64_t)b * sa + (int64_t)(kb + q) * lda + kb;
+  if (q < nbc) {
+    for (int c = 0; c < nbc; ++c) Pq[c] = s_u[q][c];
+    return;
+  }
+  int src = q;
+  const int nt = s_nt;
+  for (int t = 0; t < nt; ++t)
+    if (s_tp[t] == q) src = s_tr[t];
+  const double2* row = (src < nbc) ? top + ((int64_t)b * NB + src) * NB
+                                   : A + (int64_t)b * sa + (int64_t)(kb + src) * lda + kb;
+  double2 a[NB];
+#pragma unroll
+  for (int c = 0; c < NB; ++c) {
+    const double2 v = row[min(c, nbc - 1)];
+    a[c] = make_double2(c < nbc ? v.x : 0.0, c < nbc ? v.y : 0.0);
+  }
+#pragma unroll 1
+  for (int c = 0; c < nbc; ++c) {
+    const double2 d = s_u[c][c];
+    double2 l = a[0];
+    if (d.x != 0.0 || d.y != 0.0) l = cdiv(l, d);
+    Pq[c] = l;
+#pragma unroll
+    for (int k = 0; k < NB - 1; ++k) a[k] = csub(a[k + 1], cmul(l, s_u[c][min(c + 1 + k, NB - 1)]));
+    a[NB - 1] = czero();
+  }
+}
 
-  for (int c0 = 0; c0 < nbc; c0 += LW) {
-    const int lw = (nbc - c0 < LW) ? nbc - c0 : LW;
-    double2 x[RPT][LW];
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const int r = tid + s * RT;
-#pragma unroll
-      for (int q = 0; q < LW; ++q)
-        x[s][q] = (r < m && q < lw) ? P[(int64_t)(c0 + q) * n + r] : make_double2(0.0, 0.0);
-    }
-    for (int jj = 0; jj < lw; ++jj) {
-      const int j = c0 + jj;
-      // pivot search over rows >= j (first of the largest |re|+|im|)
-      double best = -1.0;
-      int bidx = m;
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = tid + s * RT;
-        double v = -1.0;
-#pragma unroll
-        for (int q = 0; q < LW; ++q)
-          if (q == jj) v = cabs1(x[s][q]);
-        if (r >= j && r < m && v > best) { best = v; bidx = r; }
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bidx, off);
-        if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
-      }
-      if (lane == 0) { s_val[wave] = best; s_idx[wave] = bidx; }
-      __syncthreads();
-      if (tid < 64) {
-        double bv = (tid < RT / 64) ? s_val[tid] : -2.0;
-        int bi = (tid < RT / 64) ? s_idx[tid] : m;
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(bv, off);
-          const int oi = __shfl_xor(bi, off);
-          if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
-        }
-        if (tid == 0) {
-          if (bi >= m) bi = j;
-          s_piv = bi;
-          ipiv[kb + j] = kb + bi;
-          if (bv == 0.0 && info[b] == 0) info[b] = kb + j + 1;
-        }
-      }
-      __syncthreads();
-      const int piv = s_piv;
-      // exchange leaf rows j and piv through LDS
-      const int oj = j % RT, sj = j / RT, op = piv % RT, sp = piv / RT;
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        if (tid == oj && s == sj)
-#pragma unroll
-          for (int q = 0; q < LW; ++q) s_row[0][q] = x[s][q];
-        if (tid == op && s == sp)
-#pragma unroll
-          for (int q = 0; q < LW; ++q) s_row[1][q] = x[s][q];
-      }
-      if (tid == 0) s_pivs[jj] = piv;
-      __syncthreads();
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        if (tid == oj && s == sj)
-#pragma unroll
-          for (int q = 0; q < LW; ++q) x[s][q] = s_row[1][q];
-        if (piv != j && tid == op && s == sp)
-#pragma unroll
-          for (int q = 0; q < LW; ++q) x[s][q] = s_row[0][q];
-      }
-      // pivot row = old row piv (s_row[1]); eliminate below it
-      const double2 d = s_row[1][jj];
-      const bool nz = (d.x != 0.0 || d.y != 0.0);
-      const double2 dinv = nz ? cdiv(make_double2(1.0, 0.0), d) : make_double2(0.0, 0.0);
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = tid + s * RT;
-        if (r > j && r < m && nz) {
-          double2 l = make_double2(0.0, 0.0);
-#pragma unroll
-          for (int q = 0; q < LW; ++q)
-            if (q == jj) {
-              l = cmul(x[s][q], dinv);
-              x[s][q] = l;
-            }
-#pragma unroll
-          for (int q = 0; q < LW; ++q)
-            if (q > jj && q < lw) x[s][q] = csub(x[s][q], cmul(l, s_row[1][q]));
-        }
-      }
-      __syncthreads();
-    }
-    // write the leaf back and stage L11 (rows c0..c0+lw-1 of the leaf)
-#pragma unroll
-    for (int s = 0; s < RPT; ++s) {
-      const int r = tid + s * RT;
-      if (r < m)
-#pragma unroll
-        for (int q = 0; q < LW; ++q)
-          if (q < lw) P[(int64_t)(c0 + q) * n + r] = x[s][q];
-      if (r >= c0 && r < c0 + lw)
-#pragma unroll
-        for (int q = 0; q < LW; ++q) s_l11[r - c0][q] = x[s][q];
-    }
-    // apply this leaf's interchanges, in order, to the panel's other columns
-    if (tid < nbc && (tid < c0 || tid >= c0 + lw)) {
-      double2* cq = P + (int64_t)tid * n;
-      for (int jj = 0; jj < lw; ++jj) {
-        const int j = c0 + jj, piv = s_pivs[jj];
-        if (piv != j) {
-          const double2 t = cq[j];
-          cq[j] = cq[piv];
-          cq[piv] = t;
-        }
-      }
-    }
-    const int c1 = c0 + lw;
-    const int nr = nbc - c1;  // columns right of the leaf
-    if (nr <= 0) break;
+// ---------------------------------------------------------------------------
+// LU without row interchanges.  Each 32-wide panel starts with its 32 x 32
+// diagonal block, factored in LDS by one workgroup per batch entry, which also
+// forms U11^-1 and L11^-1; the panel's column and row blocks then become
+// GEMMs (L21 = A21 U11^-1, U12 = L11^-1 A12), as does the trailing update.
+// No column step of the factorisation runs over the panel's full height, so
+// a panel costs a few short launches instead of 32 dependent pivot searches.
+__global__ __launch_bounds__(NB * NB) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
+                                                          int nbc, double2* __restrict__ uinv,
+                                                          double2* __restrict__ linv, int* __restrict__ ipiv, int n,
+                                                          int* __restrict__ info) {
+  __shared__ double2 a[NB][NB + 1];
+  __shared__ double2 x[NB][NB + 1];
+  const int b = blockIdx.x, tid = threadIdx.x, r = tid / NB, c = tid % NB;
+  double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
+  {
+    const bool in = r < nbc && c < nbc;
+    const double2 v = A11[(int64_t)min(r, nbc - 1) * lda + min(c, nbc - 1)];
+    const double pad = (r == c) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
+    a[r][c] = make_double2(in ? v.x : pad, in ? v.y : 0.0);
+  }
+#pragma unroll 1
+  for (int j = 0; j < NB; ++j) {
     __syncthreads();
-    // U12 = L11^-1 A12 on the pivot rows, one thread per right column
-    if (tid < nr) {
-      double2* cq = P + (int64_t)(c1 + tid) * n;
-      double2 u[LW];
-#pragma unroll
-      for (int i = 0; i < LW; ++i) {
-        if (i < lw) {
-          double2 v = cq[c0 + i];
-#pragma unroll
-          for (int p = 0; p < LW; ++p)
-            if (p < i) v = csub(v, cmul(s_l11[i][p], u[p]));
-          u[i] = v;
-          cq[c0 + i] = v;
-          s_u[i][tid] = v;
-        } else {
-          u[i] = make_double2(0.0, 0.0);
-        }
-      }
+    if (c == j && r > j) {
+      const double2 d = a[j][j];
+      if (d.x != 0.0 || d.y != 0.0) a[r][j] = cdiv(a[r][j], d);
     }
     __syncthreads();
-    // A22 -= L21 U12 for the rows below the leaf's pivot rows; columns in
-    // chunks of 8 so that the thread's RPT x 8 loads are in flight together
-    for (int q0 = 0; q0 < nr; q0 += 8) {
-      double2 v[RPT][8];
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = tid + s * RT;
-#pragma unroll
-        for (int qq = 0; qq < 8; ++qq)
-          v[s][qq] = (r >= c1 && r < m && q0 + qq < nr) ? P[(int64_t)(c1 + q0 + qq) * n + r]
-                                                         : make_double2(0.0, 0.0);
-      }
-#pragma unroll
-      for (int s = 0; s < RPT; ++s) {
-        const int r = tid + s * RT;
-        if (r >= c1 && r < m) {
-#pragma unroll
-          for (int qq = 0; qq < 8; ++qq) {
-            if (q0 + qq < nr) {
-              double2 t = v[s][qq];
-#pragma unroll
-              for (int p = 0; p < LW; ++p)
-                if (p < lw) t = csub(t, cmul(x[s][p], s_u[p][q0 + qq]));
-              P[(int64_t)(c1 + q0 + qq) * n + r] = t;
-            }
-          }
-        }
-      }
+    if (r > j && c > j) a[r][c] = csub(a[r][c], cmul(a[r][j], a[j][c]));
+  }
+  __syncthreads();
+  if (r < nbc && c < nbc) A11[(int64_t)r * lda + c] = a[r][c];
+  if (tid < nbc) ipiv[(int64_t)b * n + kb + tid] = kb + tid;
+  if (tid == 0)
+    for (int j = 0; j < nbc; ++j)
+      if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+  // U11^-1, one column per thread: X(i, col) for i = col .. 0
+  if (tid < NB) {
+    const int col = tid;
+    for (int i = NB - 1; i >= 0; --i) {
+      double2 t = make_double2(i == col ? 1.0 : 0.0, 0.0);
+      for (int k = i + 1; k <= col; ++k) t = csub(t, cmul(a[i][k], x[k][col]));
+      x[i][col] = (i <= col) ? cdiv(t, a[i][i]) : make_double2(0.0, 0.0);
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  uinv[((int64_t)b * NB + r) * NB + c] = x[r][c];
+  __syncthreads();
+  // L11^-1 (unit lower), one column per thread: Y(row, col) for row = col .. NB-1
+  if (tid < NB) {
+    const int col = tid;
+    for (int row = 0; row < NB; ++row) {
+      double2 t = make_double2(row == col ? 1.0 : 0.0, 0.0);
+      for (int k = col; k < row; ++k) t = csub(t, cmul(a[row][k], x[k][col]));
+      x[row][col] = (row >= col) ? t : make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  linv[((int64_t)b * NB + r) * NB + c] = x[r][c];
+}
+
+// out[b][i][j] = in[b][i][j] for an m x ncols block (leading dims ldo, ldi)
+__global__ void copy2d_kernel(int m, int ncols, double2* __restrict__ out, int64_t ldo, int64_t so,
+                              const double2* __restrict__ in, int64_t ldi, int64_t si) {
+  const int b = blockIdx.y;
+  const int64_t total = (int64_t)m * ncols;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / ncols, j = e % ncols;
+    out[(int64_t)b * so + i * ldo + j] = in[(int64_t)b * si + i * ldi + j];
   }
 }
 
@@ -384,26 +381,48 @@ namespace euvl {
 int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info) {
   PrecScope scope(h, EUVL_PREC_FP32_LU);
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
-  double2* Pt = (double2*)aux_reserve(h, sizeof(double2) * (size_t)batch * NB * n);
-  if (!Pt) {
+  // scratch: two candidate-row lists (batch x n ints), LU11 and top blocks
+  // (batch x NB x NB complex each), the moved-row lists and the last round's
+  // multipliers (batch x groups x TS x NB complex)
+  const size_t b_rows = sizeof(int) * (size_t)batch * n;
+  const size_t b_blk = sizeof(double2) * (size_t)batch * NB * NB;
+  const size_t b_mv = sizeof(int) * (size_t)batch * (1 + 4 * NB);
+  const int gstride = (n + TSZ - 1) / TSZ;  // groups of the first round
+  const size_t b_l = sizeof(double2) * (size_t)batch * gstride * TSZ * NB;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  char* scr = (char*)aux_reserve(h, 2 * al(b_rows) + 2 * al(b_blk) + al(b_mv) + al(b_l));
+  if (!scr) {
     set_error("zgetrf: panel scratch allocation failed");
     return EUVL_ERR_NOMEM;
   }
+  int* rows[2] = {(int*)scr, (int*)(scr + al(b_rows))};
+  double2* lu11 = (double2*)(scr + 2 * al(b_rows));
+  double2* top = (double2*)(scr + 2 * al(b_rows) + al(b_blk));
+  int* moved = (int*)(scr + 2 * al(b_rows) + 2 * al(b_blk));
+  double2* lscr = (double2*)(scr + 2 * al(b_rows) + 2 * al(b_blk) + al(b_mv));
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, h->stream, d_info, batch);
   EUVL_LAUNCH_CHECK();
   for (int kb = 0; kb < n; kb += NB) {
     const int nbc = (n - kb < NB) ? n - kb : NB;
     const int m = n - kb;
-    hipLaunchKernelGGL(panel_in_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb, nbc,
-                       Pt);
-    EUVL_LAUNCH_CHECK();
-    if (m <= RT * RPT)
-      hipLaunchKernelGGL(getrf_panel_reg_kernel, dim3(batch), dim3(RT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
-    else
-      hipLaunchKernelGGL(getrf_panel_kernel, dim3(batch), dim3(PT), 0, h->stream, n, Pt, ipiv, kb, nbc, d_info);
-    EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(panel_out_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, kb,
-                       nbc, Pt);
+    // tournament rounds until one group is left
+    const int* in = nullptr;
+    int n_in = m, cur = 0;
+    for (;;) {
+      const int groups = (n_in + TSZ - 1) / TSZ;
+      const int last = groups == 1;
+      hipLaunchKernelGGL(tslu_round_kernel<TSZ>, dim3(groups, batch), dim3(TSZ), 0, h->stream, (const double2*)A, lda,
+                         sa,
+                         n, kb, nbc, in, n_in, rows[cur], last, lu11, top, lscr, gstride, ipiv, moved, d_info);
+      EUVL_LAUNCH_CHECK();
+      if (last) break;
+      const int tail = n_in - (groups - 1) * TSZ;
+      n_in = (groups - 1) * nbc + (tail < nbc ? tail : nbc);
+      in = rows[cur];
+      cur ^= 1;
+    }
+    hipLaunchKernelGGL(tslu_apply_kernel, dim3((m + 255) / 256, batch), dim3(256), 0, h->stream, A, lda, sa, kb, m,
+                       nbc, (const double2*)lu11, (const double2*)top, (const int*)moved);
     EUVL_LAUNCH_CHECK();
     hipLaunchKernelGGL(laswp_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, h->stream, n, A, lda, sa, ipiv,
                        (int64_t)n, kb, kb + nbc, kb, kb + nbc);
@@ -419,6 +438,51 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
       EUVL_LAUNCH_CHECK();
       EUVL_TRY(zgemm(h, rest, rest, nbc, mone, A21, lda, sa, A12, lda, sa, one, A22, lda, sa, zero, batch));
     }
+  }
+  return EUVL_OK;
+}
+
+int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info) {
+  PrecScope scope(h, EUVL_PREC_FP32_LU);
+  const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  // scratch: U11^-1 and L11^-1 (batch x NB x NB each), the L21 / U12 products (batch x n x NB)
+  const size_t b_blk = sizeof(double2) * (size_t)batch * NB * NB;
+  const size_t b_t = sizeof(double2) * (size_t)batch * n * NB;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  char* scr = (char*)aux_reserve(h, 2 * al(b_blk) + al(b_t));
+  if (!scr) {
+    set_error("zgetrf_nopiv: scratch allocation failed");
+    return EUVL_ERR_NOMEM;
+  }
+  double2* uinv = (double2*)scr;
+  double2* linv = (double2*)(scr + al(b_blk));
+  double2* T = (double2*)(scr + 2 * al(b_blk));
+  const int64_t st = (int64_t)n * NB;
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, h->stream, d_info, batch);
+  EUVL_LAUNCH_CHECK();
+  for (int kb = 0; kb < n; kb += NB) {
+    const int nbc = (n - kb < NB) ? n - kb : NB;
+    hipLaunchKernelGGL(diag_lu_kernel, dim3(batch), dim3(NB * NB), 0, h->stream, A, lda, sa, kb, nbc, uinv, linv,
+                       ipiv, n, d_info);
+    EUVL_LAUNCH_CHECK();
+    const int rest = n - kb - nbc;
+    if (rest <= 0) continue;
+    double2* A11 = A + (int64_t)kb * lda + kb;
+    double2* A12 = A11 + nbc;
+    double2* A21 = A11 + (int64_t)nbc * lda;
+    double2* A22 = A21 + nbc;
+    const dim3 cg((unsigned)std::min<int64_t>(((int64_t)rest * nbc + 255) / 256, 4096), batch);
+    // L21 = A21 U11^-1 (rest x nbc, through T)
+    EUVL_TRY(zgemm(h, rest, nbc, nbc, one, A21, lda, sa, uinv, NB, (int64_t)NB * NB, zero, T, NB, st, zero, batch));
+    hipLaunchKernelGGL(copy2d_kernel, cg, dim3(256), 0, h->stream, rest, nbc, A21, lda, sa, (const double2*)T,
+                       (int64_t)NB, st);
+    EUVL_LAUNCH_CHECK();
+    // U12 = L11^-1 A12 (nbc x rest, through T)
+    EUVL_TRY(zgemm(h, nbc, rest, nbc, one, linv, NB, (int64_t)NB * NB, A12, lda, sa, zero, T, rest, st, zero, batch));
+    hipLaunchKernelGGL(copy2d_kernel, cg, dim3(256), 0, h->stream, nbc, rest, A12, lda, sa, (const double2*)T,
+                       (int64_t)rest, st);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, rest, rest, nbc, mone, A21, lda, sa, A12, lda, sa, one, A22, lda, sa, zero, batch));
   }
   return EUVL_OK;
 }
@@ -477,6 +541,26 @@ extern "C" int euvl_zgetrf_batched(euvl_handle h, int n, double* d_A, int64_t ld
   for (int b = 0; b < batch; ++b)
     if (hi[b] != 0) {
       euvl::set_error("euvl_zgetrf_batched: matrix %d has a zero pivot at column %d", b, hi[b] - 1);
+      return EUVL_ERR_SINGULAR;
+    }
+  return EUVL_OK;
+}
+
+extern "C" int euvl_zgetrf_nopiv_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t sa, int* d_ipiv,
+                                         int batch) {
+  if (!h || n < 0 || batch < 0 || batch > 1024 || lda < n) {
+    euvl::set_error("euvl_zgetrf_nopiv_batched: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  if (n == 0 || batch == 0) return EUVL_OK;
+  int* d_info = (int*)h->d_red;
+  EUVL_TRY(euvl::zgetrf_nopiv(h, n, (double2*)d_A, lda, sa, d_ipiv, batch, d_info));
+  int* hi = (int*)h->h_pinned;
+  EUVL_HIP_CHECK(hipMemcpyAsync(hi, d_info, sizeof(int) * batch, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int b = 0; b < batch; ++b)
+    if (hi[b] != 0) {
+      euvl::set_error("euvl_zgetrf_nopiv_batched: matrix %d has a zero pivot at column %d", b, hi[b] - 1);
       return EUVL_ERR_SINGULAR;
     }
   return EUVL_OK;

@@ -91,6 +91,23 @@ class Engine:
         except Exception:
             pass
 
+    def autotune(self):
+        """Time the GEMM kernel variants on this problem's main product shape
+        (N x N x N, batched over the source offsets) in FP64 and, for the
+        precision form, in FP32 compute; the fastest is kept for that size
+        bucket by every later GEMM of the process.  Returns
+        {"fp64" | "fp32": (variant, [ms per variant])}."""
+        n, b = len(self.lidx), len(self.cfg.offsets())
+        out = {}
+        for fp32 in ((0, 1) if self.precision else (0,)):
+            v = ctypes.c_int()
+            ms = (ctypes.c_double * 5)()
+            self._bind_stream()
+            _lib.check(self.lib.euvl_autotune_gemm(self.h, n, n, n, b, fp32, ctypes.byref(v), ms),
+                       "euvl_autotune_gemm")
+            out["fp32" if fp32 else "fp64"] = (v.value, list(ms))
+        return out
+
     def _bind_stream(self):
         s = torch.cuda.current_stream(self.device).cuda_stream
         _lib.check(self.lib.euvl_set_stream(self.h, ctypes.c_void_p(s)), "euvl_set_stream")
@@ -105,11 +122,22 @@ class Engine:
         return torch.from_numpy(raw.copy()).to(self.device)
 
     def mask_spectrum(self, bits, lw, mw, n_clips=1):
-        """Indicator spectrum window (n_clips, 2lw+1, 2mw+1) complex128 (euvl_mask_spectrum)."""
+        """Indicator spectrum window (n_clips, 2lw+1, 2mw+1) complex128 (euvl_mask_spectrum).
+
+        The last window computed is remembered with the mask tensor it came
+        from (object and in-place version): the M3D fit and the images ask for
+        the pupil window, which lies inside the EM solve's window of the same
+        clip, and get a slice of it instead of a second transform."""
+        last = getattr(self, "_spec_last", None)
+        if last is not None:
+            b, ver, n, lw0, mw0, spec = last
+            if b is bits and ver == bits._version and n == n_clips and lw <= lw0 and mw <= mw0:
+                return spec[:, lw0 - lw:lw0 + lw + 1, mw0 - mw:mw0 + mw + 1].contiguous()
         out = torch.empty((n_clips, 2 * lw + 1, 2 * mw + 1), dtype=torch.complex128, device=self.device)
         self._bind_stream()
         _lib.check(self.lib.euvl_mask_spectrum(self.h, _lib.ptr(bits), self.cfg.ndivx, self.cfg.ndivy, n_clips,
                                                lw, mw, _lib.ptr(out)), "euvl_mask_spectrum")
+        self._spec_last = (bits, bits._version, n_clips, lw, mw, out)
         return out
 
     # ---------------------------------------------------------------- EM solve

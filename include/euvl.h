@@ -77,6 +77,17 @@ int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 #define EUVL_PREC_FP32_GEMM 8
 int euvl_set_precision(euvl_handle h, int mask);
 
+/* GEMM autotuner: times every kernel variant (tile shape, waves per
+ * workgroup) of the m x n x k x batch product in FP64 (fp32 = 0) or
+ * FP32-compute (fp32 = 1) on scratch operands, and keeps the fastest for that
+ * size bucket (m, n, k rounded up to 64 below 256 and to 128 above; batch
+ * exact) for every later GEMM of the process.  *out_variant receives the
+ * choice; out_ms (optional, 5 doubles) the time per variant.  Allocates and
+ * frees its scratch; synchronises the handle's stream.  No reference
+ * counterpart. */
+int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch, int fp32, int* out_variant,
+                       double* out_ms);
+
 /* ---------------------------------------------------------------------------
  * Dense complex linear algebra (batched, row-major, interleaved complex).
  * ------------------------------------------------------------------------- */
@@ -100,6 +111,16 @@ int euvl_zgemm_batched(euvl_handle h, int m, int n, int k,
  * Returns EUVL_ERR_SINGULAR if a pivot is exactly zero. */
 int euvl_zgetrf_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t sa,
                         int* d_ipiv, int batch);
+
+/* In-place LU without row interchanges (d_ipiv receives the identity), for
+ * matrices whose unpivoted elimination is stable (diagonally dominant, or
+ * well conditioned with small growth, as the EM top-solve operator): each
+ * 32-wide panel is its diagonal block factored in LDS plus GEMMs, with no
+ * pivot search.  Returns EUVL_ERR_SINGULAR on an exactly zero pivot; it does
+ * not detect growth, so callers verify what they solve with it.  No reference
+ * counterpart (the reference always pivots). */
+int euvl_zgetrf_nopiv_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t sa, int* d_ipiv,
+                              int batch);
 
 /* Solve A[b] X = B[b] in place of B (n x nrhs, leading dim ldb) using the
  * factors of euvl_zgetrf_batched.  Replaces cusolverDnXgetrs in matinv
@@ -165,8 +186,10 @@ int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n,
  * Output d_Ax: n_off x (2 lsmax_x+1) x (2 lsmax_y+1) x nrange complex;
  * the entry [o][ls+lsmax_x][ms+lsmax_y][:] is Ax of header.h:1394-1465 for
  * the lattice incidence (ls, ms) inside the NA and 0 elsewhere.
- * h_info (optional, 4 ints per offset): 0, cos/sinc doublings, Neumann
- * factors applied, incidence columns solved.
+ * h_info (optional, 4 ints per offset): LU path of the top solve (0: without
+ * row interchanges, verified by its residual; 1: the partial-pivoting
+ * fallback), cos/sinc doublings, Neumann factors applied, incidence columns
+ * solved.
  * Replaces ampS (cnn/include/header.h:1208-1466) with absorberS0
  * (header.h:754-1145) and the top solve (header.h:1355-1391). */
 int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p,

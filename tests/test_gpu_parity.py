@@ -91,6 +91,36 @@ def test_getrf_getrs(lib, handle, n, nrhs, batch):
     assert np.abs(X - ref).max() < 1e-10 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2), (1681, 12, 2)])
+def test_getrf_nopiv_getrs(lib, handle, n, nrhs, batch):
+    """Unpivoted LU on well-conditioned, not diagonally dominant matrices
+    (identity plus a random part of norm ~0.5, like the top-solve operator)."""
+    say(f"getrf_nopiv n={n} nrhs={nrhs} batch={batch}")
+    rng = np.random.default_rng(n + 7)
+    A = np.eye(n)[None] + 0.5 * crand(rng, batch, n, n) / np.sqrt(2 * n)
+    Bm = crand(rng, batch, n, nrhs)
+    dA, dB = dev(A), dev(Bm)
+    ipiv = torch.empty((batch, n), dtype=torch.int32, device="cuda")
+    assert lib.euvl_zgetrf_nopiv_batched(handle, n, P(dA), n, n * n, P(ipiv), batch) == 0, err(lib)
+    assert np.array_equal(ipiv.cpu().numpy(), np.tile(np.arange(n, dtype=np.int32), (batch, 1)))
+    assert lib.euvl_zgetrs_batched(handle, n, nrhs, P(dA), n, n * n, P(ipiv), P(dB), nrhs, n * nrhs, batch) == 0
+    torch.cuda.synchronize()
+    X = dB.cpu().numpy()
+    for b in range(batch):
+        ref = np.linalg.solve(A[b], Bm[b])
+        assert np.abs(X[b] - ref).max() < 1e-12 * np.abs(ref).max()
+    # the packed factors reproduce A
+    LU = dA.cpu().numpy()[0]
+    L = np.tril(LU, -1) + np.eye(n)
+    assert np.abs(L @ np.triu(LU) - A[0]).max() < 1e-12
+
+
+def test_getrf_nopiv_zero_pivot_reports(lib, handle):
+    A = dev(np.array([[0.0, 1.0], [1.0, 0.0]], dtype=np.complex128))
+    ipiv = torch.empty(2, dtype=torch.int32, device="cuda")
+    assert lib.euvl_zgetrf_nopiv_batched(handle, 2, P(A), 2, 4, P(ipiv), 1) == -3
+
+
 def test_getrf_singular_reports(lib, handle):
     say("getrf singular")
     A = np.zeros((1, 8, 8), complex)
@@ -183,6 +213,7 @@ def test_amp_s_c1_vs_oracle(pol):
     offs = cfg.offsets()[[0, 3], :2]
     Ax = eng.amp_s(eng.upload_mask(m), pol, offs).cpu().numpy()
     say("  info", eng.last_info.tolist())
+    assert (eng.last_info[:, 0] == 0).all()  # the unpivoted top solve passed its residual check
     p, d = oracle_problem(cfg)
     for o in range(2):
         ref = OE.amp_s(pol, m, p, d, eng.lidx.astype(np.int64), eng.midx.astype(np.int64), eng.L, eng.M,

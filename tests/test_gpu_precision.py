@@ -98,3 +98,26 @@ def test_precision_form_clip_vs_fp64(lib):
     assert ea < 1e-3 and e0 < 1e-3
     assert ei < 1e-3 and es < 1e-3
     assert ei > 0.0  # the mixed form is a different computation
+
+
+def test_autotune_keeps_results(lib):
+    """The autotuner picks a valid variant per size bucket; results after
+    tuning equal the untuned ones (every variant sums k in the same order)."""
+    from euvlitho_amd.config import preset
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from euvlitho_amd import _lib
+    cfg = preset("c1")
+    m = manhattan(cfg.ndivx, cfg.ndivy, 0.5, 51)
+    eng = Engine(cfg, precision=_lib.PREC_MIXED)
+    bits = eng.upload_mask(m)
+    before = eng.amp_s(bits, "X").cpu().numpy()
+    tuned = eng.autotune()
+    say(f"  tuned {tuned}")
+    for key in ("fp64", "fp32"):
+        v, ms = tuned[key]
+        assert 0 <= v < 5 and all(t > 0 for t in ms) and ms[v] == min(ms)
+    after = eng.amp_s(bits, "X").cpu().numpy()
+    e = np.abs(after - before).max() / np.abs(before).max()
+    say(f"  amplitudes after tuning vs before: {e:.1e}")
+    assert e < 1e-12
