@@ -1,6 +1,9 @@
 // Batched complex GEMM on CDNA4 MFMA, FP64 storage.
 //
 // C = alpha*A*B + beta*C + gamma*I, row-major, interleaved complex double.
+// C may overlap A (or B) when every output tile spans all columns (rows) of
+// C: a workgroup then reads only what it overwrites, and all of its A and B
+// loads precede its epilogue (the LU's in-place panel products rely on this).
 // A workgroup of WM x WN waves computes a (16 TM WM) x (16 TN WN) tile; each
 // wave owns TM x TN MFMA tiles of 16 x 16 with separate real/imag accumulators
 // (4 real MFMAs per complex MFMA step: C_re += A_re B_re - A_im B_im,
@@ -54,7 +57,7 @@ struct F32 {
 template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
 __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     int M, int N, int K, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sa,
-    const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* __restrict__ C,
+    const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* C,
     int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles) {
   constexpr int NT = 64 * WM * WN;                      // WM x WN waves
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;   // workgroup tile

@@ -243,69 +243,75 @@ __global__ __launch_bounds__(256) void tslu_apply_kernel(double2* __restrict__ A
 // GEMMs (L21 = A21 U11^-1, U12 = L11^-1 A12), as does the trailing update.
 // No column step of the factorisation runs over the panel's full height, so
 // a panel costs a few short launches instead of 32 dependent pivot searches.
-__global__ __launch_bounds__(NB * NB) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
-                                                          int nbc, double2* __restrict__ uinv,
-                                                          double2* __restrict__ linv, int* __restrict__ ipiv, int n,
-                                                          int* __restrict__ info) {
+__global__ __launch_bounds__(64) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
+                                                     int nbc, double2* __restrict__ uinv, double2* __restrict__ linv,
+                                                     int* __restrict__ ipiv, int n, int* __restrict__ info) {
+  // one wave: lane r < 32 holds row r of the block in registers; the pivot
+  // row of each step is broadcast through LDS (no barrier needed in a single
+  // wave); then lanes 0-31 form the columns of U11^-1 and lanes 32-63 those
+  // of L11^-1, reading the factors from LDS
   __shared__ double2 a[NB][NB + 1];
-  __shared__ double2 x[NB][NB + 1];
-  const int b = blockIdx.x, tid = threadIdx.x, r = tid / NB, c = tid % NB;
+  const int b = blockIdx.x, lane = threadIdx.x;
   double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
-  {
-    const bool in = r < nbc && c < nbc;
-    const double2 v = A11[(int64_t)min(r, nbc - 1) * lda + min(c, nbc - 1)];
-    const double pad = (r == c) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
-    a[r][c] = make_double2(in ? v.x : pad, in ? v.y : 0.0);
+  const int r = lane & (NB - 1);
+  double2 x[NB];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const bool in = r < nbc && q < nbc;
+    const double2 v = A11[(int64_t)min(r, nbc - 1) * lda + min(q, nbc - 1)];
+    const double pad = (r == q) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
+    x[q] = make_double2(in ? v.x : pad, in ? v.y : 0.0);
   }
-#pragma unroll 1
+#pragma unroll
   for (int j = 0; j < NB; ++j) {
-    __syncthreads();
-    if (c == j && r > j) {
-      const double2 d = a[j][j];
-      if (d.x != 0.0 || d.y != 0.0) a[r][j] = cdiv(a[r][j], d);
+    if (lane == j) {
+#pragma unroll
+      for (int q = 0; q < NB; ++q) a[j][q] = x[q];  // row j is final: U(j, j:) and L(j, :j)
     }
-    __syncthreads();
-    if (r > j && c > j) a[r][c] = csub(a[r][c], cmul(a[r][j], a[j][c]));
+    __syncthreads();  // one wave: orders the LDS write before the reads
+    const double2 d = a[j][j];
+    if (lane < NB && r > j && (d.x != 0.0 || d.y != 0.0)) {
+      const double2 l = cdiv(x[j], d);
+      x[j] = l;
+#pragma unroll
+      for (int q = j + 1; q < NB; ++q) x[q] = csub(x[q], cmul(l, a[j][q]));
+    }
   }
   __syncthreads();
-  if (r < nbc && c < nbc) A11[(int64_t)r * lda + c] = a[r][c];
-  if (tid < nbc) ipiv[(int64_t)b * n + kb + tid] = kb + tid;
-  if (tid == 0)
+  if (lane < nbc) {
+#pragma unroll
+    for (int q = 0; q < NB; ++q)
+      if (q < nbc) A11[(int64_t)lane * lda + q] = a[lane][q];
+    ipiv[(int64_t)b * n + kb + lane] = kb + lane;
+  }
+  if (lane == 0)
     for (int j = 0; j < nbc; ++j)
       if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
-  // U11^-1, one column per thread: X(i, col) for i = col .. 0
-  if (tid < NB) {
-    const int col = tid;
+  // column c of U11^-1 (lanes 0-31): y(i) for i = c .. 0, y(i) = (e_c(i) - sum_{k>i} U(i,k) y(k)) / U(i,i);
+  // column c of L11^-1 (lanes 32-63): y(i) for i = c .. NB-1, y(i) = e_c(i) - sum_{k<i} L(i,k) y(k)
+  const int c = r;
+  double2 y[NB];
+  if (lane < NB) {
+#pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
-      double2 t = make_double2(i == col ? 1.0 : 0.0, 0.0);
-      for (int k = i + 1; k <= col; ++k) t = csub(t, cmul(a[i][k], x[k][col]));
-      x[i][col] = (i <= col) ? cdiv(t, a[i][i]) : make_double2(0.0, 0.0);
+      double2 t = make_double2(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll
+      for (int k = i + 1; k < NB; ++k) t = csub(t, cmul(a[i][k], y[k]));
+      const double2 u = cdiv(t, a[i][i]);
+      y[i] = make_double2(i <= c ? u.x : 0.0, i <= c ? u.y : 0.0);
     }
-  }
-  __syncthreads();
-  uinv[((int64_t)b * NB + r) * NB + c] = x[r][c];
-  __syncthreads();
-  // L11^-1 (unit lower), one column per thread: Y(row, col) for row = col .. NB-1
-  if (tid < NB) {
-    const int col = tid;
-    for (int row = 0; row < NB; ++row) {
-      double2 t = make_double2(row == col ? 1.0 : 0.0, 0.0);
-      for (int k = col; k < row; ++k) t = csub(t, cmul(a[row][k], x[k][col]));
-      x[row][col] = (row >= col) ? t : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) uinv[((int64_t)b * NB + i) * NB + c] = y[i];
+  } else {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      double2 t = make_double2(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < i; ++k) t = csub(t, cmul(a[i][k], y[k]));
+      y[i] = make_double2(i >= c ? t.x : 0.0, i >= c ? t.y : 0.0);
     }
-  }
-  __syncthreads();
-  linv[((int64_t)b * NB + r) * NB + c] = x[r][c];
-}
-
-// out[b][i][j] = in[b][i][j] for an m x ncols block (leading dims ldo, ldi)
-__global__ void copy2d_kernel(int m, int ncols, double2* __restrict__ out, int64_t ldo, int64_t so,
-                              const double2* __restrict__ in, int64_t ldi, int64_t si) {
-  const int b = blockIdx.y;
-  const int64_t total = (int64_t)m * ncols;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = e / ncols, j = e % ncols;
-    out[(int64_t)b * so + i * ldo + j] = in[(int64_t)b * si + i * ldi + j];
+#pragma unroll
+    for (int i = 0; i < NB; ++i) linv[((int64_t)b * NB + i) * NB + c] = y[i];
   }
 }
 
@@ -445,25 +451,22 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
 int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info) {
   PrecScope scope(h, EUVL_PREC_FP32_LU);
   const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
-  // scratch: U11^-1 and L11^-1 (batch x NB x NB each), the L21 / U12 products (batch x n x NB)
+  // scratch: U11^-1 and L11^-1 (batch x NB x NB each)
   const size_t b_blk = sizeof(double2) * (size_t)batch * NB * NB;
-  const size_t b_t = sizeof(double2) * (size_t)batch * n * NB;
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
-  char* scr = (char*)aux_reserve(h, 2 * al(b_blk) + al(b_t));
+  char* scr = (char*)aux_reserve(h, 2 * al(b_blk));
   if (!scr) {
     set_error("zgetrf_nopiv: scratch allocation failed");
     return EUVL_ERR_NOMEM;
   }
   double2* uinv = (double2*)scr;
   double2* linv = (double2*)(scr + al(b_blk));
-  double2* T = (double2*)(scr + 2 * al(b_blk));
-  const int64_t st = (int64_t)n * NB;
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, h->stream, d_info, batch);
   EUVL_LAUNCH_CHECK();
   for (int kb = 0; kb < n; kb += NB) {
     const int nbc = (n - kb < NB) ? n - kb : NB;
-    hipLaunchKernelGGL(diag_lu_kernel, dim3(batch), dim3(NB * NB), 0, h->stream, A, lda, sa, kb, nbc, uinv, linv,
-                       ipiv, n, d_info);
+    hipLaunchKernelGGL(diag_lu_kernel, dim3(batch), dim3(64), 0, h->stream, A, lda, sa, kb, nbc, uinv, linv, ipiv,
+                       n, d_info);
     EUVL_LAUNCH_CHECK();
     const int rest = n - kb - nbc;
     if (rest <= 0) continue;
@@ -471,17 +474,12 @@ int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int*
     double2* A12 = A11 + nbc;
     double2* A21 = A11 + (int64_t)nbc * lda;
     double2* A22 = A21 + nbc;
-    const dim3 cg((unsigned)std::min<int64_t>(((int64_t)rest * nbc + 255) / 256, 4096), batch);
-    // L21 = A21 U11^-1 (rest x nbc, through T)
-    EUVL_TRY(zgemm(h, rest, nbc, nbc, one, A21, lda, sa, uinv, NB, (int64_t)NB * NB, zero, T, NB, st, zero, batch));
-    hipLaunchKernelGGL(copy2d_kernel, cg, dim3(256), 0, h->stream, rest, nbc, A21, lda, sa, (const double2*)T,
-                       (int64_t)NB, st);
-    EUVL_LAUNCH_CHECK();
-    // U12 = L11^-1 A12 (nbc x rest, through T)
-    EUVL_TRY(zgemm(h, nbc, rest, nbc, one, linv, NB, (int64_t)NB * NB, A12, lda, sa, zero, T, rest, st, zero, batch));
-    hipLaunchKernelGGL(copy2d_kernel, cg, dim3(256), 0, h->stream, nbc, rest, A12, lda, sa, (const double2*)T,
-                       (int64_t)rest, st);
-    EUVL_LAUNCH_CHECK();
+    // L21 = A21 U11^-1 and U12 = L11^-1 A12 in place: with nbc <= 32 columns
+    // (rows) every GEMM tile spans all of them, so each workgroup reads the
+    // rows (columns) it overwrites and no other, and it has read all of them
+    // (K = nbc, at most two slabs) before its epilogue writes
+    EUVL_TRY(zgemm(h, rest, nbc, nbc, one, A21, lda, sa, uinv, NB, (int64_t)NB * NB, zero, A21, lda, sa, zero, batch));
+    EUVL_TRY(zgemm(h, nbc, rest, nbc, one, linv, NB, (int64_t)NB * NB, A12, lda, sa, zero, A12, lda, sa, zero, batch));
     EUVL_TRY(zgemm(h, rest, rest, nbc, mone, A21, lda, sa, A12, lda, sa, one, A22, lda, sa, zero, batch));
   }
   return EUVL_OK;
