@@ -250,7 +250,11 @@ def main():
                      "launches_per_clip": int(launches),
                      "avg_launch_us": 1000.0 * gemm_ms / max(launches, 1),
                      "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms,
-                     "autotuned_variant": f64["tuned"]["fp64"][0]},
+                     "autotuned_variant": f64["tuned"]["fp64"][0],
+                     # achieved counts 8mnk flops per complex product; the 3M form issues 6mnk on the MFMAs
+                     "flops_counted": "8mnk per complex GEMM",
+                     "complex_form": "3M (3 real MFMAs per complex product)",
+                     "mfma_issue_frac": 0.75 * achieved / FP64_PEAK_TFLOPS},
         "forms": {"mixed": mixed} if mixed else {},
     }
     if args.served > 0:

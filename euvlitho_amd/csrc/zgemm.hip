@@ -10,8 +10,16 @@
 // C_im += A_re B_im + A_im B_re).  K is staged through LDS in slabs of BK
 // with a register prefetch of the next slab.
 //
+// The FP64 kernel can also form complex products with 3 real MFMAs (the
+// 3M / Karatsuba form of zgemm3m): P1 = A_re B_re, P2 = A_im B_im,
+// P3 = (A_re + A_im)(B_re + B_im), C_re = P1 - P2, C_im = P3 - P1 - P2.
+// A quarter fewer MFMAs at the cost of a third accumulator; its rounding error
+// is bounded like the 4M form's, in norm ((|A_re| + |A_im|)(|B_re| + |B_im|)
+// in place of |A||B|), which is what every product of the EM solve needs.
+//
 // Two compute precisions share the kernel:
-//   F64: v_mfma_f64_16x16x4_f64, FP64 accumulation (the FP64 drop-in);
+//   F64: v_mfma_f64_16x16x4_f64, FP64 accumulation (the FP64 drop-in), in the
+//        3M form (F64M3, the default) or the 4M form (F64, EUVL_PREC_GEMM_4M);
 //   F32: operands rounded to FP32 when staged into LDS, v_mfma_f32_16x16x4_f32
 //        with FP32 accumulation over K, FP64 epilogue (the precision form:
 //        twice the MFMA rate of FP64 on MI355X, half the LDS and registers).
@@ -38,16 +46,22 @@ struct F64 {
   using T = double;
   using T2 = double2;
   using V4 = double4_t;
+  static constexpr bool three = false;
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int r, int lk) { return 4 * r + lk; }
 };
 
+struct F64M3 : F64 {
+  static constexpr bool three = true;
+};
+
 struct F32 {
   using T = float;
   using T2 = float2;
   using V4 = float4_t;
+  static constexpr bool three = false;
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -123,7 +137,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     }
   };
 
-  V4 acc_re[TM][TN], acc_im[TM][TN];
+  // 4M: acc_re, acc_im hold C_re, C_im.  3M: acc_re = P1, acc_im = P2, acc_s = P3.
+  constexpr bool M3 = P::three;
+  constexpr int SM = M3 ? TM : 1, SN = M3 ? TN : 1;
+  V4 acc_re[TM][TN], acc_im[TM][TN], acc_s[SM][SN];
 #pragma unroll
   for (int t = 0; t < TM; ++t)
 #pragma unroll
@@ -131,6 +148,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
       acc_re[t][u] = (V4){0, 0, 0, 0};
       acc_im[t][u] = (V4){0, 0, 0, 0};
     }
+#pragma unroll
+  for (int t = 0; t < SM; ++t)
+#pragma unroll
+    for (int u = 0; u < SN; ++u) acc_s[t][u] = (V4){0, 0, 0, 0};
 
   load_slab(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
@@ -152,6 +173,29 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
       }
 #pragma unroll
       for (int u = 0; u < TN; ++u) fb[u] = Bs[ks + lk][wn * 16 * TN + u * 16 + li];
+      if constexpr (M3) {
+        T sa[TM], sb[TN];
+#pragma unroll
+        for (int t = 0; t < TM; ++t) sa[t] = fa[t].x + fa[t].y;
+#pragma unroll
+        for (int u = 0; u < TN; ++u) sb[u] = fb[u].x + fb[u].y;
+#pragma unroll
+        for (int t = 0; t < TM; ++t)
+#pragma unroll
+          for (int u = 0; u < TN; ++u)
+            acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
+#pragma unroll
+        for (int t = 0; t < TM; ++t)
+#pragma unroll
+          for (int u = 0; u < TN; ++u)
+            acc_im[t][u] = P::mfma(fa[t].y, fb[u].y, acc_im[t][u]);
+#pragma unroll
+        for (int t = 0; t < SM; ++t)
+#pragma unroll
+          for (int u = 0; u < SN; ++u)
+            acc_s[t][u] = P::mfma(sa[t], sb[u], acc_s[t][u]);
+        continue;
+      }
       // four passes over the tiles so that successive updates of one
       // accumulator are separated by independent MFMAs
 #pragma unroll
@@ -187,7 +231,13 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
         const int gr = row0 + wm * 16 * TM + t * 16 + P::row(r, lk);
         const int gc = col0 + wn * 16 * TN + u * 16 + li;
         if (gr < M && gc < N) {
-          double2 v = cmul(alpha, make_double2((double)acc_re[t][u][r], (double)acc_im[t][u][r]));
+          double2 v;
+          if constexpr (M3) {
+            const double p1 = acc_re[t][u][r], p2 = acc_im[t][u][r];
+            v = cmul(alpha, make_double2(p1 - p2, (double)acc_s[t % SM][u % SN][r] - p1 - p2));
+          } else {
+            v = cmul(alpha, make_double2((double)acc_re[t][u][r], (double)acc_im[t][u][r]));
+          }
           double2* cp = C + (int64_t)gr * ldc + gc;
           if (use_beta) v = cadd(v, cmul(beta, *cp));
           if (gr == gc) v = cadd(v, gamma);
@@ -204,7 +254,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 // 54/68/66 (earlier 2x2-wave 128x64 and 128x128 tiles with 1 wave/SIMD reached
 // 32-40, and a K slab of 32 48-55).  The FP64 kernel is bound by the MFMA
 // issue rate, not by operand bytes, so 1 stays the default; EUVL_GEMM_TILE
-// overrides it for tuning runs.
+// overrides it for tuning runs.  The 3M form has the same variants except 3
+// (4x4 waves of 2x1 tiles, 128x64: three accumulator sets of 2x2 tiles spill
+// at 1024 threads); in complex-product TFLOP/s (8mnk per time) at 1681 x 4 /
+// 4096: 58/72, 66/84, 58/73, 63/77, 56/69, against 55/69 for the 4M default.
 // FP32-compute variants (waves WM x WN, each TM x TN MFMA tiles, K slab 16):
 // 0 = 2x2 waves, 64x64; 1 = 2x4 waves, 64x128; 2 = 4x2 waves, 128x64;
 // 3 = 4x4 waves, 128x128; 4 = 2x2 waves of 4x2 tiles, 128x64.
@@ -227,21 +280,21 @@ std::mutex g_tuned_mu;
 
 int bucket(int x) { return x < 256 ? ((x + 63) / 64) * 64 : ((x + 127) / 128) * 128; }
 
-std::tuple<int, int, int, int, int> tune_key(int fp32, int m, int n, int k, int batch) {
-  return std::make_tuple(fp32, bucket(m), bucket(n), bucket(k), batch);
+std::tuple<int, int, int, int, int> tune_key(int kind, int m, int n, int k, int batch) {
+  return std::make_tuple(kind, bucket(m), bucket(n), bucket(k), batch);
 }
 
-int tuned_variant(int fp32, int m, int n, int k, int batch) {
+int tuned_variant(int kind, int m, int n, int k, int batch) {
   std::lock_guard<std::mutex> lock(g_tuned_mu);
   if (g_tuned.empty()) return -1;
-  auto it = g_tuned.find(tune_key(fp32, m, n, k, batch));
+  auto it = g_tuned.find(tune_key(kind, m, n, k, batch));
   return it == g_tuned.end() ? -1 : it->second;
 }
 
-int pick_variant(int m, int n, int k, int batch) {
+int pick_variant(int kind, int m, int n, int k, int batch) {
   if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 4);
   if (getenv("EUVL_GEMM_TILE")) return g_variant;
-  const int t = tuned_variant(0, m, n, k, batch);
+  const int t = tuned_variant(kind, m, n, k, batch);
   return t >= 0 ? t : g_variant;
 }
 
@@ -271,11 +324,25 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles);
 }
 
-void launch_variant(euvl_handle h, int fp32, int v, int m, int n, int k, double2 alpha, const double2* A,
+// kind: 0 = FP64 4M, 1 = FP32 compute, 2 = FP64 3M
+int gemm_kind(euvl_handle h) {
+  if (h->prec_mask & h->prec_scope) return 1;
+  return (h->prec_mask & EUVL_PREC_GEMM_4M) ? 0 : 2;
+}
+
+void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2 alpha, const double2* A,
                     int64_t lda, int64_t sa, const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C,
                     int64_t ldc, int64_t sc, double2 gamma, int batch) {
 #define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch
-  if (fp32) {
+  if (kind == 2) {
+    switch (v) {
+      case 0: launch<F64M3, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F64M3, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F64M3, 4, 4, 2, 1, 16, 1>(EUVL_GEMM_ARGS); break;  // 128x64: 3 accumulator sets of 2x2 spill at 1024 threads
+      case 4: launch<F64M3, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      default: launch<F64M3, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
+    }
+  } else if (kind == 1) {
     switch (v) {
       case 1: launch<F32, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
       case 2: launch<F32, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
@@ -320,12 +387,9 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     h->ev_next += 2;
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev], h->stream));
   }
-  if (h->prec_mask & h->prec_scope)
-    launch_variant(h, 1, pick_variant32(m, n, k, batch), m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
-                   gamma, batch);
-  else
-    launch_variant(h, 0, pick_variant(m, n, k, batch), m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc,
-                   gamma, batch);
+  const int kind = gemm_kind(h);
+  launch_variant(h, kind, kind == 1 ? pick_variant32(m, n, k, batch) : pick_variant(kind, m, n, k, batch), m, n, k,
+                 alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch);
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev + 1], h->stream));
@@ -397,6 +461,8 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
     euvl::set_error("euvl_autotune_gemm: allocation failed");
     return EUVL_ERR_NOMEM;
   }
+  // FP64 variants are tuned in the product form the handle's mask selects
+  const int kind = fp32 ? 1 : ((h->prec_mask & EUVL_PREC_GEMM_4M) ? 0 : 2);
   int rc = EUVL_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double best_ms = 1e300;
@@ -412,12 +478,12 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
     }
     const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
     for (int v = 0; v < N_VARIANTS && rc == EUVL_OK; ++v) {
-      launch_variant(h, fp32, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
+      launch_variant(h, kind, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
                      (int64_t)m * n, zero, batch);
       const int reps = 3;
       if (hipEventRecord(e0, h->stream) != hipSuccess) rc = EUVL_ERR_HIP;
       for (int r = 0; r < reps; ++r)
-        launch_variant(h, fp32, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
+        launch_variant(h, kind, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
                        (int64_t)m * n, zero, batch);
       if (hipGetLastError() != hipSuccess || hipEventRecord(e1, h->stream) != hipSuccess ||
           hipEventSynchronize(e1) != hipSuccess) {
@@ -441,7 +507,7 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
   if (rc != EUVL_OK) return rc;
   {
     std::lock_guard<std::mutex> lock(g_tuned_mu);
-    g_tuned[tune_key(fp32, m, n, k, batch)] = best;
+    g_tuned[tune_key(kind, m, n, k, batch)] = best;
   }
   if (out_variant) *out_variant = best;
   return EUVL_OK;

@@ -75,6 +75,13 @@ int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 #define EUVL_PREC_FP32_COUPLING 2
 #define EUVL_PREC_FP32_LU 4
 #define EUVL_PREC_FP32_GEMM 8
+/* FP64 products (those not moved to FP32) form each complex product with 3
+ * real MFMAs by default (the zgemm3m form: Re = ArBr - AiBi,
+ * Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi), a quarter fewer MFMAs than the
+ * textbook 4 with a rounding error bounded like theirs in norm
+ * ((|Ar| + |Ai|)(|Br| + |Bi|) in place of |A||B|).  EUVL_PREC_GEMM_4M selects
+ * the 4-multiplication form. */
+#define EUVL_PREC_GEMM_4M 16
 int euvl_set_precision(euvl_handle h, int mask);
 
 /* GEMM autotuner: times every kernel variant (tile shape, waves per

@@ -69,6 +69,75 @@ def test_zgemm_fp32_compute(lib, m, n, k, batch):
         lib.euvl_destroy(h)
 
 
+@pytest.mark.parametrize("m,n,k,batch", [(17, 33, 5, 2), (100, 70, 130, 3), (300, 257, 2100, 1), (1681, 12, 1681, 2),
+                                         (200, 300, 400, 5)])
+def test_zgemm_3m_form(lib, m, n, k, batch):
+    """FP64 products in the 3-multiplication form (the default) and in the
+    4-multiplication form (EUVL_PREC_GEMM_4M): both within the FP64 GEMM bar
+    of the exact product (1e-13 of its largest entry per sqrt(k)), and not
+    bit-identical to the 4M form (it really ran), on operands whose real and
+    imaginary parts differ in scale (the case where 3M's cancellation in the
+    imaginary part is largest)."""
+    from euvlitho_amd import _lib
+    say(f"3M zgemm m={m} n={n} k={k} batch={batch}")
+    rng = np.random.default_rng(7 * m + n + k)
+    A = 3.0 * rng.standard_normal((batch, m, k)) + 0.01j * rng.standard_normal((batch, m, k))
+    B = rng.standard_normal((batch, k, n)) - 2j * rng.standard_normal((batch, k, n))
+    C0 = rng.standard_normal((batch, m, n)) + 1j * rng.standard_normal((batch, m, n))
+    ref = (0.5 - 1j) * A @ B + (0.25 + 0.5j) * C0 + 2.0 * np.eye(m, n)[None]
+    h = ctypes.c_void_p()
+    assert lib.euvl_create(ctypes.byref(h)) == 0
+    out = {}
+    try:
+        for name, mask in (("4m", _lib.PREC_GEMM_4M), ("3m", _lib.PREC_FP64)):
+            dA, dB, dC = (torch.from_numpy(x).cuda() for x in (A, B, C0))
+            assert lib.euvl_set_precision(h, mask) == 0
+            al, be, ga = (ctypes.c_double * 2)(0.5, -1.0), (ctypes.c_double * 2)(0.25, 0.5), (ctypes.c_double * 2)(2, 0)
+            rc = lib.euvl_zgemm_batched(h, m, n, k, al, ctypes.c_void_p(dA.data_ptr()), k, m * k,
+                                        ctypes.c_void_p(dB.data_ptr()), n, k * n, be, ctypes.c_void_p(dC.data_ptr()),
+                                        n, m * n, ga, batch)
+            assert rc == 0, lib.euvl_last_error().decode()
+            torch.cuda.synchronize()
+            out[name] = dC.cpu().numpy()
+    finally:
+        lib.euvl_destroy(h)
+    tol = 1e-13 * np.abs(ref).max() * np.sqrt(k)
+    for name, got in out.items():
+        e = np.abs(got - ref).max()
+        say(f"  {name}: max err {e:.2e} (bar {tol:.2e}); imag part err {np.abs(got.imag - ref.imag).max():.2e}")
+        assert e < tol
+    assert np.abs(out["3m"] - out["4m"]).max() > 0.0
+
+
+def test_3m_form_clip_vs_4m(lib):
+    """512 nm clip through the EM solve, M3D fit and Abbe image with FP64
+    products in the 3M form (the default) against the 4M form: amplitudes, a0
+    and the image agree to 1e-9 of their largest value (the drop-in's FP64
+    bar against the reference is 1e-9 for amplitudes, 1e-6 of peak for
+    intensity)."""
+    from euvlitho_amd import _lib
+    from euvlitho_amd.config import preset
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    cfg = preset("c1")
+    m = manhattan(cfg.ndivx, cfg.ndivy, 0.5, 43)
+    out = {}
+    for name, prec in (("4m", _lib.PREC_GEMM_4M), ("3m", _lib.PREC_FP64)):
+        eng = Engine(cfg, precision=prec)
+        bits = eng.upload_mask(m)
+        Ax = eng.amp_s(bits, "X")
+        a0, ax, ay = eng.m3d(bits, Ax0=Ax[0])
+        img = eng.abbe(bits, Ax=Ax)
+        out[name] = [t.cpu().numpy() for t in (Ax, a0, img)]
+        eng.close()
+    (A4, a04, I4), (A3, a03, I3) = out["4m"], out["3m"]
+    ea = np.abs(A3 - A4).max() / np.abs(A4).max()
+    e0 = np.abs(a03 - a04).max() / np.abs(a04).max()
+    ei = np.abs(I3 - I4).max() / I4.max()
+    say(f"  3M vs 4M: amplitudes {ea:.2e}, a0 {e0:.2e}, Abbe image {ei:.2e}")
+    assert ea < 1e-9 and e0 < 1e-9 and ei < 1e-9
+
+
 def test_precision_form_clip_vs_fp64(lib):
     """512 nm clip, +-10 orders, 4 offsets: amplitudes, M3D parameters, Abbe
     and SOCS images of the mixed form against the FP64 drop-in."""
