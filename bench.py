@@ -191,9 +191,12 @@ def main():
         _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
         outs = [t.cpu().numpy() for t in step(bits[args.warmup])]
         _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
+        info = eng.last_info  # per offset: LU path, doublings, Neumann factors, incidences
+        em = {"lu_path": [int(x) for x in info[:, 0]], "doublings": int(info[:, 1].max()),
+              "neumann_factors": int(info[:, 2].max()), "incidences": int(info[:, 3].max())}
         eng.close()
         return dict(lat=lat, wall=wall, tcc_ms=tcc_ms, outs=outs, launches=stats[0], gemm_ms=stats[1],
-                    gemm_flops=stats[2], gemm_bytes=stats[3], tuned=tuned)
+                    gemm_flops=stats[2], gemm_bytes=stats[3], tuned=tuned, em=em)
 
     f64 = run_form(_lib.PREC_FP64)
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
@@ -241,7 +244,7 @@ def main():
         "config": {"workload": "plain FP64 drop-in, 1024x1024 nm clip, +-20 orders (N=1681), dipole+annular "
                                "190 source pts on 2x2 divisions (4 EM solves), M3D params + Abbe 256x256 + "
                                "24-kernel SOCS 256x256",
-                   "form": "fp64", "clips": args.steps, "socs_kernel_build_ms": tcc_ms},
+                   "form": "fp64", "clips": args.steps, "socs_kernel_build_ms": tcc_ms, "em_solve": f64["em"]},
         "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": traffic.get("traffic_bytes_per_launch"),

@@ -3,7 +3,8 @@
 // All three are even in sqrt(D), i.e. power series in D: no square root, no
 // branch choice and no eigendecomposition.  They are evaluated by a degree-9
 // Taylor series in x = h^2 D (h = d / 2^s, Paterson-Stockmeyer in blocks of
-// x^3) followed by s doublings C <- 2 C^2 - I, F <- 2 F C.
+// x^3) followed by s doublings C <- 2 C^2 - I, F <- 2 F C; s is set by the
+// norms of D^2 and D^3 rather than of D.
 #include <cmath>
 
 #include "ew.h"
@@ -29,35 +30,45 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     }
   }
   const int64_t sm = (int64_t)n * n;
-  // ||D||_inf bounds every ||D^j||^(1/j); choose s so that ||x||_inf <= 1.5,
-  // where the degree-9 remainder ||x||^10/20! is below 3e-17.
-  std::vector<double> nr(batch);
-  EUVL_TRY(reduce_to_host(h, 1, n, batch, D, nr.data()));
-  double nd = 0.0;
-  for (double v : nr) nd = std::max(nd, std::isnan(v) ? INFINITY : v);
-  if (!std::isfinite(nd)) {
-    set_error("cos_sinc: non-finite ||D||");
+  // Unscaled powers P2 = D^2, P3 = D^3 (the Paterson-Stockmeyer blocks).  For
+  // every j >= 2, ||D^j||^(1/j) <= alpha = max(||D^2||^(1/2), ||D^3||^(1/3))
+  // (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009), Lemma 4.1), which
+  // is often well below ||D||: s is the smallest with h^2 alpha <= 1.5, so that
+  // the degree-9 remainder, bounded by (h^2 alpha)^10 / 20!, is below 3e-17.
+  const double2* X1 = D;
+  double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *T = s[5];
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
+  EUVL_TRY(reduce(h, 1, n, batch, X2, h->d_red));
+  EUVL_TRY(reduce(h, 1, n, batch, X3, h->d_red + 256));
+  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 512, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  double alpha = 0.0;
+  for (int b = 0; b < batch; ++b) {
+    const double a2 = std::sqrt(h->h_pinned[b]), a3 = std::cbrt(h->h_pinned[256 + b]);
+    alpha = std::max(alpha, (std::isnan(a2) || std::isnan(a3)) ? INFINITY : std::max(a2, a3));
+  }
+  if (!std::isfinite(alpha)) {
+    set_error("cos_sinc: non-finite ||D^2|| or ||D^3||");
     return EUVL_ERR_NOT_CONVERGED;
   }
   int sq = 0;
-  while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * nd > 1.5 && sq < 60) ++sq;
+  while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > 1.5 && sq < 60) ++sq;
   if (doublings) *doublings = sq;
   const double hh = std::ldexp(d, -sq);
-  double2 *X1 = s[0], *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *T = s[5];
-  EUVL_TRY(ew_lincomb(h, n, batch, X1, re(hh * hh), D, ZERO, nullptr, ZERO, nullptr, ZERO));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X1, n, sm, X1, n, sm, ZERO, X2, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, X1, n, sm, ZERO, X3, n, sm, ZERO, batch));
+  // q(x) = sum c_j x^j with x = h^2 D is evaluated as sum (c_j h^(2j)) D^j on the unscaled powers
   double cc[10], cf[10];
   double fact = 1.0;  // m!
   for (int m = 0; m < 20; ++m) {
     if (m > 0) fact *= m;
     const double sgn = ((m / 2) % 2) ? -1.0 : 1.0;
+    const double scale = std::pow(hh * hh, m / 2);
     if (m % 2 == 0)
-      cc[m / 2] = sgn / fact;
+      cc[m / 2] = sgn * scale / fact;
     else
-      cf[m / 2] = sgn * hh / fact;
+      cf[m / 2] = sgn * hh * scale / fact;
   }
-  // q(x) = sum_{j<10} c_j x^j = Q0 + x^3 (Q1 + x^3 (Q2 + c9 x^3))
+  // q = Q0 + P3 (Q1 + P3 (Q2 + c9 P3)), Qi = c_3i I + c_3i+1 D + c_3i+2 P2
   auto poly = [&](const double* c, double2* out) -> int {
     EUVL_TRY(ew_lincomb(h, n, batch, H, re(c[9]), X3, re(c[8]), X2, re(c[7]), X1, re(c[6])));
     EUVL_TRY(ew_lincomb(h, n, batch, Q, re(c[5]), X2, re(c[4]), X1, ZERO, nullptr, re(c[3])));
@@ -69,7 +80,7 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   EUVL_TRY(poly(cc, C));
   EUVL_TRY(poly(cf, F));
   // doublings: F <- 2 F C, C <- 2 C^2 - I  (angle h -> 2h)
-  double2 *c = C, *f = F, *tc = X1, *tf = X2;
+  double2 *c = C, *f = F, *tc = s[0], *tf = X2;
   for (int q = 0; q < sq; ++q) {
     EUVL_TRY(zgemm(h, n, n, n, re(2.0), f, n, sm, c, n, sm, ZERO, tf, n, sm, ZERO, batch));
     EUVL_TRY(zgemm(h, n, n, n, re(2.0), c, n, sm, c, n, sm, ZERO, tc, n, sm, re(-1.0), batch));
