@@ -123,6 +123,61 @@ def cpu_baseline(cfg, masks, threads):
                        f"({t3 - t2:.1f} s); {t3 - t0:.1f} s per clip; LAPACK threads = cores")
 
 
+def mask_producer(n_masks=64, ndiv=2048, reps=5, with_cpu=True):
+    """The line/space mask program (mask.cpp) on the GPU: n_masks 2048 x 2048
+    clips per call (host draws + one raster launch).  Raster roofline: the
+    kernel writes n_masks * ndiv^2 / 8 bytes and reads one pattern byte per
+    pixel from a pool of ~(ndiv/56) patterns per clip that stays in L2, so the
+    algorithmic HBM bytes are the packed output plus the per-clip column
+    table and pool.  CPU baseline: the reference program built from its own
+    sources (oracle/_ref/mask.out: one clip plus its maskimage.csv, as the
+    program always writes both), where it has been built."""
+    import ctypes
+    import subprocess
+    import tempfile
+    import torch
+    from euvlitho_amd import _lib
+    from euvlitho_amd.masks import line_space_bits, line_space_plan
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    _lib.check(lib.euvl_create(ctypes.byref(h)), "euvl_create")
+    out = line_space_bits(ndiv, ndiv, n_masks, 1, handle=h)  # warm-up (workspace)
+    torch.cuda.synchronize()
+    walls, kern = [], []
+    ms = ctypes.c_double()
+    for r in range(reps):
+        t0 = time.perf_counter()
+        out = line_space_bits(ndiv, ndiv, n_masks, 1 + r, handle=h)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        _lib.check(lib.euvl_mask_raster_ms(h, ctypes.byref(ms)), "euvl_mask_raster_ms")
+        kern.append(ms.value * 1e-3)
+    lib.euvl_destroy(h)
+    lays = line_space_plan(ndiv, ndiv, 1, 1)
+    pool = sum(len(p) for p in lays[0].patterns[0])  # bytes of the clip's pattern pool
+    algo = n_masks * (ndiv * ndiv // 8 + 4 * ndiv + pool)
+    wall, dt = float(np.median(walls)), float(np.median(kern))
+    res = {"value": n_masks / wall, "unit": "masks/s", "config": f"{n_masks} clips of {ndiv}x{ndiv} px per call "
+           f"(cd 56, gap 80: the reference mask program's clip)", "ms_per_call": 1000.0 * wall,
+           "kernel_ms_per_call": 1000.0 * dt,
+           "roofline": {"bound": "hbm", "kernel": "mask_raster_words_kernel", "achieved": algo / dt / 1e9,
+                        "peak": 8000.0, "unit": "GB/s", "frac": algo / dt / 1e9 / 8000.0,
+                        "algorithmic_bytes_per_launch": algo,
+                        "note": "the call's rate is set by the host draws and the layout upload"}}
+    ref = os.path.join(ROOT, "oracle", "_ref", "mask.out")
+    if with_cpu and os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as td:
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                subprocess.run([ref], cwd=td, check=True, timeout=120)
+                ts.append(time.perf_counter() - t0)
+        res["cpu_baseline"] = {"value": 1.0 / float(np.median(ts)), "unit": "masks/s", "cores": 1,
+                               "kind": "reference", "sample": "the reference mask program (one 2048 x 2048 clip "
+                               "and its maskimage.csv), median of 3 runs"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +190,8 @@ def main():
     ap.add_argument("--slots", type=int, default=4, help="streams of the served form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy that scales the HBM byte counters")
+    ap.add_argument("--no-mask-producer", action="store_true", help="skip the mask producer leg")
+    ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); clips are independent, so N GPUs "
@@ -143,6 +200,9 @@ def main():
 
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.mask_producer_only:
+        print(json.dumps({"mask_producer": mask_producer(with_cpu=not args.no_cpu_baseline)}), flush=True)
         return
     import ctypes
     import torch
@@ -215,6 +275,7 @@ def main():
             "p50_ms": 1000.0 * float(np.percentile(mix["lat"], 50)),
             "p99_ms": 1000.0 * float(np.percentile(mix["lat"], 99)),
             "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
+            "em_solve": mix["em"],
             "err_vs_fp64": {"abbe_image_of_peak": float(np.abs(I32 - I64).max() / I64.max()),
                             "socs_image_of_peak": float(np.abs(S32 - S64).max() / np.abs(S64).max()),
                             "amplitudes_of_max": float(np.abs(A32 - A64).max() / np.abs(A64).max()),
@@ -262,6 +323,8 @@ def main():
     }
     if args.served > 0:
         line["forms"]["served"] = run_served(args.served, args.slots)
+    if not args.no_mask_producer:
+        line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cpu_baseline(cfg, masks, threads)

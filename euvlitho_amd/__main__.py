@@ -5,6 +5,7 @@ reading and writing the same files in the working directory:
   python -m euvlitho_amd abbe    mask.bin -> emint.csv | emintY.csv     (intensity.cpp main)
   python -m euvlitho_amd linear  mask.bin + inputxx.csv -> intlinear.csv, intft.csv  (linear.py __main__)
   python -m euvlitho_amd socs    mask.bin + inputxx.csv -> intsocs.csv  (socs.py __main__)
+  python -m euvlitho_amd mask    -> mask.bin, maskimage.csv             (mask.cpp main)
 
 --preset picks the problem (reference: the drivers' own optics; c1, c2: the
 high-NA clips); --format g writes the reference C++ 6-digit numbers.
@@ -19,22 +20,45 @@ from .api import Session
 from .config import preset
 
 
+def write_masks(out, ndiv, ndata, seed, cd, gap):
+    """mask.cpp:28-62: ndata line/space clips (drawn on the host, rasterised on
+    the GPU) -> mask.bin; the first clip as 0/1 text -> maskimage.csv (row j
+    holds mask2d[NDIVX*j + i] for every i, each followed by a comma)."""
+    import numpy as np
+    from .masks import line_space_bits, unpack_bits
+    bits = line_space_bits(ndiv, ndiv, ndata, seed, cd, gap).cpu().numpy()
+    with open(out("mask.bin"), "wb") as f:
+        f.write(bits.tobytes())
+    m = unpack_bits(bits.tobytes(), ndiv, ndiv, 0).reshape(ndiv, ndiv)
+    lines = np.where(m == 1, "1,", "0,")
+    with open(out("maskimage.csv"), "w") as f:
+        f.write("\n".join("".join(row) for row in lines) + "\n")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m euvlitho_amd", description=__doc__.split("\n\n")[0])
-    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs"])
+    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs", "mask"])
     ap.add_argument("--preset", default="reference")
     ap.add_argument("--mask", default="mask.bin")
     ap.add_argument("--inputxx", default="inputxx.csv")
-    ap.add_argument("--ndata", type=int, default=1, help="m3d: clips in mask.bin")
+    ap.add_argument("--ndata", type=int, default=1, help="m3d, mask: clips in mask.bin")
+    ap.add_argument("--ndiv", type=int, default=2048, help="mask: clip size in pixels (NDIVX = NDIVY)")
+    ap.add_argument("--seed", type=int, default=1, help="mask: srand seed (the reference program's: 1)")
+    ap.add_argument("--cd", type=int, default=56, help="mask: column band width")
+    ap.add_argument("--gap", type=int, default=80, help="mask: line/space segment unit")
     ap.add_argument("--polar", default="X", choices=["X", "Y"], help="abbe: polarisation")
     ap.add_argument("--z", type=float, default=0.0, help="defocus (nm)")
     ap.add_argument("--format", default="repr", choices=list(csvio.FORMATS))
     ap.add_argument("--out", default=".", help="output directory")
     a = ap.parse_args(argv)
 
-    ses = Session(preset(a.preset))
     out = lambda name: os.path.join(a.out, name)
     t0 = time.perf_counter()
+    if a.command == "mask":
+        write_masks(out, a.ndiv, a.ndata, a.seed, a.cd, a.gap)
+        print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
+        return 0
+    ses = Session(preset(a.preset))
     if a.command == "m3d":
         lin, min_, _ = ses.cfg.pupil
         csvio.write_inputlm(out("inputlm.csv"), lin, min_, ses.xinput)

@@ -72,6 +72,12 @@ _SIGS = {
     "euvl_socs_image": [c_void_p, ctypes.POINTER(EmParams), c_void_p, c_int, c_int, P_double, P_double, P_double,
                         c_void_p, c_void_p, c_void_p, c_int, P_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
                         c_double, c_double, c_double, c_int, c_int, c_void_p],
+    "euvl_mask_linespace": [c_void_p, ctypes.c_uint, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    "euvl_mask_linespace_plan": [ctypes.c_uint, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_int64, ctypes.POINTER(c_int64)],
+    "euvl_mask_raster": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
+                         c_int, c_void_p],
+    "euvl_mask_raster_ms": [c_void_p, P_double],
 }
 
 EXPORTS = tuple(_SIGS) + ("euvl_last_error",)

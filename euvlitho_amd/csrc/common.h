@@ -96,6 +96,8 @@ struct euvl_handle_s {
   // current scope is set in prec_mask (euvl_set_precision), else on FP64 MFMA
   int prec_mask = 0;
   int prec_scope = 0;
+  // events around the last mask raster launch (euvl_mask_raster_ms)
+  hipEvent_t raster_ev[2] = {nullptr, nullptr};
 };
 
 namespace euvl {

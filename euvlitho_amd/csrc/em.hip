@@ -443,12 +443,14 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   EUVL_TRY(ew_lincomb(h, N, B, Fm, ONE, G, ONE, Gp, ZERO, nullptr, ONE));                   // I + G + G^2
   EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, Gp, N, sm, ONE, Fm, N, sm, ZERO, B));           // + G^3
   // stop once the remainder bound ||G^(2^(j+1))|| <= ||G^(2^j)||^2 is below 1e-16
+  // (1e-10 when these products run in FP32, well below their rounding)
+  const double neumann_tol = (h->prec_mask & EUVL_PREC_FP32_COUPLING) ? 1e-10 : 1e-16;
   for (;;) {
     std::vector<double> nr(B);
     EUVL_TRY(reduce_to_host(h, 1, N, B, Gp, nr.data()));
     double worst = 0.0;
     for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
-    if (worst * worst < 1e-16) break;
+    if (worst * worst < neumann_tol) break;
     if (!(worst < 0.5) || nfac >= 8) {
       set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)|| = %.3e)", nfac, worst);
       return EUVL_ERR_NOT_CONVERGED;

@@ -82,6 +82,8 @@ extern "C" int euvl_destroy(euvl_handle h) {
   if (!h) return EUVL_OK;
   (void)hipStreamSynchronize(h->stream);
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->raster_ev)
+    if (e) (void)hipEventDestroy(e);
   if (h->ws) (void)hipFree(h->ws);
   if (h->aux) (void)hipFree(h->aux);
   if (h->d_red) (void)hipFree(h->d_red);

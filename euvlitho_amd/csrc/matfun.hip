@@ -34,7 +34,8 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   // every j >= 2, ||D^j||^(1/j) <= alpha = max(||D^2||^(1/2), ||D^3||^(1/3))
   // (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009), Lemma 4.1), which
   // is often well below ||D||: s is the smallest with h^2 alpha <= 1.5, so that
-  // the degree-9 remainder, bounded by (h^2 alpha)^10 / 20!, is below 3e-17.
+  // the degree-9 remainder, bounded by (h^2 alpha)^10 / 20!, is below 3e-17
+  // (FP64 products).
   const double2* X1 = D;
   double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *T = s[5];
   EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
@@ -52,8 +53,11 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     set_error("cos_sinc: non-finite ||D^2|| or ||D^3||");
     return EUVL_ERR_NOT_CONVERGED;
   }
+  // With FP32 products (the precision form) the remainder only has to stay
+  // below FP32 rounding: h^2 alpha <= 8 bounds it by 8^10 / 20! = 4.4e-10.
+  const double theta = (h->prec_mask & h->prec_scope) ? 8.0 : 1.5;
   int sq = 0;
-  while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > 1.5 && sq < 60) ++sq;
+  while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > theta && sq < 60) ++sq;
   if (doublings) *doublings = sq;
   const double hh = std::ldexp(d, -sq);
   // q(x) = sum c_j x^j with x = h^2 D is evaluated as sum (c_j h^(2j)) D^j on the unscaled powers

@@ -276,6 +276,51 @@ int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const double* d_fmas
                     const double* d_phi, int nk, int nsocs, int nsocsxy, int nout, double kx0, double ky0,
                     double z0, int nx, int ny, double* d_img);
 
+/* ---- mask producers (cnn/intensity/mask/, elitho/mask_pattern.py) ----
+ * Masks are written packed as mask.bin: n_masks clips of ndivx*ndivy/8 bytes,
+ * MSB-first, bit index ndivx*j + i, 1 = vacuum.  ndivx must be a multiple
+ * of 8. */
+
+/* The program cnn/intensity/mask/mask.cpp:28-62 with maskgen/randmask of
+ * mVBF.h:1-70: n_masks clips drawn in sequence from one C-library rand()
+ * stream seeded with srand(seed) (mask.cpp never calls srand: seed 1), column
+ * bands of width cd (56 in the reference) that alternate between vacuum and a
+ * line/space pattern along y whose segments are multiples of gap (80).  The
+ * draws run on the host, the raster on the device (d_bits: n_masks *
+ * ndivx*ndivy/8 bytes).  Needs ndivx > cd and ndivy > gap. */
+int euvl_mask_linespace(euvl_handle h, unsigned seed, int ndivx, int ndivy, int cd, int gap, int n_masks,
+                        uint8_t* d_bits);
+
+/* Host-only (no device work): the band layout euvl_mask_linespace
+ * rasterises, in the form euvl_mask_raster takes with one layer per clip:
+ * h_sel (n_masks x max(ndivx, ndivy) ints), h_pat_off (n_masks), the pattern
+ * pool h_pool (pool_cap bytes; *h_pool_bytes receives the size needed, and
+ * EUVL_ERR_ARG is returned if pool_cap is smaller).  No reference
+ * counterpart (the reference draws and rasterises in one loop). */
+int euvl_mask_linespace_plan(unsigned seed, int ndivx, int ndivy, int cd, int gap, int n_masks, int* h_sel,
+                             int64_t* h_pat_off, uint8_t* h_pool, int64_t pool_cap, int64_t* h_pool_bytes);
+
+/* Rasterises band layouts: pixel (i, j) of clip m is the AND over its
+ * n_layers (<= EUVL_MASK_MAX_LAYERS) layers, then inverted if invert != 0.
+ * Layer L = m*n_layers + l cuts the x axis (h_axis[L] = 0: column i selects a
+ * pattern over y, of ndivy bytes) or the y axis (1: row j selects a pattern
+ * over x, of ndivx bytes): h_sel[L*max(ndivx, ndivy) + position] is a pattern
+ * index counted from byte h_pat_off[L] of the pool h_pool (pool_bytes bytes,
+ * one 0/1 byte per pixel), or -1 (constant 1) or -2 (constant 0).  Every
+ * index is checked against the pool on the host.  Drives the Python
+ * generators LinePattern and RandomLineSpacePattern
+ * (elitho/mask_pattern.py:18-196), whose draws run on the host. */
+#define EUVL_MASK_MAX_LAYERS 2
+int euvl_mask_raster(euvl_handle h, int ndivx, int ndivy, int n_masks, int n_layers, const int* h_axis,
+                     const int* h_sel, const int64_t* h_pat_off, const uint8_t* h_pool, int64_t pool_bytes,
+                     int invert, uint8_t* d_bits);
+
+/* Device time in milliseconds of the last raster kernel launched by
+ * euvl_mask_raster or euvl_mask_linespace on this handle (HIP events on the
+ * handle's stream; waits for it).  No reference counterpart
+ * (instrumentation). */
+int euvl_mask_raster_ms(euvl_handle h, double* ms);
+
 #ifdef __cplusplus
 }
 #endif

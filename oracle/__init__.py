@@ -28,6 +28,11 @@ How each part is pinned (checked against something other than itself):
 * Abbe image: ``emint.csv`` (end to end), blank-mask uniformity, grid
   refinement sub-sampling identity.
 * M3D fit: exact recovery of synthetic linear data.
+* mask generators (maskgen.py): the C++ line/space program with the C
+  library's own rand() reproduces the reference fixture ``mask.bin`` bit for
+  bit; LinePattern fed the same rand() draws reproduces it too (y-flipped);
+  RandomLineSpacePattern by closed forms under constant draws and by its band
+  width bounds.
 * linear.py / socs.py images: parity unpinned by reference fixtures (the
   reference's intft/intlinear/intsocs CSVs are not in the repository); pinned
   by invariants only (a0=ax=ay=0 => intlinear == intft; SOCS with all kernels
