@@ -178,6 +178,95 @@ def mask_producer(n_masks=64, ndiv=2048, reps=5, with_cpu=True):
     return res
 
 
+def cnn_predictor(batch=64, reps=3, with_cpu=True):
+    """The CNN M3D predictor (predict.py): six networks of the reference's
+    architecture with seeded random FP32 weights (the trained checkpoints are
+    not distributed), on `batch` line/space clips (2048 x 2048 px, compmask to
+    512 x 512).  Conv roofline: the algorithmic conv flops (2 * H * W * cin *
+    cout * 9 per block and sample) over the conv blocks' time (HIP events on
+    the stream); FC roofline: fc1 + fc2 weight bytes (read once per batch)
+    over the FC time.  CPU baseline: the reference's model class on torch CPU
+    (FP32, all host threads) for one clip."""
+    import torch
+    from euvlitho_amd.cnn import CNNmodel, CNNmodels, CHANNELS, compmask
+    from euvlitho_amd.masks import line_space_bits
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(0)
+
+    def params(n_out):
+        p = {}
+        for i in range(1, 6):
+            cin, cout = CHANNELS[i - 1], CHANNELS[i]
+            bd = 1.0 / (cin * 9) ** 0.5
+            p[f"conv{i}.weight"] = (torch.rand((cout, cin, 3, 3), generator=g, device="cuda:0") * 2 - 1) * bd
+            p[f"conv{i}.bias"] = (torch.rand(cout, generator=g, device="cuda:0") * 2 - 1) * bd
+            p[f"norm{i}.weight"] = torch.ones(cout, device="cuda:0")
+            p[f"norm{i}.bias"] = torch.zeros(cout, device="cuda:0")
+            p[f"norm{i}.running_mean"] = torch.zeros(cout, device="cuda:0")
+            p[f"norm{i}.running_var"] = torch.ones(cout, device="cuda:0")
+        for name, (i, o) in (("fc1", (65536, 4096)), ("fc2", (4096, n_out))):
+            bd = 1.0 / i ** 0.5
+            p[f"{name}.weight"] = (torch.rand((o, i), generator=g, device="cuda:0") * 2 - 1) * bd
+            p[f"{name}.bias"] = (torch.rand(o, generator=g, device="cuda:0") * 2 - 1) * bd
+        return p
+
+    nets = [CNNmodel(params(1901)), CNNmodel(params(1901))] + [CNNmodels(params(1749)) for _ in range(4)]
+    bits = line_space_bits(2048, 2048, batch, 1)
+    torch.cuda.synchronize()
+
+    def run(timing=None):
+        x = compmask(bits)
+        return [n.forward(x, timing) for n in nets]
+
+    run()
+    torch.cuda.synchronize()
+    walls, conv_ms, fc_ms = [], [], []
+    for _ in range(reps):
+        tm = {}
+        t0 = time.perf_counter()
+        run(tm)
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+        conv_ms.append(sum(e[0].elapsed_time(e[1]) for e in tm["events"]))
+        fc_ms.append(sum(e[1].elapsed_time(e[2]) for e in tm["events"]))
+    wall, tc, tf = float(np.median(walls)), float(np.median(conv_ms)), float(np.median(fc_ms))
+    conv_flops, hw = 0.0, 512
+    for i in range(5):
+        conv_flops += 2.0 * hw * hw * CHANNELS[i] * CHANNELS[i + 1] * 9
+        hw //= 2
+    conv_flops *= 6 * batch
+    fc_bytes = 4.0 * (2 * (65536 * 4096 + 4096 * 1901) + 4 * (65536 * 4096 + 4096 * 1749))
+    res = {"value": batch / wall, "unit": "clips/s", "config": f"6 networks (re0 im0 rex imx rey imy), batch "
+           f"{batch} clips of 2048x2048 px -> 512x512, FP32, seeded random weights", "ms_per_batch": 1000 * wall,
+           "conv_ms": tc, "fc_ms": tf,
+           "roofline": {"bound": "mfma", "kernel": "conv_block_kernel", "achieved": conv_flops / (tc * 1e-3) / 1e12,
+                        "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": conv_flops / (tc * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
+           "fc_roofline": {"bound": "hbm", "kernel": "fc_splitk_kernel (fc1, fc2)", "achieved": fc_bytes / (tf * 1e-3) / 1e9,
+                           "peak": 8000.0, "unit": "GB/s", "frac": fc_bytes / (tf * 1e-3) / 1e9 / 8000.0}}
+    del nets
+    torch.cuda.empty_cache()
+    if with_cpu:
+        from oracle import cnn as OC
+        rng = np.random.default_rng(0)
+        threads = len(os.sched_getaffinity(0))
+        torch.set_num_threads(threads)
+        x = torch.from_numpy(rng.random((1, 1, 512, 512), dtype=np.float32))
+        ts = []
+        for n_out in (1901, 1749):
+            m = OC.torch_model(OC.random_params(n_out, 1, nin=512, dtype=np.float32), n_out, 512, "float32")
+            with torch.no_grad():
+                t0 = time.perf_counter()
+                m(x)
+                ts.append(time.perf_counter() - t0)
+            del m
+        per_clip = 2 * ts[0] + 4 * ts[1]
+        res["cpu_baseline"] = {"value": 1.0 / per_clip, "unit": "clips/s", "cores": threads, "kind": "port",
+                               "sample": "one clip through the six networks: the reference's model class on torch "
+                                         f"CPU FP32 ({per_clip:.2f} s per clip)"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +280,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy that scales the HBM byte counters")
     ap.add_argument("--no-mask-producer", action="store_true", help="skip the mask producer leg")
+    ap.add_argument("--no-cnn", action="store_true", help="skip the CNN predictor leg")
+    ap.add_argument("--cnn-only", action="store_true", help="only run the CNN predictor leg")
     ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -203,6 +294,9 @@ def main():
         return
     if args.mask_producer_only:
         print(json.dumps({"mask_producer": mask_producer(with_cpu=not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.cnn_only:
+        print(json.dumps({"cnn_predictor": cnn_predictor(with_cpu=not args.no_cpu_baseline)}), flush=True)
         return
     import ctypes
     import torch
@@ -325,6 +419,8 @@ def main():
         line["forms"]["served"] = run_served(args.served, args.slots)
     if not args.no_mask_producer:
         line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)
+    if not args.no_cnn:
+        line["cnn_predictor"] = cnn_predictor(with_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cpu_baseline(cfg, masks, threads)

@@ -78,6 +78,10 @@ _SIGS = {
     "euvl_mask_raster": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int64,
                          c_int, c_void_p],
     "euvl_mask_raster_ms": [c_void_p, P_double],
+    "euvl_cnn_compmask": [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p],
+    "euvl_cnn_conv_block": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_void_p, c_void_p],
+    "euvl_cnn_linear": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
 }
 
 EXPORTS = tuple(_SIGS) + ("euvl_last_error",)

@@ -321,6 +321,33 @@ int euvl_mask_raster(euvl_handle h, int ndivx, int ndivy, int n_masks, int n_lay
  * (instrumentation). */
 int euvl_mask_raster_ms(euvl_handle h, double* ms);
 
+/* ---- CNN M3D predictor (cnn/cnn/predict/predict.py) ----
+ * FP32 tensors, NCHW, caller-owned device memory. */
+
+/* compmask (cnn/cnn/predict/compmask.py:17-41): n_clips mask.bin clips of
+ * ndiv x ndiv bits -> d_out[c][r][x] (nout x nout floats), the mean of the
+ * (ndiv/nout)^2 block of bits at block row r (mask.bin rows, top first) and
+ * block column x.  ndiv must be a multiple of nout. */
+int euvl_cnn_compmask(euvl_handle h, const uint8_t* d_bits, int ndiv, int n_clips, int nout, float* d_out);
+
+/* One conv block of CNNmodel.forward (predict.py:34-45): x = norm(pool(
+ * relu(conv(x)))) with conv = Conv2d(cin, cout, 3, padding=1,
+ * padding_mode='circular') (d_w: [cout][cin][3][3], d_bias: [cout]), pool =
+ * MaxPool2d(2, 2) and norm = BatchNorm2d in eval mode folded into a
+ * per-channel d_scale = weight / sqrt(running_var + eps) and d_shift = bias -
+ * running_mean * d_scale.  d_x: [batch][cin][height][width] -> d_y:
+ * [batch][cout][height/2][width/2]; height and width even. */
+int euvl_cnn_conv_block(euvl_handle h, int batch, int cin, int cout, int height, int width, const float* d_x,
+                        const float* d_w, const float* d_bias, const float* d_scale, const float* d_shift,
+                        float* d_y);
+
+/* nn.Linear (predict.py:32-33,49-50): d_y[batch][out] = act(d_x[batch][in]
+ * d_w[out][in]^T + d_bias) / d_div, act = ReLU if relu != 0, d_div (per
+ * output feature) optional.  A split-K FP32 MFMA GEMM whose K slices are
+ * summed in a fixed order (partials in the handle's workspace). */
+int euvl_cnn_linear(euvl_handle h, int batch, int in_features, int out_features, const float* d_x,
+                    const float* d_w, const float* d_bias, int relu, const float* d_div, float* d_y);
+
 #ifdef __cplusplus
 }
 #endif
