@@ -6,6 +6,7 @@ reading and writing the same files in the working directory:
   python -m euvlitho_amd linear  mask.bin + inputxx.csv -> intlinear.csv, intft.csv  (linear.py __main__)
   python -m euvlitho_amd socs    mask.bin + inputxx.csv -> intsocs.csv  (socs.py __main__)
   python -m euvlitho_amd mask    -> mask.bin, maskimage.csv             (mask.cpp main)
+  python -m euvlitho_amd predict mask.bin + model/*/model.ckpt -> inputxx.csv  (predict.py __main__)
 
 --preset picks the problem (reference: the drivers' own optics; c1, c2: the
 high-NA clips); --format g writes the reference C++ 6-digit numbers.
@@ -35,9 +36,26 @@ def write_masks(out, ndiv, ndata, seed, cd, gap):
         f.write("\n".join("".join(row) for row in lines) + "\n")
 
 
+def predict(out, mask, model_dir, ndiv, nout):
+    """predict.py:122-151: the six networks from model_dir/<name>/model.ckpt,
+    the factors from re0/factor0.csv, rex/factorx.csv, rey/factory.csv, the
+    clips of mask.bin through compmask -> inputxx.csv (np.savetxt blocks)."""
+    import numpy as np
+    from .cnn import CNNmodel, CNNmodels, compmask, predict_m3d, write_inputxx
+    d = lambda *p: os.path.join(model_dir, *p)
+    models = [CNNmodel.load_from_checkpoint(d("re0", "model.ckpt")),
+              CNNmodel.load_from_checkpoint(d("im0", "model.ckpt"))]
+    models += [CNNmodels.load_from_checkpoint(d(n, "model.ckpt")) for n in ("rex", "imx", "rey", "imy")]
+    factor0 = np.loadtxt(d("re0", "factor0.csv"), delimiter=",")
+    factorx = np.loadtxt(d("rex", "factorx.csv"), delimiter=",")
+    factory = np.loadtxt(d("rey", "factory.csv"), delimiter=",")
+    x = compmask(mask, ndiv, nout)
+    write_inputxx(out("inputxx.csv"), predict_m3d(x, *models, factor0, factorx, factory))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m euvlitho_amd", description=__doc__.split("\n\n")[0])
-    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs", "mask"])
+    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs", "mask", "predict"])
     ap.add_argument("--preset", default="reference")
     ap.add_argument("--mask", default="mask.bin")
     ap.add_argument("--inputxx", default="inputxx.csv")
@@ -46,6 +64,9 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1, help="mask: srand seed (the reference program's: 1)")
     ap.add_argument("--cd", type=int, default=56, help="mask: column band width")
     ap.add_argument("--gap", type=int, default=80, help="mask: line/space segment unit")
+    ap.add_argument("--model-dir", default="../model",
+                    help="predict: directory of re0/ im0/ rex/ imx/ rey/ imy/ with model.ckpt and the factor csv files")
+    ap.add_argument("--nout", type=int, default=512, help="predict: network input size (compmask output)")
     ap.add_argument("--polar", default="X", choices=["X", "Y"], help="abbe: polarisation")
     ap.add_argument("--z", type=float, default=0.0, help="defocus (nm)")
     ap.add_argument("--format", default="repr", choices=list(csvio.FORMATS))
@@ -56,6 +77,10 @@ def main(argv=None):
     t0 = time.perf_counter()
     if a.command == "mask":
         write_masks(out, a.ndiv, a.ndata, a.seed, a.cd, a.gap)
+        print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
+        return 0
+    if a.command == "predict":
+        predict(out, a.mask, a.model_dir, a.ndiv, a.nout)
         print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
         return 0
     ses = Session(preset(a.preset))

@@ -55,14 +55,13 @@ __global__ void compmask_kernel(int ndiv, int nout, int n_clips, const uint8_t* 
 // pooled pixel q: the accumulator element r of lane l is row 4(l/16) + r
 // (gfx950 f32 16x16x4 layout), so each lane holds one whole pooling window
 // and the max pool is a max over its four accumulator elements.
-constexpr int CK = 4;
-constexpr int KC = CK * 9;  // K per chunk (9 MFMA steps of 4)
-
-template <int NTU>
+template <int NTU, int CK>
 __global__ __launch_bounds__(256) void conv_block_kernel(int cin, int cout, int H, int W, const float* __restrict__ X,
                                                         const float* __restrict__ Wt, const float* __restrict__ bias,
                                                         const float* __restrict__ scale,
                                                         const float* __restrict__ shift, float* __restrict__ Y) {
+  constexpr int KC = CK * 9;          // K per chunk
+  constexpr int STEPS = (KC + 3) / 4;  // MFMA steps of 4 (the last padded with zeros when KC % 4 != 0)
   __shared__ float xs[CK][18][19];
   __shared__ float ws[16 * NTU][KC + 1];
   const int tiles_x = (W + 15) / 16;
@@ -99,14 +98,16 @@ __global__ __launch_bounds__(256) void conv_block_kernel(int cin, int cout, int 
     }
     __syncthreads();
 #pragma unroll
-    for (int s = 0; s < KC / 4; ++s) {
+    for (int s = 0; s < STEPS; ++s) {
       const int k = 4 * s + lg;
-      const int ci = k / 9, tap = k % 9, dy = tap / 3, dx = tap % 3;
+      const bool kin = (KC % 4 == 0) || k < KC;
+      const int kc = kin ? k : 0;
+      const int ci = kc / 9, tap = kc % 9, dy = tap / 3, dx = tap % 3;
       float a[4], w[NTU];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) a[t] = xs[ci][py[t] + dy][px[t] + dx];
+      for (int t = 0; t < 4; ++t) a[t] = kin ? xs[ci][py[t] + dy][px[t] + dx] : 0.f;
 #pragma unroll
-      for (int u = 0; u < NTU; ++u) w[u] = ws[16 * u + li][k];
+      for (int u = 0; u < NTU; ++u) w[u] = kin ? ws[16 * u + li][kc] : 0.f;
 #pragma unroll
       for (int t = 0; t < 4; ++t)
 #pragma unroll
@@ -131,31 +132,44 @@ __global__ __launch_bounds__(256) void conv_block_kernel(int cin, int cout, int 
 
 // part[s][b][n] = sum over k in slice s of x[b][k] W[n][k]: a 64 (batch) x
 // 64 (features) tile per workgroup of 4 waves (2 x 2, each 32 x 32 in 2 x 2
-// MFMA tiles); K staged through LDS in slabs of 16, k-major, with a register
-// prefetch of the next slab.
-__global__ __launch_bounds__(256) void fc_splitk_kernel(int B, int N, int K, int kslice,
+// MFMA tiles); K staged through LDS in slabs of 32, k-major, with a register
+// prefetch of the next slab (16-byte loads where the slab is whole and the
+// rows are 16-byte aligned).
+__global__ __launch_bounds__(256) void fc_splitk_kernel(int B, int N, int K, int kslice, int vec,
                                                        const float* __restrict__ x, const float* __restrict__ W,
                                                        float* __restrict__ part) {
-  __shared__ float xs[16][68];
-  __shared__ float wsm[16][68];
+  constexpr int BK = 32;
+  __shared__ float xs[BK][68];
+  __shared__ float wsm[BK][68];
   const int n0 = blockIdx.x * 64, s = blockIdx.y, b0 = blockIdx.z * 64;
   const int k_begin = s * kslice, k_end = min(K, k_begin + kslice);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lg = lane >> 4, li = lane & 15;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  // slab loads: thread -> row r (0..63), 4 consecutive k at c
-  const int r = tid >> 2, c = (tid & 3) * 4;
-  float4 rx, rw;
+  // slab loads: thread -> row r (0..63), 8 consecutive k at c
+  const int r = tid >> 2, c = (tid & 3) * 8;
+  float rx[8], rw[8];
   auto load = [&](int k0) {
-    float vx[4], vw[4];
+    const bool whole = vec && k0 + BK <= k_end;
+    const bool xr = b0 + r < B, wr_ = n0 + r < N;
+    if (whole) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int k = k0 + c + q;
-      vx[q] = (b0 + r < B && k < k_end) ? x[(int64_t)(b0 + r) * K + k] : 0.f;
-      vw[q] = (n0 + r < N && k < k_end) ? W[(int64_t)(n0 + r) * K + k] : 0.f;
+      for (int h2 = 0; h2 < 2; ++h2) {
+        const float4 vx = xr ? *reinterpret_cast<const float4*>(x + (int64_t)(b0 + r) * K + k0 + c + 4 * h2)
+                             : make_float4(0.f, 0.f, 0.f, 0.f);
+        const float4 vw = wr_ ? *reinterpret_cast<const float4*>(W + (int64_t)(n0 + r) * K + k0 + c + 4 * h2)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        rx[4 * h2] = vx.x; rx[4 * h2 + 1] = vx.y; rx[4 * h2 + 2] = vx.z; rx[4 * h2 + 3] = vx.w;
+        rw[4 * h2] = vw.x; rw[4 * h2 + 1] = vw.y; rw[4 * h2 + 2] = vw.z; rw[4 * h2 + 3] = vw.w;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int k = k0 + c + q;
+        rx[q] = (xr && k < k_end) ? x[(int64_t)(b0 + r) * K + k] : 0.f;
+        rw[q] = (wr_ && k < k_end) ? W[(int64_t)(n0 + r) * K + k] : 0.f;
+      }
     }
-    rx = make_float4(vx[0], vx[1], vx[2], vx[3]);
-    rw = make_float4(vw[0], vw[1], vw[2], vw[3]);
   };
   float4_t acc[2][2];
 #pragma unroll
@@ -163,14 +177,17 @@ __global__ __launch_bounds__(256) void fc_splitk_kernel(int B, int N, int K, int
 #pragma unroll
     for (int u = 0; u < 2; ++u) acc[t][u] = (float4_t){0.f, 0.f, 0.f, 0.f};
   load(k_begin);
-  for (int k0 = k_begin; k0 < k_end; k0 += 16) {
+  for (int k0 = k_begin; k0 < k_end; k0 += BK) {
     __syncthreads();
-    xs[c + 0][r] = rx.x; xs[c + 1][r] = rx.y; xs[c + 2][r] = rx.z; xs[c + 3][r] = rx.w;
-    wsm[c + 0][r] = rw.x; wsm[c + 1][r] = rw.y; wsm[c + 2][r] = rw.z; wsm[c + 3][r] = rw.w;
-    __syncthreads();
-    if (k0 + 16 < k_end) load(k0 + 16);
 #pragma unroll
-    for (int kk = 0; kk < 16; kk += 4) {
+    for (int q = 0; q < 8; ++q) {
+      xs[c + q][r] = rx[q];
+      wsm[c + q][r] = rw[q];
+    }
+    __syncthreads();
+    if (k0 + BK < k_end) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
       float a[2], bw[2];
 #pragma unroll
       for (int t = 0; t < 2; ++t) a[t] = xs[kk + lg][wr + 16 * t + li];
@@ -234,13 +251,19 @@ extern "C" int euvl_cnn_conv_block(euvl_handle h, int batch, int cin, int cout, 
     return EUVL_ERR_ARG;
   }
   const int tiles = ((height + 15) / 16) * ((width + 15) / 16);
-  if (cout <= 16) {
-    hipLaunchKernelGGL(conv_block_kernel<1>, dim3(tiles, (cout + 15) / 16, batch), dim3(256), 0, h->stream, cin,
+  // one input channel (the first block): chunks of 1 channel; otherwise 8
+  if (cout <= 16 && cin == 1)
+    hipLaunchKernelGGL((conv_block_kernel<1, 1>), dim3(tiles, (cout + 15) / 16, batch), dim3(256), 0, h->stream, cin,
                        cout, height, width, d_x, d_w, d_bias, d_scale, d_shift, d_y);
-  } else {
-    hipLaunchKernelGGL(conv_block_kernel<2>, dim3(tiles, (cout + 31) / 32, batch), dim3(256), 0, h->stream, cin,
+  else if (cout <= 16)
+    hipLaunchKernelGGL((conv_block_kernel<1, 8>), dim3(tiles, (cout + 15) / 16, batch), dim3(256), 0, h->stream, cin,
                        cout, height, width, d_x, d_w, d_bias, d_scale, d_shift, d_y);
-  }
+  else if (cin == 1)
+    hipLaunchKernelGGL((conv_block_kernel<2, 1>), dim3(tiles, (cout + 31) / 32, batch), dim3(256), 0, h->stream, cin,
+                       cout, height, width, d_x, d_w, d_bias, d_scale, d_shift, d_y);
+  else
+    hipLaunchKernelGGL((conv_block_kernel<2, 8>), dim3(tiles, (cout + 31) / 32, batch), dim3(256), 0, h->stream, cin,
+                       cout, height, width, d_x, d_w, d_bias, d_scale, d_shift, d_y);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }
@@ -253,16 +276,18 @@ extern "C" int euvl_cnn_linear(euvl_handle h, int batch, int in_features, int ou
     return EUVL_ERR_ARG;
   }
   // K slices: enough workgroups to fill the device (~2048), slices of at
-  // least 256 and a multiple of 16
+  // least 256 and a multiple of 32 (the slab)
   const int tiles = ((out_features + 63) / 64) * ((batch + 63) / 64);
   int S = std::max(1, std::min((2048 + tiles - 1) / tiles, (in_features + 255) / 256));
-  int kslice = ((in_features + S - 1) / S + 15) / 16 * 16;
+  int kslice = ((in_features + S - 1) / S + 31) / 32 * 32;
   S = (in_features + kslice - 1) / kslice;
   const size_t part_bytes = sizeof(float) * (size_t)S * batch * out_features;
   EUVL_TRY(euvl::ws_reserve(h, part_bytes + 256));
   float* part = (float*)euvl::ws_take(h, part_bytes);
+  // 16-byte loads when every row of x and W starts on a 16-byte boundary
+  const int vec = (in_features % 4 == 0) && ((uintptr_t)d_x % 16 == 0) && ((uintptr_t)d_w % 16 == 0);
   hipLaunchKernelGGL(fc_splitk_kernel, dim3((out_features + 63) / 64, S, (batch + 63) / 64), dim3(256), 0, h->stream,
-                     batch, out_features, in_features, kslice, d_x, d_w, part);
+                     batch, out_features, in_features, kslice, vec, d_x, d_w, part);
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(fc_reduce_kernel, grid1((int64_t)batch * out_features), dim3(256), 0, h->stream, batch,
                      out_features, S, part, d_bias, relu, d_div, d_y);

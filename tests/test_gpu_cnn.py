@@ -117,3 +117,34 @@ def test_predict_m3d_and_checkpoints(tmp_path):
     for g, w in zip(got, want):
         assert g.dtype == np.float64 and g.shape == w.shape
         assert np.abs(g - w).max() / np.abs(w).max() < 1e-4
+
+
+def test_cli_predict_writes_inputxx(tmp_path):
+    """python -m euvlitho_amd predict: checkpoints and factor files laid out
+    as predict.py reads them (here at a 64 x 64 network input, so that the
+    six checkpoints stay small) -> inputxx.csv rows (np.savetxt) == the
+    oracle's predict_m3d on the reference mask."""
+    import os
+    from euvlitho_amd.__main__ import main
+    names = ("re0", "im0", "rex", "imx", "rey", "imy")
+    ps = [OC.random_params(1901 if k < 2 else 1749, 60 + k, nin=64) for k in range(6)]
+    rng = np.random.default_rng(9)
+    facs = {"re0/factor0.csv": rng.uniform(0.5, 2, 1901), "rex/factorx.csv": rng.uniform(0.5, 2, 1749),
+            "rey/factory.csv": rng.uniform(0.5, 2, 1749)}
+    for name, p in zip(names, ps):
+        (tmp_path / "model" / name).mkdir(parents=True)
+        torch.save({"state_dict": {n: torch.from_numpy(v.astype(np.float32)) for n, v in p.items()}},
+                   tmp_path / "model" / name / "model.ckpt")
+    for f, v in facs.items():
+        np.savetxt(tmp_path / "model" / f, v[None], delimiter=",")
+    mask = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_mask.bin")
+    assert main(["predict", "--mask", mask, "--model-dir", str(tmp_path / "model"), "--nout", "64",
+                 "--out", str(tmp_path)]) == 0
+    rows = [np.array(l.split(","), dtype=np.float64) for l in open(tmp_path / "inputxx.csv")]
+    assert len(rows) == 6
+    x = OC.compmask(open(mask, "rb").read(), 2048, 64)[None, None]
+    f = [facs["re0/factor0.csv"], facs["rex/factorx.csv"], facs["rey/factory.csv"]]
+    p32 = [{n: v.astype(np.float32).astype(np.float64) for n, v in p.items()} for p in ps]
+    want = OC.predict_m3d(x, p32, *f)
+    for r, w in zip(rows, want):
+        assert np.abs(r - w[0]).max() / np.abs(w).max() < 1e-4
