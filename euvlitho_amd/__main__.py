@@ -7,6 +7,8 @@ reading and writing the same files in the working directory:
   python -m euvlitho_amd socs    mask.bin + inputxx.csv -> intsocs.csv  (socs.py __main__)
   python -m euvlitho_amd mask    -> mask.bin, maskimage.csv             (mask.cpp main)
   python -m euvlitho_amd predict mask.bin + model/*/model.ckpt -> inputxx.csv  (predict.py __main__)
+  python -m euvlitho_amd compress mask.bin + inputxx.csv + inputlm.csv -> maskdata/ ampdata/ inputlm/ (compress.py)
+  python -m euvlitho_amd dataset  -> the same npy files, generated on the GPU (mask.cpp -> m3d.cpp -> compress.py)
 
 --preset picks the problem (reference: the drivers' own optics; c1, c2: the
 high-NA clips); --format g writes the reference C++ 6-digit numbers.
@@ -55,10 +57,11 @@ def predict(out, mask, model_dir, ndiv, nout):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m euvlitho_amd", description=__doc__.split("\n\n")[0])
-    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs", "mask", "predict"])
+    ap.add_argument("command", choices=["m3d", "abbe", "linear", "socs", "mask", "predict", "compress", "dataset"])
     ap.add_argument("--preset", default="reference")
     ap.add_argument("--mask", default="mask.bin")
     ap.add_argument("--inputxx", default="inputxx.csv")
+    ap.add_argument("--inputlm", default="inputlm.csv", help="compress: the pupil point list")
     ap.add_argument("--ndata", type=int, default=1, help="m3d, mask: clips in mask.bin")
     ap.add_argument("--ndiv", type=int, default=2048, help="mask: clip size in pixels (NDIVX = NDIVY)")
     ap.add_argument("--seed", type=int, default=1, help="mask: srand seed (the reference program's: 1)")
@@ -81,6 +84,17 @@ def main(argv=None):
         return 0
     if a.command == "predict":
         predict(out, a.mask, a.model_dir, a.ndiv, a.nout)
+        print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
+        return 0
+    if a.command == "compress":
+        from .dataset import compress
+        compress(a.mask, a.inputxx, a.inputlm, a.ndata, a.out, a.ndiv, a.nout)
+        print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
+        return 0
+    if a.command == "dataset":
+        from .dataset import generate
+        generate(a.ndata, a.out, preset(a.preset), a.seed, a.cd, a.gap, a.nout, write_intermediates=True,
+                 progress=lambda i, n: print(f"clip {i}/{n}", file=sys.stderr, flush=True))
         print(f"elapsed time (s) {time.perf_counter() - t0:.3f}", file=sys.stderr)
         return 0
     ses = Session(preset(a.preset))

@@ -82,6 +82,8 @@ _SIGS = {
     "euvl_cnn_conv_block": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p],
     "euvl_cnn_linear": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
+    "euvl_augment_batch": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
 }
 
 EXPORTS = tuple(_SIGS) + ("euvl_last_error",)

@@ -348,6 +348,22 @@ int euvl_cnn_conv_block(euvl_handle h, int batch, int cin, int cout, int height,
 int euvl_cnn_linear(euvl_handle h, int batch, int in_features, int out_features, const float* d_x,
                     const float* d_w, const float* d_bias, int relu, const float* d_div, float* d_y);
 
+/* ---- training data (cnn/intensity/m3d/compress.py, the model scripts of cnn/cnn/model) ---- */
+
+/* The CNN data loader's augmentation (MaskampDataset.__getitem__,
+ * cnn/cnn/model/re0/re0.py:24-62) for n samples at once: h_shifts holds
+ * (lshift, mshift) per sample.  Masks (float16 bit patterns, [n][nout][nout],
+ * optional: both d_masks and d_out_masks or neither) are rolled as
+ * np.roll(x, (mshift, -lshift), axis=(0, 1)); the amplitudes d_re, d_im
+ * ([n][ncut] float32) become d_out_re = re cos - im sin and d_out_im = re sin
+ * + im cos at index (l lshift + m mshift) mod nout of the float32 tables
+ * h_cos, h_sin (nout entries), with h_lorder, h_morder the pupil orders (the
+ * rshift / ishift of re0.py:24-36, float32, unfused). */
+int euvl_augment_batch(euvl_handle h, int n, int nout, const uint16_t* d_masks, const int* h_shifts, int ncut,
+                       const int* h_lorder, const int* h_morder, const float* d_re, const float* d_im,
+                       const float* h_cos, const float* h_sin, uint16_t* d_out_masks, float* d_out_re,
+                       float* d_out_im);
+
 #ifdef __cplusplus
 }
 #endif
