@@ -36,6 +36,14 @@ class EmParams(ctypes.Structure):
     ]
 
 
+class StccParams(ctypes.Structure):
+    _fields_ = [
+        ("k", c_double), ("kx0", c_double), ("ky0", c_double), ("z0", c_double), ("dx", c_double),
+        ("dy", c_double), ("na", c_double), ("mx", c_int), ("my", c_int), ("lpmax_x", c_int), ("lpmax_y", c_int),
+        ("xdiv", c_int),
+    ]
+
+
 # name -> (argtypes); every entry point returns int
 _SIGS = {
     "euvl_create": [ctypes.POINTER(c_void_p)],
@@ -82,6 +90,8 @@ _SIGS = {
     "euvl_cnn_conv_block": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                             c_void_p, c_void_p],
     "euvl_cnn_linear": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
+    "euvl_stcc_image": [c_void_p, c_char, ctypes.POINTER(StccParams), c_void_p, P_double, P_double, P_double, c_void_p,
+                        c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
     "euvl_augment_batch": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
 }

@@ -1,0 +1,167 @@
+"""The alternative STCC image of the Python elitho package: TCCEE
+(cnn/intensity/stccsocs/elitho/stcc.py:12-431) on the GPU.
+
+``intensity(mask, polar, a0xx, axxx, ayxx)`` keeps the reference's call: the
+mask is the package's array M[x][y'] (0/1, as LinePattern returns it), the
+optional M3D arrays are (noutX, noutY) complex.  The pupil points
+(pupil.py:5-66) and the uniform k-source (source.py:157-207) are built on the
+host from the constants; the mask spectrum, the field terms, the three TCCs
+(batched ZGEMMs over the source points), the TCCEE gather and the image
+transform run in libeuvl (``euvl_stcc_image``).
+
+Constants follow elitho/const.py (the non-high-NA branch: NA 0.33, 4x, 2048 nm,
+mesh 0.5 deg, annular 0.55-0.9).  The zero-order amplitudes (amp_absorber,
+amp_vacuum, phasexx of diffraction_amplitude.zero_order_amplitude) come from
+this library's blank-mask solve of the same optics, or are passed in.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+@dataclass(frozen=True)
+class StccConfig:
+    ndiv: int = 2048
+    mx: int = 4
+    my: int = 4
+    na: float = 0.33
+    wavelength: float = 13.5
+    theta0: float = -6.0
+    azimuth: float = 0.0
+    mesh: float = 0.5
+    sigma1: float = 0.9
+    sigma2: float = 0.55
+    illumination: str = "annular"   # or "circular"
+    z0: float = 102.0               # dabst + 42
+
+    @property
+    def k(self):
+        return 2.0 * np.pi / self.wavelength
+
+    @property
+    def kx0(self):
+        return self.k * np.sin(np.deg2rad(self.theta0)) * np.cos(np.deg2rad(90.0 - self.azimuth))
+
+    @property
+    def ky0(self):
+        return self.k * np.sin(np.deg2rad(self.theta0)) * np.sin(np.deg2rad(90.0 - self.azimuth))
+
+    @property
+    def xdiv(self):
+        return self.ndiv // self.mx
+
+    @property
+    def lsmax(self):
+        return (int(self.na * self.ndiv / self.mx / self.wavelength + 1),
+                int(self.na * self.ndiv / self.my / self.wavelength + 1))
+
+    @property
+    def lpmax(self):
+        return (int(self.na * self.ndiv / self.mx * 2 / self.wavelength + 0.0001),
+                int(self.na * self.ndiv / self.my * 2 / self.wavelength + 0.0001))
+
+    def pupil(self):
+        """pupil.py:24-44 (offsets 0): (l, m) of the points with at least one
+        lattice source inside the NA whose shifted point is inside too,
+        ip-major."""
+        lsx, lsy = self.lsmax
+        lpx, lpy = self.lpmax
+        na2 = (self.na / self.wavelength) ** 2
+        sx, sy = np.meshgrid(np.arange(-lsx, lsx + 1), np.arange(-lsy, lsy + 1), indexing="ij")
+        src_ok = (sx * self.mx / self.ndiv) ** 2 + (sy * self.my / self.ndiv) ** 2 <= na2
+        out = []
+        for ip in range(2 * lpx + 1):
+            for jp in range(2 * lpy + 1):
+                ok = ((ip - lpx + sx) * self.mx / self.ndiv) ** 2 + ((jp - lpy + sy) * self.my / self.ndiv) ** 2 <= na2
+                if np.any(src_ok & ok):
+                    out.append((ip - lpx, jp - lpy))
+        return np.array(out, dtype=np.int32)
+
+    def source(self):
+        """source.py:157-199: (sx, sy) of the 0.5-degree mesh inside the source."""
+        k = self.k
+        kmesh = k * self.mesh * (np.pi / 180.0)
+        l0max = int(k * self.na / self.mx * self.sigma1 / kmesh + 1)
+        m0max = int(k * self.na / self.my * self.sigma1 / kmesh + 1)
+        L, M = np.meshgrid(np.arange(-l0max, l0max + 1), np.arange(-m0max, m0max + 1), indexing="ij")
+        skx, sky = L * kmesh, M * kmesh
+        skxo, skyo = skx * self.mx, sky * self.my
+        if self.illumination == "circular":
+            sel = skxo ** 2 + skyo ** 2 <= (k * self.na * self.sigma1) ** 2
+        elif self.illumination == "annular":
+            r = np.sqrt(skxo ** 2 + skyo ** 2)
+            sel = (k * self.na * self.sigma2 <= r) & (r <= k * self.na * self.sigma1)
+        else:
+            raise ValueError("Invalid illumination type")
+        return np.stack([skx[sel], sky[sel]], axis=1).astype(np.float64)
+
+
+def zero_order_amplitude(cfg: StccConfig = StccConfig(), device="cuda:0"):
+    """(amp_absorber, amp_vacuum, phasexx) of the blank-mask solve at the
+    central incidence (diffraction_amplitude.py:156-169), from this library's
+    EM solve of the matching reference optics."""
+    from .config import reference_default
+    from .engine import Engine
+    if cfg != StccConfig():
+        raise ValueError("zero-order amplitudes are solved for the reference optics only; pass them explicitly")
+    eng = Engine(reference_default(), device)
+    vc, ab, ph = eng.reflect_amplitude()
+    eng.close()
+    return ab, vc, ph
+
+
+def _bits_of(mask, ndiv):
+    """The package's mask array M[x][y'] -> mask.bin bytes (pixel (x, ndiv-1-y'))."""
+    m = np.asarray(mask)
+    if m.shape != (ndiv, ndiv):
+        raise ValueError(f"mask must be {ndiv} x {ndiv}")
+    return np.packbits(np.flipud(m.T).astype(np.uint8).reshape(-1))
+
+
+def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig = StccConfig(), amplitudes=None,
+              device="cuda:0", handle=None):
+    """stcc.py:394-431: the TCCEE image [x][y] (xdiv x xdiv float64 host
+    array).  polar 'X' or 'Y' (elitho PolarizationDirection); the M3D arrays
+    are all given or all None ('Y' needs them, as the reference's branch
+    reads them unconditionally).  amplitudes = (amp_absorber, amp_vacuum,
+    phasexx) overrides the blank-mask solve."""
+    polar = getattr(polar, "name", polar)
+    if polar not in ("X", "Y"):
+        raise ValueError("polar must be 'X' or 'Y'")
+    ab, vc, ph = amplitudes if amplitudes is not None else zero_order_amplitude(cfg, device)
+    lpx, lpy = cfg.lpmax
+    dev = torch.device(device)
+    bits = torch.from_numpy(_bits_of(mask, cfg.ndiv)).to(dev)
+    fm = torch.empty((2 * lpx + 1, 2 * lpy + 1), dtype=torch.complex128, device=dev)
+    own = handle is None
+    lib = _lib.load()
+    h = ctypes.c_void_p() if own else handle
+    if own:
+        _lib.check(lib.euvl_create(ctypes.byref(h)), "euvl_create")
+    try:
+        _lib.check(lib.euvl_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)),
+                   "euvl_set_stream")
+        _lib.check(lib.euvl_mask_spectrum(h, _lib.ptr(bits), cfg.ndiv, cfg.ndiv, 1, lpx, lpy, _lib.ptr(fm)),
+                   "euvl_mask_spectrum")
+        m3d = [None if a is None else torch.as_tensor(np.asarray(a, dtype=np.complex128)).to(dev).contiguous()
+               for a in (a0xx, axxx, ayxx)]
+        if any(a is not None and tuple(a.shape) != tuple(fm.shape) for a in m3d):
+            raise ValueError(f"M3D arrays must be {tuple(fm.shape)}")
+        pupil = np.ascontiguousarray(cfg.pupil())
+        src = np.ascontiguousarray(cfg.source())
+        prm = _lib.StccParams(cfg.k, cfg.kx0, cfg.ky0, cfg.z0, float(cfg.ndiv), float(cfg.ndiv), cfg.na, cfg.mx,
+                              cfg.my, lpx, lpy, cfg.xdiv)
+        img = torch.empty((cfg.xdiv, cfg.xdiv), dtype=torch.float64, device=dev)
+        cplx = lambda z: (ctypes.c_double * 2)(complex(z).real, complex(z).imag)
+        p = lambda t: _lib.ptr(t) if t is not None else None
+        _lib.check(lib.euvl_stcc_image(h, polar.encode(), ctypes.byref(prm), _lib.ptr(fm), cplx(ab), cplx(vc),
+                                       cplx(ph), p(m3d[0]), p(m3d[1]), p(m3d[2]), len(pupil), pupil.ctypes.data,
+                                       len(src), src.ctypes.data, _lib.ptr(img)), "euvl_stcc_image")
+        return img.cpu().numpy()
+    finally:
+        if own:
+            lib.euvl_destroy(h)

@@ -348,6 +348,31 @@ int euvl_cnn_conv_block(euvl_handle h, int batch, int cin, int cout, int height,
 int euvl_cnn_linear(euvl_handle h, int batch, int in_features, int out_features, const float* d_x,
                     const float* d_w, const float* d_bias, int relu, const float* d_div, float* d_y);
 
+/* ---- the TCCEE image (cnn/intensity/stccsocs/elitho/stcc.py) ---- */
+typedef struct {
+  double k, kx0, ky0, z0, dx, dy, na; /* elitho const: k, kx0, ky0, z0, dx, dy, NA */
+  int mx, my;                         /* MX, MY */
+  int lpmax_x, lpmax_y;               /* pupil window half widths (lpmaxX, lpmaxY) */
+  int xdiv;                           /* image grid XDIV (both axes, as stcc.py:300) */
+} euvl_stcc_params;
+
+/* intensity() of elitho/stcc.py:394-431: the image ifft2(TCCEE,
+ * norm="forward").real on xdiv x xdiv (d_img, index x*xdiv+y), from the
+ * forward-normalised FFT of the 0/1 mask pattern on the pupil window
+ * (d_fmask, (2 lpmax_x+1) x (2 lpmax_y+1) complex, index [l+lpmax_x][m+lpmax_y]),
+ * the zero-order amplitudes (ab = amp_absorber, vc = amp_vacuum, phase0 =
+ * phasexx; host (re, im)), optional M3D windows d_a0, d_ax, d_ay (same
+ * layout as d_fmask; all three or none; required for polar 'Y', whose branch
+ * always reads them, stcc.py:239), the pupil points h_pupil (ncut (l, m)
+ * pairs, pupil.py:24-44) and the uniform k-source h_src (n_src (sx, sy)
+ * pairs, source.py:157-199).  The TCCs of the polarisation are three batched
+ * ZGEMMs over the source points; TCCEE is gathered over the difference
+ * window in a fixed order. */
+int euvl_stcc_image(euvl_handle h, char polar, const euvl_stcc_params* p, const double* d_fmask, const double* ab,
+                    const double* vc, const double* phase0, const double* d_a0, const double* d_ax,
+                    const double* d_ay, int ncut, const int* h_pupil, int n_src, const double* h_src,
+                    double* d_img);
+
 /* ---- training data (cnn/intensity/m3d/compress.py, the model scripts of cnn/cnn/model) ---- */
 
 /* The CNN data loader's augmentation (MaskampDataset.__getitem__,
