@@ -267,6 +267,52 @@ def cnn_predictor(batch=64, reps=3, with_cpu=True):
     return res
 
 
+def stcc_leg(reps=5, with_cpu=True):
+    """The TCCEE image of elitho/stcc.py at the reference constants (2048 nm,
+    NA 0.33, 1901 pupil points, the 0.5-degree annular source), X
+    polarisation with M3D arrays: images/s over the whole call (mask spectrum,
+    field terms, three TCCs, TCCEE, transform).  Roofline: the TCC ZGEMMs
+    (8 ncut^2 n_src flops each, three per image), the call's dominant
+    arithmetic, over the call's time.  CPU baseline: the oracle (NumPy,
+    vectorised) on the same inputs."""
+    import torch
+    from euvlitho_amd.stcc import StccConfig, intensity
+    cfg = StccConfig()
+    rng = np.random.default_rng(0)
+    mask = (rng.random((cfg.ndiv, cfg.ndiv)) > 0.5).astype(int)
+    lpx, lpy = cfg.lpmax
+    shape = (2 * lpx + 1, 2 * lpy + 1)
+    extra = [0.05 * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)]
+    amps = (0.21 - 0.13j, 0.83 + 0.31j, np.exp(0.37j))
+    intensity(mask, "X", *extra, cfg=cfg, amplitudes=amps)
+    torch.cuda.synchronize()
+    ts, dts = [], []
+    for _ in range(reps):
+        tm = {}
+        t0 = time.perf_counter()
+        intensity(mask, "X", *extra, cfg=cfg, amplitudes=amps, timing=tm)
+        ts.append(time.perf_counter() - t0)
+        dts.append(tm["device_ms"] * 1e-3)
+    t, dt = float(np.median(ts)), float(np.median(dts))
+    ncut, ns = len(cfg.pupil()), len(cfg.source())
+    flops = 3 * 8.0 * ncut * ncut * ns
+    res = {"value": 1.0 / t, "unit": "images/s", "config": f"TCCEE X with M3D arrays, 2048 nm, {ncut} pupil "
+           f"points, {ns} source points, {cfg.xdiv}x{cfg.xdiv} image", "ms_per_image": 1000 * t,
+           "device_ms_per_image": 1000 * dt,
+           "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel (TCC)", "achieved": flops / dt / 1e12,
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": flops / dt / 1e12 / FP64_PEAK_TFLOPS,
+                        "note": "TCC flops over the stream time of all the image's kernels and uploads"}}
+    if with_cpu:
+        from oracle import stcc as OS
+        c = OS.constants()
+        t0 = time.perf_counter()
+        OS.intensity(c, "X", OS.mask_fft_window(c, mask), *amps, *extra)
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": 1.0 / tc, "unit": "images/s", "cores": len(os.sched_getaffinity(0)),
+                               "kind": "port", "sample": f"one image, the oracle (NumPy, vectorised sums): {tc:.1f} s"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +328,7 @@ def main():
     ap.add_argument("--no-mask-producer", action="store_true", help="skip the mask producer leg")
     ap.add_argument("--no-cnn", action="store_true", help="skip the CNN predictor leg")
     ap.add_argument("--cnn-only", action="store_true", help="only run the CNN predictor leg")
+    ap.add_argument("--no-stcc", action="store_true", help="skip the TCCEE image leg")
     ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -421,6 +468,8 @@ def main():
         line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)
     if not args.no_cnn:
         line["cnn_predictor"] = cnn_predictor(with_cpu=not args.no_cpu_baseline)
+    if not args.no_stcc:
+        line["tccee_image"] = stcc_leg(with_cpu=not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cpu_baseline(cfg, masks, threads)

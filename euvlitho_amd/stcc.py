@@ -15,6 +15,7 @@ amp_vacuum, phasexx of diffraction_amplitude.zero_order_amplitude) come from
 this library's blank-mask solve of the same optics, or are passed in.
 """
 import ctypes
+import functools
 from dataclasses import dataclass
 
 import numpy as np
@@ -68,36 +69,47 @@ class StccConfig:
         """pupil.py:24-44 (offsets 0): (l, m) of the points with at least one
         lattice source inside the NA whose shifted point is inside too,
         ip-major."""
-        lsx, lsy = self.lsmax
-        lpx, lpy = self.lpmax
-        na2 = (self.na / self.wavelength) ** 2
-        sx, sy = np.meshgrid(np.arange(-lsx, lsx + 1), np.arange(-lsy, lsy + 1), indexing="ij")
-        src_ok = (sx * self.mx / self.ndiv) ** 2 + (sy * self.my / self.ndiv) ** 2 <= na2
-        out = []
-        for ip in range(2 * lpx + 1):
-            for jp in range(2 * lpy + 1):
-                ok = ((ip - lpx + sx) * self.mx / self.ndiv) ** 2 + ((jp - lpy + sy) * self.my / self.ndiv) ** 2 <= na2
-                if np.any(src_ok & ok):
-                    out.append((ip - lpx, jp - lpy))
-        return np.array(out, dtype=np.int32)
+        return _pupil(self).copy()
 
     def source(self):
-        """source.py:157-199: (sx, sy) of the 0.5-degree mesh inside the source."""
-        k = self.k
-        kmesh = k * self.mesh * (np.pi / 180.0)
-        l0max = int(k * self.na / self.mx * self.sigma1 / kmesh + 1)
-        m0max = int(k * self.na / self.my * self.sigma1 / kmesh + 1)
-        L, M = np.meshgrid(np.arange(-l0max, l0max + 1), np.arange(-m0max, m0max + 1), indexing="ij")
-        skx, sky = L * kmesh, M * kmesh
-        skxo, skyo = skx * self.mx, sky * self.my
-        if self.illumination == "circular":
-            sel = skxo ** 2 + skyo ** 2 <= (k * self.na * self.sigma1) ** 2
-        elif self.illumination == "annular":
-            r = np.sqrt(skxo ** 2 + skyo ** 2)
-            sel = (k * self.na * self.sigma2 <= r) & (r <= k * self.na * self.sigma1)
-        else:
-            raise ValueError("Invalid illumination type")
-        return np.stack([skx[sel], sky[sel]], axis=1).astype(np.float64)
+        """source.py:157-199: (sx, sy) of the mesh points inside the source."""
+        return _source(self).copy()
+
+
+@functools.lru_cache(maxsize=16)
+def _pupil(cfg):
+    lsx, lsy = cfg.lsmax
+    lpx, lpy = cfg.lpmax
+    na2 = (cfg.na / cfg.wavelength) ** 2
+    sx = np.arange(-lsx, lsx + 1)[None, None, :, None]
+    sy = np.arange(-lsy, lsy + 1)[None, None, None, :]
+    ip = np.arange(2 * lpx + 1)[:, None, None, None]
+    jp = np.arange(2 * lpy + 1)[None, :, None, None]
+    src_ok = (sx * cfg.mx / cfg.ndiv) ** 2 + (sy * cfg.my / cfg.ndiv) ** 2 <= na2
+    ok = ((ip - lpx + sx) * cfg.mx / cfg.ndiv) ** 2 + ((jp - lpy + sy) * cfg.my / cfg.ndiv) ** 2 <= na2
+    hit = np.any(src_ok & ok, axis=(2, 3))
+    I, J = np.nonzero(hit)  # ip-major, as the reference's loops
+    return np.stack([I - lpx, J - lpy], axis=1).astype(np.int32)
+
+
+@functools.lru_cache(maxsize=16)
+def _source(cfg):
+    """source.py:157-199: (sx, sy) of the mesh points inside the source."""
+    k = cfg.k
+    kmesh = k * cfg.mesh * (np.pi / 180.0)
+    l0max = int(k * cfg.na / cfg.mx * cfg.sigma1 / kmesh + 1)
+    m0max = int(k * cfg.na / cfg.my * cfg.sigma1 / kmesh + 1)
+    L, M = np.meshgrid(np.arange(-l0max, l0max + 1), np.arange(-m0max, m0max + 1), indexing="ij")
+    skx, sky = L * kmesh, M * kmesh
+    skxo, skyo = skx * cfg.mx, sky * cfg.my
+    if cfg.illumination == "circular":
+        sel = skxo ** 2 + skyo ** 2 <= (k * cfg.na * cfg.sigma1) ** 2
+    elif cfg.illumination == "annular":
+        r = np.sqrt(skxo ** 2 + skyo ** 2)
+        sel = (k * cfg.na * cfg.sigma2 <= r) & (r <= k * cfg.na * cfg.sigma1)
+    else:
+        raise ValueError("Invalid illumination type")
+    return np.stack([skx[sel], sky[sel]], axis=1).astype(np.float64)
 
 
 def zero_order_amplitude(cfg: StccConfig = StccConfig(), device="cuda:0"):
@@ -123,12 +135,13 @@ def _bits_of(mask, ndiv):
 
 
 def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig = StccConfig(), amplitudes=None,
-              device="cuda:0", handle=None):
+              device="cuda:0", handle=None, timing=None):
     """stcc.py:394-431: the TCCEE image [x][y] (xdiv x xdiv float64 host
     array).  polar 'X' or 'Y' (elitho PolarizationDirection); the M3D arrays
     are all given or all None ('Y' needs them, as the reference's branch
     reads them unconditionally).  amplitudes = (amp_absorber, amp_vacuum,
-    phasexx) overrides the blank-mask solve."""
+    phasexx) overrides the blank-mask solve.  timing (a dict) receives the
+    stream time in ms of the library calls ("device_ms")."""
     polar = getattr(polar, "name", polar)
     if polar not in ("X", "Y"):
         raise ValueError("polar must be 'X' or 'Y'")
@@ -145,6 +158,9 @@ def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig 
     try:
         _lib.check(lib.euvl_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)),
                    "euvl_set_stream")
+        if timing is not None:
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+            ev[0].record()
         _lib.check(lib.euvl_mask_spectrum(h, _lib.ptr(bits), cfg.ndiv, cfg.ndiv, 1, lpx, lpy, _lib.ptr(fm)),
                    "euvl_mask_spectrum")
         m3d = [None if a is None else torch.as_tensor(np.asarray(a, dtype=np.complex128)).to(dev).contiguous()
@@ -161,6 +177,10 @@ def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig 
         _lib.check(lib.euvl_stcc_image(h, polar.encode(), ctypes.byref(prm), _lib.ptr(fm), cplx(ab), cplx(vc),
                                        cplx(ph), p(m3d[0]), p(m3d[1]), p(m3d[2]), len(pupil), pupil.ctypes.data,
                                        len(src), src.ctypes.data, _lib.ptr(img)), "euvl_stcc_image")
+        if timing is not None:
+            ev[1].record()
+            ev[1].synchronize()
+            timing["device_ms"] = ev[0].elapsed_time(ev[1])
         return img.cpu().numpy()
     finally:
         if own:
