@@ -47,6 +47,8 @@ struct F64 {
   using T2 = double2;
   using V4 = double4_t;
   static constexpr bool three = false;
+  static constexpr int load_step = 4;  // the next slab's loads follow the first K step's MFMAs
+  static constexpr bool fast_loads = true;  // unchecked loads for interior tiles
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -62,6 +64,8 @@ struct F32 {
   using T2 = float2;
   using V4 = float4_t;
   static constexpr bool three = false;
+  static constexpr int load_step = 0;  // ahead of the slab's MFMAs
+  static constexpr bool fast_loads = false;  // (more registers: 128x128 would lose its second workgroup per CU)
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -118,16 +122,34 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
   const int b_k = tid / (BN / QB), b_c = (tid % (BN / QB)) * QB;
 
   T2 ra[QA], rb[QB];
+  const bool interior = row0 + BM <= M && col0 + BN <= N;
   auto load_slab = [&](int k0) {
     const int gr = row0 + a_r;
+    const int gk = k0 + b_k;
+    if (P::fast_loads && interior && k0 + BK <= K) {  // uniform: no per-element checks
+      const double2* pa = A + (int64_t)gr * lda + k0 + a_k;
+      const double2* pb = B + (int64_t)gk * ldb + col0 + b_c;
+#pragma unroll
+      for (int q = 0; q < QA; ++q) {
+        const double2 v = pa[q];
+        ra[q].x = (T)v.x;
+        ra[q].y = (T)v.y;
+      }
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+        const double2 v = pb[q];
+        rb[q].x = (T)v.x;
+        rb[q].y = (T)v.y;
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
-      const int gk = k0 + a_k + q;
-      const double2 v = (gr < M && gk < K) ? A[(int64_t)gr * lda + gk] : make_double2(0.0, 0.0);
+      const int ak = k0 + a_k + q;
+      const double2 v = (gr < M && ak < K) ? A[(int64_t)gr * lda + ak] : make_double2(0.0, 0.0);
       ra[q].x = (T)v.x;
       ra[q].y = (T)v.y;
     }
-    const int gk = k0 + b_k;
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       const int gc = col0 + b_c + q;
@@ -161,9 +183,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 #pragma unroll
     for (int q = 0; q < QB; ++q) Bs[b_k][b_c + q] = rb[q];
     __syncthreads();
-    if (k0 + BK < K) load_slab(k0 + BK);
+    const bool more = k0 + BK < K;
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
+      if (ks == P::load_step && more) load_slab(k0 + BK);
       T2 fa[TM], fb[TN];
       T nai[TM];
 #pragma unroll
@@ -257,7 +280,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 // overrides it for tuning runs.  The 3M form has the same variants except 3
 // (4x4 waves of 2x1 tiles, 128x64: three accumulator sets of 2x2 tiles spill
 // at 1024 threads); in complex-product TFLOP/s (8mnk per time) at 1681 x 4 /
-// 4096: 58/72, 66/84, 58/73, 63/77, 56/69, against 55/69 for the 4M default.
+// 4096: 48/57, 71/93, 64/80, 66/83, 63/85 with unchecked interior loads
+// issued after the first K step (before: 58/72, 66/84, 58/73, 63/77, 56/69;
+// 55/69 for the 4M form).  A register-only loop of f64 16x16x4 MFMAs
+// sustains 77.9 TFLOP/s on MI355X, f32 16x16x4 155.5.
 // FP32-compute variants (waves WM x WN, each TM x TN MFMA tiles, K slab 16):
 // 0 = 2x2 waves, 64x64; 1 = 2x4 waves, 64x128; 2 = 4x2 waves, 128x64;
 // 3 = 4x4 waves, 128x128; 4 = 2x2 waves of 4x2 tiles, 128x64.
