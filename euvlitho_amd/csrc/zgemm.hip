@@ -175,52 +175,23 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 #pragma unroll
     for (int u = 0; u < SN; ++u) acc_s[t][u] = (V4){0, 0, 0, 0};
 
-  load_slab(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    __syncthreads();
+  // one K step of 4: the fragments of As/Bs rows ks..ks+3, then the MFMAs
+  auto kstep = [&](int ks) {
+    T2 fa[TM], fb[TN];
+    T nai[TM];
 #pragma unroll
-    for (int q = 0; q < QA; ++q) As[a_k + q][a_r] = ra[q];
+    for (int t = 0; t < TM; ++t) {
+      fa[t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
+      nai[t] = -fa[t].y;
+    }
 #pragma unroll
-    for (int q = 0; q < QB; ++q) Bs[b_k][b_c + q] = rb[q];
-    __syncthreads();
-    const bool more = k0 + BK < K;
+    for (int u = 0; u < TN; ++u) fb[u] = Bs[ks + lk][wn * 16 * TN + u * 16 + li];
+    if constexpr (M3) {
+      T sa[TM], sb[TN];
 #pragma unroll
-    for (int ks = 0; ks < BK; ks += 4) {
-      if (ks == P::load_step && more) load_slab(k0 + BK);
-      T2 fa[TM], fb[TN];
-      T nai[TM];
+      for (int t = 0; t < TM; ++t) sa[t] = fa[t].x + fa[t].y;
 #pragma unroll
-      for (int t = 0; t < TM; ++t) {
-        fa[t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
-        nai[t] = -fa[t].y;
-      }
-#pragma unroll
-      for (int u = 0; u < TN; ++u) fb[u] = Bs[ks + lk][wn * 16 * TN + u * 16 + li];
-      if constexpr (M3) {
-        T sa[TM], sb[TN];
-#pragma unroll
-        for (int t = 0; t < TM; ++t) sa[t] = fa[t].x + fa[t].y;
-#pragma unroll
-        for (int u = 0; u < TN; ++u) sb[u] = fb[u].x + fb[u].y;
-#pragma unroll
-        for (int t = 0; t < TM; ++t)
-#pragma unroll
-          for (int u = 0; u < TN; ++u)
-            acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
-#pragma unroll
-        for (int t = 0; t < TM; ++t)
-#pragma unroll
-          for (int u = 0; u < TN; ++u)
-            acc_im[t][u] = P::mfma(fa[t].y, fb[u].y, acc_im[t][u]);
-#pragma unroll
-        for (int t = 0; t < SM; ++t)
-#pragma unroll
-          for (int u = 0; u < SN; ++u)
-            acc_s[t][u] = P::mfma(sa[t], sb[u], acc_s[t][u]);
-        continue;
-      }
-      // four passes over the tiles so that successive updates of one
-      // accumulator are separated by independent MFMAs
+      for (int u = 0; u < TN; ++u) sb[u] = fb[u].x + fb[u].y;
 #pragma unroll
       for (int t = 0; t < TM; ++t)
 #pragma unroll
@@ -230,17 +201,64 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_im[t][u] = P::mfma(fa[t].x, fb[u].y, acc_im[t][u]);
+          acc_im[t][u] = P::mfma(fa[t].y, fb[u].y, acc_im[t][u]);
 #pragma unroll
-      for (int t = 0; t < TM; ++t)
+      for (int t = 0; t < SM; ++t)
 #pragma unroll
-        for (int u = 0; u < TN; ++u)
-          acc_re[t][u] = P::mfma(nai[t], fb[u].y, acc_re[t][u]);
+        for (int u = 0; u < SN; ++u)
+          acc_s[t][u] = P::mfma(sa[t], sb[u], acc_s[t][u]);
+      return;
+    }
+    // four passes over the tiles so that successive updates of one
+    // accumulator are separated by independent MFMAs
 #pragma unroll
-      for (int t = 0; t < TM; ++t)
+    for (int t = 0; t < TM; ++t)
 #pragma unroll
-        for (int u = 0; u < TN; ++u)
-          acc_im[t][u] = P::mfma(fa[t].y, fb[u].x, acc_im[t][u]);
+      for (int u = 0; u < TN; ++u)
+        acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+      for (int u = 0; u < TN; ++u)
+        acc_im[t][u] = P::mfma(fa[t].x, fb[u].y, acc_im[t][u]);
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+      for (int u = 0; u < TN; ++u)
+        acc_re[t][u] = P::mfma(nai[t], fb[u].y, acc_re[t][u]);
+#pragma unroll
+    for (int t = 0; t < TM; ++t)
+#pragma unroll
+      for (int u = 0; u < TN; ++u)
+        acc_im[t][u] = P::mfma(fa[t].y, fb[u].x, acc_im[t][u]);
+  };
+
+  // waves whose whole sub-tile lies outside C (edge tiles) add nothing: they
+  // skip their MFMAs (FP64) and only stage their share of the slabs
+  const bool active = row0 + wm * 16 * TM < M && col0 + wn * 16 * TN < N;
+  load_slab(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QA; ++q) As[a_k + q][a_r] = ra[q];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) Bs[b_k][b_c + q] = rb[q];
+    __syncthreads();
+    const bool more = k0 + BK < K;
+    if constexpr (!P::fast_loads) {  // FP32: the plain loop keeps 128x128 at two workgroups per CU
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        if (ks == P::load_step && more) load_slab(k0 + BK);
+        kstep(ks);
+      }
+    } else if (active) {
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        if (ks == P::load_step && more) load_slab(k0 + BK);
+        kstep(ks);
+      }
+    } else if (more) {
+      load_slab(k0 + BK);
     }
   }
 
