@@ -35,6 +35,7 @@
 #include <map>
 #include <mutex>
 #include <tuple>
+#include <utility>
 
 #include "ew.h"
 
@@ -76,7 +77,7 @@ template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
 __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     int M, int N, int K, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sa,
     const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* C,
-    int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles) {
+    int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles, int shift) {
   constexpr int NT = 64 * WM * WN;                      // WM x WN waves
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;   // workgroup tile
   constexpr int QA = BM * BK / NT, QB = BN * BK / NT;   // complex values per thread per slab
@@ -107,8 +108,16 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
   const int first = group * GM;
   const int gsize = min(tiles_m - first, GM);
   const int in_group = tile - group * GM * tiles_n;
-  const int row0 = (first + in_group % gsize) * BM;
-  const int col0 = (in_group / gsize) * BN;
+  // the tile this workgroup owns (and stores)
+  const int own_r = (first + in_group % gsize) * BM;
+  const int own_c = (in_group / gsize) * BN;
+  // shift (set when C is at least one tile in each direction and overlaps no
+  // operand): an edge tile is moved back inside C, so that every load is in
+  // bounds and takes the unchecked path; it recomputes rows/columns of its
+  // neighbour and stores only its own part
+  const bool shifted = P::fast_loads && shift;
+  const int row0 = shifted ? min(own_r, M - BM) : own_r;
+  const int col0 = shifted ? min(own_c, N - BN) : own_c;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -235,7 +244,8 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 
   // waves whose whole sub-tile lies outside C (edge tiles) add nothing: they
   // skip their MFMAs (FP64) and only stage their share of the slabs
-  const bool active = row0 + wm * 16 * TM < M && col0 + wn * 16 * TN < N;
+  const int wr0 = row0 + wm * 16 * TM, wc0 = col0 + wn * 16 * TN;
+  const bool active = wr0 < M && wr0 + 16 * TM > own_r && wc0 < N && wc0 + 16 * TN > own_c;
   load_slab(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
     __syncthreads();
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
       for (int r = 0; r < 4; ++r) {
         const int gr = row0 + wm * 16 * TM + t * 16 + P::row(r, lk);
         const int gc = col0 + wn * 16 * TN + u * 16 + li;
-        if (gr < M && gc < N) {
+        if (gr < M && gc < N && gr >= own_r && gc >= own_c) {
           double2 v;
           if constexpr (M3) {
             const double p1 = acc_re[t][u][r], p2 = acc_im[t][u][r];
@@ -364,8 +374,20 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;
   const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
   const int tiles = tiles_m * tiles_n;
+  // edge tiles may be shifted back inside C unless C overlaps an operand (an
+  // in-place product: a shifted tile would read rows another tile writes)
+  auto span = [](const double2* p, int64_t stride, int nb, int rows, int64_t ld, int cols) {
+    const char* lo = (const char*)p;
+    const int64_t n = (nb - 1) * stride + (int64_t)(rows - 1) * ld + cols;
+    return std::make_pair(lo, lo + 16 * n);
+  };
+  const auto c = span(C, sc, batch, m, ldc, n);
+  const auto a = span(A, sa, batch, m, lda, k), b = span(B, sb, batch, k, ldb, n);
+  const bool overlap = k > 0 && ((a.first < c.second && c.first < a.second) || (b.first < c.second && c.first < b.second));
+  const int shift = (m >= BM && n >= BN && !overlap) ? 1 : 0;
   hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
-                     h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles);
+                     h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles,
+                     shift);
 }
 
 // kind: 0 = FP64 4M, 1 = FP32 compute, 2 = FP64 3M

@@ -190,3 +190,34 @@ def test_autotune_keeps_results(lib):
     e = np.abs(after - before).max() / np.abs(before).max()
     say(f"  amplitudes after tuning vs before: {e:.1e}")
     assert e < 1e-12
+
+
+@pytest.mark.parametrize("m,kn,batch", [(200, 48, 2), (1681, 64, 1), (130, 20, 3)])
+def test_zgemm_in_place_rows(lib, m, kn, batch):
+    """C = A B with C the same storage as A (k = n <= one tile column, as the
+    LU's panel products use it): every workgroup reads only the rows it
+    overwrites, in both FP64 forms, and edge tiles are not moved onto rows
+    that another workgroup writes (1e-13 of the exact product)."""
+    from euvlitho_amd import _lib
+    say(f"in-place zgemm m={m} k=n={kn} batch={batch}")
+    rng = np.random.default_rng(m + kn)
+    A = rng.standard_normal((batch, m, kn)) + 1j * rng.standard_normal((batch, m, kn))
+    B = rng.standard_normal((batch, kn, kn)) + 1j * rng.standard_normal((batch, kn, kn))
+    ref = A @ B
+    h = ctypes.c_void_p()
+    assert lib.euvl_create(ctypes.byref(h)) == 0
+    try:
+        for mask in (_lib.PREC_FP64, _lib.PREC_GEMM_4M):
+            dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+            assert lib.euvl_set_precision(h, mask) == 0
+            one, zero = (ctypes.c_double * 2)(1.0, 0.0), (ctypes.c_double * 2)(0.0, 0.0)
+            rc = lib.euvl_zgemm_batched(h, m, kn, kn, one, ctypes.c_void_p(dA.data_ptr()), kn, m * kn,
+                                        ctypes.c_void_p(dB.data_ptr()), kn, kn * kn, zero,
+                                        ctypes.c_void_p(dA.data_ptr()), kn, m * kn, None, batch)
+            assert rc == 0, lib.euvl_last_error().decode()
+            torch.cuda.synchronize()
+            e = np.abs(dA.cpu().numpy() - ref).max() / np.abs(ref).max()
+            say(f"  prec {mask}: rel err {e:.2e}")
+            assert e < 1e-13
+    finally:
+        lib.euvl_destroy(h)
