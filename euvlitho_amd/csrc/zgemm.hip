@@ -48,8 +48,7 @@ struct F64 {
   using T2 = double2;
   using V4 = double4_t;
   static constexpr bool three = false;
-  static constexpr int load_step = 4;  // the next slab's loads follow the first K step's MFMAs
-  static constexpr bool fast_loads = true;  // unchecked loads for interior tiles
+  static constexpr bool fast_loads = true;  // default of the FAST kernel option
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -65,15 +64,18 @@ struct F32 {
   using T2 = float2;
   using V4 = float4_t;
   static constexpr bool three = false;
-  static constexpr int load_step = 0;  // ahead of the slab's MFMAs
-  static constexpr bool fast_loads = false;  // (more registers: 128x128 would lose its second workgroup per CU)
+  static constexpr bool fast_loads = false;
   static __device__ __forceinline__ V4 mfma(T a, T b, V4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int r, int lk) { return 4 * lk + r; }
 };
 
-template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
+// FAST: unchecked loads for interior tiles (edge tiles shifted back inside C
+// when allowed), issued after the first K step's MFMAs, and MFMAs skipped by
+// waves outside C.  It costs registers: the FP32 128x128 tile keeps the plain
+// form (checked loads ahead of the MFMAs) to stay at two workgroups per CU.
+template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST>
 __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     int M, int N, int K, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sa,
     const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* C,
@@ -115,7 +117,7 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
   // operand): an edge tile is moved back inside C, so that every load is in
   // bounds and takes the unchecked path; it recomputes rows/columns of its
   // neighbour and stores only its own part
-  const bool shifted = P::fast_loads && shift;
+  const bool shifted = FAST && shift;
   const int row0 = shifted ? min(own_r, M - BM) : own_r;
   const int col0 = shifted ? min(own_c, N - BN) : own_c;
 
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
   auto load_slab = [&](int k0) {
     const int gr = row0 + a_r;
     const int gk = k0 + b_k;
-    if (P::fast_loads && interior && k0 + BK <= K) {  // uniform: no per-element checks
+    if (FAST && interior && k0 + BK <= K) {  // uniform: no per-element checks
       const double2* pa = A + (int64_t)gr * lda + k0 + a_k;
       const double2* pb = B + (int64_t)gk * ldb + col0 + b_c;
 #pragma unroll
@@ -255,16 +257,16 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     for (int q = 0; q < QB; ++q) Bs[b_k][b_c + q] = rb[q];
     __syncthreads();
     const bool more = k0 + BK < K;
-    if constexpr (!P::fast_loads) {  // FP32: the plain loop keeps 128x128 at two workgroups per CU
+    if constexpr (!FAST) {
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
-        if (ks == P::load_step && more) load_slab(k0 + BK);
+        if (ks == 0 && more) load_slab(k0 + BK);
         kstep(ks);
       }
     } else if (active) {
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
-        if (ks == P::load_step && more) load_slab(k0 + BK);
+        if (ks == 4 && more) load_slab(k0 + BK);
         kstep(ks);
       }
     } else if (more) {
@@ -352,22 +354,20 @@ int pick_variant(int kind, int m, int n, int k, int batch) {
   return t >= 0 ? t : g_variant;
 }
 
-// FP32 compute: larger tiles halve the operand bytes per flop, which is what
-// limits this kernel once the MFMA rate doubles, but need enough workgroups
-// to fill 256 CUs.  Measured on MI355X (TFLOP/s, N x N x N x batch):
-//   1681 x 4: 64x64 75, 64x128 84, 128x128 74;  4096: 98, 109, 123;
-//   8705: 67, 92, 108.
+// FP32 compute.  With the FAST form the 64x64 tile (five workgroups per CU)
+// is the best or tied at every size measured on MI355X (TFLOP/s, 8mnk per
+// time, variants 0/1/2/3/4; variant 3 keeps the plain form):
+//   1681 x 4: 102/99/100/77/96;  4096: 130/124/127/124/114;
+//   8705: 123/119/123/108/110;  2048 x 4: 128/121/125/121/113;  420 x 4: 40/29/30/17/24
+// (before the FAST form, 1681 x 4: 79/88/89/78/65 and 4096: 102/124/110/124/86).
 int pick_variant32(int m, int n, int k, int batch) {
   if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 4);
   if (g_variant32 >= 0) return g_variant32;
   const int t = tuned_variant(1, m, n, k, batch);
-  if (t >= 0) return t;
-  if (n <= 64) return 0;
-  const int64_t big_tiles = (int64_t)((m + 127) / 128) * ((n + 127) / 128) * batch;
-  return big_tiles >= 1024 ? 3 : 1;
+  return t >= 0 ? t : 0;
 }
 
-template <class P, int WM, int WN, int TM, int TN, int BK, int MINB>
+template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST = P::fast_loads>
 void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
             const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
             double2 gamma, int batch) {
@@ -385,7 +385,7 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
   const auto a = span(A, sa, batch, m, lda, k), b = span(B, sb, batch, k, ldb, n);
   const bool overlap = k > 0 && ((a.first < c.second && c.first < a.second) || (b.first < c.second && c.first < b.second));
   const int shift = (m >= BM && n >= BN && !overlap) ? 1 : 0;
-  hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
+  hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB, FAST>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles,
                      shift);
 }
@@ -410,11 +410,11 @@ void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2
     }
   } else if (kind == 1) {
     switch (v) {
-      case 1: launch<F32, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      case 2: launch<F32, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      case 1: launch<F32, 2, 4, 2, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F32, 4, 2, 2, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
       case 3: launch<F32, 4, 4, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
-      case 4: launch<F32, 2, 2, 4, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      default: launch<F32, 2, 2, 2, 2, 16, 4>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F32, 2, 2, 4, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      default: launch<F32, 2, 2, 2, 2, 16, 4, true>(EUVL_GEMM_ARGS); break;
     }
   } else {
     switch (v) {
