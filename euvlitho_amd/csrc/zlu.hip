@@ -243,75 +243,77 @@ __global__ __launch_bounds__(256) void tslu_apply_kernel(double2* __restrict__ A
 // GEMMs (L21 = A21 U11^-1, U12 = L11^-1 A12), as does the trailing update.
 // No column step of the factorisation runs over the panel's full height, so
 // a panel costs a few short launches instead of 32 dependent pivot searches.
-__global__ __launch_bounds__(64) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
-                                                     int nbc, double2* __restrict__ uinv, double2* __restrict__ linv,
-                                                     int* __restrict__ ipiv, int n, int* __restrict__ info) {
-  // one wave: lane r < 32 holds row r of the block in registers; the pivot
-  // row of each step is broadcast through LDS (no barrier needed in a single
-  // wave); then lanes 0-31 form the columns of U11^-1 and lanes 32-63 those
-  // of L11^-1, reading the factors from LDS
+// The block is small and factored once per launch, so the kernel is written
+// as short loops over LDS (two waves): a fully unrolled register form runs
+// to ~20k instructions and is bound by instruction fetch.
+constexpr int DLU_THREADS = 128;
+__global__ __launch_bounds__(DLU_THREADS) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa,
+                                                              int kb, int nbc, double2* __restrict__ uinv,
+                                                              double2* __restrict__ linv, int* __restrict__ ipiv,
+                                                              int n, int* __restrict__ info) {
   __shared__ double2 a[NB][NB + 1];
-  const int b = blockIdx.x, lane = threadIdx.x;
+  __shared__ double2 ys[2][NB][NB + 1];  // ys[0][i][c] = U11^-1(i, c), ys[1][i][c] = L11^-1(i, c)
+  const int b = blockIdx.x, t = threadIdx.x;
   double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
-  const int r = lane & (NB - 1);
-  double2 x[NB];
-#pragma unroll
-  for (int q = 0; q < NB; ++q) {
+  for (int e = t; e < NB * NB; e += DLU_THREADS) {
+    const int r = e / NB, q = e % NB;
     const bool in = r < nbc && q < nbc;
-    const double2 v = A11[(int64_t)min(r, nbc - 1) * lda + min(q, nbc - 1)];
     const double pad = (r == q) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
-    x[q] = make_double2(in ? v.x : pad, in ? v.y : 0.0);
+    a[r][q] = in ? A11[(int64_t)r * lda + q] : make_double2(pad, 0.0);
   }
-#pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (lane == j) {
-#pragma unroll
-      for (int q = 0; q < NB; ++q) a[j][q] = x[q];  // row j is final: U(j, j:) and L(j, :j)
-    }
-    __syncthreads();  // one wave: orders the LDS write before the reads
+  __syncthreads();
+  // right-looking elimination: multipliers of column j, then the rank-1
+  // update of the trailing block (4 rows x 32 columns of threads)
+  const int tq = t & (NB - 1), tr = t / NB;
+#pragma unroll 1
+  for (int j = 0; j < NB - 1; ++j) {
     const double2 d = a[j][j];
-    if (lane < NB && r > j && (d.x != 0.0 || d.y != 0.0)) {
-      const double2 l = cdiv(x[j], d);
-      x[j] = l;
-#pragma unroll
-      for (int q = j + 1; q < NB; ++q) x[q] = csub(x[q], cmul(l, a[j][q]));
+    const bool nz = d.x != 0.0 || d.y != 0.0;
+    if (nz && t > j && t < NB) a[t][j] = cdiv(a[t][j], d);
+    __syncthreads();
+    if (nz && tq > j)
+#pragma unroll 1
+      for (int i = j + 1 + tr; i < NB; i += DLU_THREADS / NB) a[i][tq] = csub(a[i][tq], cmul(a[i][j], a[j][tq]));
+    __syncthreads();
+  }
+  for (int e = t; e < NB * NB; e += DLU_THREADS) {
+    const int r = e / NB, q = e % NB;
+    if (r < nbc && q < nbc) A11[(int64_t)r * lda + q] = a[r][q];
+  }
+  if (t < nbc) ipiv[(int64_t)b * n + kb + t] = kb + t;
+  if (t == 0)
+#pragma unroll 1
+    for (int j = 0; j < nbc; ++j)
+      if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
+  // column c of U11^-1 (wave 0): y(i) for i = NB-1 .. 0, y(i) = (e_c(i) - sum_{k>i} U(i,k) y(k)) / U(i,i);
+  // column c of L11^-1 (wave 1): y(i) for i = 0 .. NB-1, y(i) = e_c(i) - sum_{k<i} L(i,k) y(k);
+  // each lane reads back only its own column of ys
+  const int w = t / 64, c = t & 63;
+  if (c < NB) {
+    if (w == 0) {
+#pragma unroll 1
+      for (int i = NB - 1; i >= 0; --i) {
+        double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll 2
+        for (int k = i + 1; k < NB; ++k) s = csub(s, cmul(a[i][k], ys[0][k][c]));
+        const double2 u = cdiv(s, a[i][i]);
+        ys[0][i][c] = make_double2(i <= c ? u.x : 0.0, i <= c ? u.y : 0.0);
+      }
+    } else {
+#pragma unroll 1
+      for (int i = 0; i < NB; ++i) {
+        double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
+#pragma unroll 2
+        for (int k = 0; k < i; ++k) s = csub(s, cmul(a[i][k], ys[1][k][c]));
+        ys[1][i][c] = make_double2(i >= c ? s.x : 0.0, i >= c ? s.y : 0.0);
+      }
     }
   }
   __syncthreads();
-  if (lane < nbc) {
-#pragma unroll
-    for (int q = 0; q < NB; ++q)
-      if (q < nbc) A11[(int64_t)lane * lda + q] = a[lane][q];
-    ipiv[(int64_t)b * n + kb + lane] = kb + lane;
-  }
-  if (lane == 0)
-    for (int j = 0; j < nbc; ++j)
-      if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
-  // column c of U11^-1 (lanes 0-31): y(i) for i = c .. 0, y(i) = (e_c(i) - sum_{k>i} U(i,k) y(k)) / U(i,i);
-  // column c of L11^-1 (lanes 32-63): y(i) for i = c .. NB-1, y(i) = e_c(i) - sum_{k<i} L(i,k) y(k)
-  const int c = r;
-  double2 y[NB];
-  if (lane < NB) {
-#pragma unroll
-    for (int i = NB - 1; i >= 0; --i) {
-      double2 t = make_double2(i == c ? 1.0 : 0.0, 0.0);
-#pragma unroll
-      for (int k = i + 1; k < NB; ++k) t = csub(t, cmul(a[i][k], y[k]));
-      const double2 u = cdiv(t, a[i][i]);
-      y[i] = make_double2(i <= c ? u.x : 0.0, i <= c ? u.y : 0.0);
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) uinv[((int64_t)b * NB + i) * NB + c] = y[i];
-  } else {
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      double2 t = make_double2(i == c ? 1.0 : 0.0, 0.0);
-#pragma unroll
-      for (int k = 0; k < i; ++k) t = csub(t, cmul(a[i][k], y[k]));
-      y[i] = make_double2(i >= c ? t.x : 0.0, i >= c ? t.y : 0.0);
-    }
-#pragma unroll
-    for (int i = 0; i < NB; ++i) linv[((int64_t)b * NB + i) * NB + c] = y[i];
+  for (int e = t; e < NB * NB; e += DLU_THREADS) {
+    const int i = e / NB, cc = e % NB;
+    uinv[((int64_t)b * NB + i) * NB + cc] = ys[0][i][cc];
+    linv[((int64_t)b * NB + i) * NB + cc] = ys[1][i][cc];
   }
 }
 
@@ -335,7 +337,12 @@ __global__ void laswp_kernel(int ncols, double2* A, int64_t lda, int64_t sa, con
 }
 
 // X = T^-1 X for an nbc x nbc triangular block T (unit lower, or non-unit
-// upper when UPPER), X nbc x ncols; one thread per column.
+// upper when UPPER), X nbc x ncols.  Each column is solved by 32 lanes, lane
+// i holding x(i): at step p the finished x(p) is broadcast to its column's
+// lanes and the rows it feeds subtract T(i, p) x(p) in parallel (a column
+// per thread would leave a 171-column right-hand side on a handful of CUs
+// behind a 500-step dependent chain).
+constexpr int TRSM_COLS = 8;  // columns per 256-thread workgroup
 template <bool UPPER>
 __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, int64_t ldt, int64_t st, double2* X,
                                                    int64_t ldx, int64_t sx, int ncols) {
@@ -344,40 +351,29 @@ __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, in
   T += (int64_t)b * st;
   X += (int64_t)b * sx;
   for (int e = threadIdx.x; e < nbc * nbc; e += blockDim.x) {
-    int r = e / nbc, c = e % nbc;
+    const int r = e / nbc, c = e % nbc;
     sT[r][c] = T[(int64_t)r * ldt + c];
   }
   __syncthreads();
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= ncols) return;
-  double2 x[NB];
-#pragma unroll
-  for (int i = 0; i < NB; ++i) x[i] = (i < nbc) ? X[(int64_t)i * ldx + j] : make_double2(0.0, 0.0);
+  const int i = threadIdx.x & (NB - 1);
+  const int j = blockIdx.x * TRSM_COLS + threadIdx.x / NB;
+  const bool live = j < ncols && i < nbc;  // every lane takes part in the broadcasts
+  double2 x = live ? X[(int64_t)i * ldx + j] : make_double2(0.0, 0.0);
   if (!UPPER) {
-#pragma unroll
-    for (int i = 0; i < NB; ++i) {
-      if (i < nbc) {
-        double2 s = x[i];
-#pragma unroll
-        for (int p = 0; p < i; ++p) s = csub(s, cmul(sT[i][p], x[p]));
-        x[i] = s;
-      }
+#pragma unroll 1
+    for (int p = 0; p < nbc - 1; ++p) {
+      const double2 xp = make_double2(__shfl(x.x, p, NB), __shfl(x.y, p, NB));
+      if (i > p) x = csub(x, cmul(sT[i][p], xp));
     }
   } else {
-#pragma unroll
-    for (int i = NB - 1; i >= 0; --i) {
-      if (i < nbc) {
-        double2 s = x[i];
-#pragma unroll
-        for (int p = i + 1; p < NB; ++p)
-          if (p < nbc) s = csub(s, cmul(sT[i][p], x[p]));
-        x[i] = cdiv(s, sT[i][i]);
-      }
+#pragma unroll 1
+    for (int p = nbc - 1; p >= 0; --p) {
+      if (i == p) x = cdiv(x, sT[p][p]);
+      const double2 xp = make_double2(__shfl(x.x, p, NB), __shfl(x.y, p, NB));
+      if (i < p) x = csub(x, cmul(sT[i][p], xp));
     }
   }
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    if (i < nbc) X[(int64_t)i * ldx + j] = x[i];
+  if (live) X[(int64_t)i * ldx + j] = x;
 }
 
 }  // namespace
@@ -439,8 +435,8 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
       double2* A12 = A11 + nbc;
       double2* A21 = A11 + (int64_t)nbc * lda;
       double2* A22 = A21 + nbc;
-      hipLaunchKernelGGL(trsm_kernel<false>, dim3((rest + 255) / 256, batch), dim3(256), 0, h->stream, nbc, A11,
-                         lda, sa, A12, lda, sa, rest);
+      hipLaunchKernelGGL(trsm_kernel<false>, dim3((rest + TRSM_COLS - 1) / TRSM_COLS, batch), dim3(256), 0,
+                         h->stream, nbc, A11, lda, sa, A12, lda, sa, rest);
       EUVL_LAUNCH_CHECK();
       EUVL_TRY(zgemm(h, rest, rest, nbc, mone, A21, lda, sa, A12, lda, sa, one, A22, lda, sa, zero, batch));
     }
@@ -465,7 +461,7 @@ int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int*
   EUVL_LAUNCH_CHECK();
   for (int kb = 0; kb < n; kb += NB) {
     const int nbc = (n - kb < NB) ? n - kb : NB;
-    hipLaunchKernelGGL(diag_lu_kernel, dim3(batch), dim3(64), 0, h->stream, A, lda, sa, kb, nbc, uinv, linv, ipiv,
+    hipLaunchKernelGGL(diag_lu_kernel, dim3(batch), dim3(DLU_THREADS), 0, h->stream, A, lda, sa, kb, nbc, uinv, linv, ipiv,
                        n, d_info);
     EUVL_LAUNCH_CHECK();
     const int rest = n - kb - nbc;
@@ -496,8 +492,8 @@ int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64
     const int nbc = (n - kb < NB) ? n - kb : NB;
     const double2* L11 = LU + (int64_t)kb * lda + kb;
     double2* Bk = B + (int64_t)kb * ldb;
-    hipLaunchKernelGGL(trsm_kernel<false>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, h->stream, nbc, L11,
-                       lda, sa, Bk, ldb, sb, nrhs);
+    hipLaunchKernelGGL(trsm_kernel<false>, dim3((nrhs + TRSM_COLS - 1) / TRSM_COLS, batch), dim3(256), 0, h->stream,
+                       nbc, L11, lda, sa, Bk, ldb, sb, nrhs);
     EUVL_LAUNCH_CHECK();
     const int rest = n - kb - nbc;
     if (rest > 0)
@@ -509,8 +505,8 @@ int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64
     const int nbc = (n - kb < NB) ? n - kb : NB;
     const double2* U11 = LU + (int64_t)kb * lda + kb;
     double2* Bk = B + (int64_t)kb * ldb;
-    hipLaunchKernelGGL(trsm_kernel<true>, dim3((nrhs + 255) / 256, batch), dim3(256), 0, h->stream, nbc, U11,
-                       lda, sa, Bk, ldb, sb, nrhs);
+    hipLaunchKernelGGL(trsm_kernel<true>, dim3((nrhs + TRSM_COLS - 1) / TRSM_COLS, batch), dim3(256), 0, h->stream,
+                       nbc, U11, lda, sa, Bk, ldb, sb, nrhs);
     EUVL_LAUNCH_CHECK();
     if (kb > 0)
       EUVL_TRY(zgemm(h, kb, nrhs, nbc, mone, LU + kb, lda, sa, Bk, ldb, sb, one, B, ldb, sb, zero, batch));

@@ -91,7 +91,7 @@ def test_getrf_getrs(lib, handle, n, nrhs, batch):
     assert np.abs(X - ref).max() < 1e-10 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2), (1681, 12, 2)])
+@pytest.mark.parametrize("n,nrhs,batch", [(5, 3, 1), (100, 7, 2), (257, 40, 2), (1681, 12, 2), (300, 171, 2)])
 def test_getrf_nopiv_getrs(lib, handle, n, nrhs, batch):
     """Unpivoted LU on well-conditioned, not diagonally dominant matrices
     (identity plus a random part of norm ~0.5, like the top-solve operator)."""
