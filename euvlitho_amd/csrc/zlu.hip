@@ -238,82 +238,137 @@ __global__ __launch_bounds__(256) void tslu_apply_kernel(double2* __restrict__ A
 
 // ---------------------------------------------------------------------------
 // LU without row interchanges.  Each 32-wide panel starts with its 32 x 32
-// diagonal block, factored in LDS by one workgroup per batch entry, which also
+// diagonal block, factored by one workgroup per batch entry, which also
 // forms U11^-1 and L11^-1; the panel's column and row blocks then become
 // GEMMs (L21 = A21 U11^-1, U12 = L11^-1 A12), as does the trailing update.
 // No column step of the factorisation runs over the panel's full height, so
 // a panel costs a few short launches instead of 32 dependent pivot searches.
-// The block is small and factored once per launch, so the kernel is written
-// as short loops over LDS (two waves): a fully unrolled register form runs
-// to ~20k instructions and is bound by instruction fetch.
-constexpr int DLU_THREADS = 128;
+// The block is small and factored once per launch, so the kernel keeps it
+// in registers of 256 threads (thread (r, c) holds rows r, r+8, r+16, r+24 of
+// column c) with short loops: a fully unrolled single-lane-per-row form runs to
+// ~20k instructions and is bound by instruction fetch.  The inverses are
+// row-parallel substitutions (lane i of a 32-lane group holds row i of eight
+// columns, the finished row is broadcast at each step).
+constexpr int DLU_THREADS = 256;
 __global__ __launch_bounds__(DLU_THREADS) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa,
                                                               int kb, int nbc, double2* __restrict__ uinv,
                                                               double2* __restrict__ linv, int* __restrict__ ipiv,
                                                               int n, int* __restrict__ info) {
+  constexpr int RG = DLU_THREADS / NB;  // row groups (8)
+  constexpr int RS = NB / RG;           // rows per thread (4)
   __shared__ double2 a[NB][NB + 1];
-  __shared__ double2 ys[2][NB][NB + 1];  // ys[0][i][c] = U11^-1(i, c), ys[1][i][c] = L11^-1(i, c)
+  // step j publishes row j and column j into buffers of parity j & 1, so one
+  // barrier per step orders them (step j+1 writes the other buffers)
+  __shared__ double2 rowb[2][NB], colb[2][NB];
   const int b = blockIdx.x, t = threadIdx.x;
-  double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
-  for (int e = t; e < NB * NB; e += DLU_THREADS) {
-    const int r = e / NB, q = e % NB;
-    const bool in = r < nbc && q < nbc;
-    const double pad = (r == q) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
-    a[r][q] = in ? A11[(int64_t)r * lda + q] : make_double2(pad, 0.0);
-  }
-  __syncthreads();
-  // right-looking elimination: multipliers of column j, then the rank-1
-  // update of the trailing block (4 rows x 32 columns of threads)
   const int tq = t & (NB - 1), tr = t / NB;
+  double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
+  double2 x[RS];
+#pragma unroll
+  for (int u = 0; u < RS; ++u) {
+    const int r = tr + RG * u;
+    const bool in = r < nbc && tq < nbc;
+    const double pad = (r == tq) ? 1.0 : 0.0;  // identity outside the nbc x nbc block
+    x[u] = in ? A11[(int64_t)r * lda + tq] : make_double2(pad, 0.0);
+  }
+  // right-looking elimination: row j and column j are final once step j-1 is
+  // done; every thread forms the multipliers of its own rows
 #pragma unroll 1
   for (int j = 0; j < NB - 1; ++j) {
-    const double2 d = a[j][j];
-    const bool nz = d.x != 0.0 || d.y != 0.0;
-    if (nz && t > j && t < NB) a[t][j] = cdiv(a[t][j], d);
+    const int pb = j & 1;
+#pragma unroll
+    for (int u = 0; u < RS; ++u) {
+      const int i = tr + RG * u;
+      if (i == j) rowb[pb][tq] = x[u];
+      if (tq == j && i >= j) colb[pb][i] = x[u];
+    }
     __syncthreads();
-    if (nz && tq > j)
-#pragma unroll 1
-      for (int i = j + 1 + tr; i < NB; i += DLU_THREADS / NB) a[i][tq] = csub(a[i][tq], cmul(a[i][j], a[j][tq]));
-    __syncthreads();
+    // the same pivot for every thread: a uniform reciprocal, and updates as
+    // selects (no divergent branches, so the LDS reads are issued together)
+    const double2 d = rowb[pb][j];
+    if (d.x != 0.0 || d.y != 0.0) {  // a zero pivot leaves the block as it is
+      const double2 rd = cdiv(make_double2(1.0, 0.0), d);
+      const double2 uj = rowb[pb][tq];
+      double2 c[RS];
+#pragma unroll
+      for (int u = 0; u < RS; ++u) c[u] = colb[pb][min(tr + RG * u, NB - 1)];
+#pragma unroll
+      for (int u = 0; u < RS; ++u) {
+        const int i = tr + RG * u;
+        const double2 l = cmul(c[u], rd);
+        const double2 upd = (tq == j) ? l : csub(x[u], cmul(l, uj));
+        const bool w = i > j && tq >= j;  // selects per component keep x in registers
+        x[u].x = w ? upd.x : x[u].x;
+        x[u].y = w ? upd.y : x[u].y;
+      }
+    }
   }
-  for (int e = t; e < NB * NB; e += DLU_THREADS) {
-    const int r = e / NB, q = e % NB;
-    if (r < nbc && q < nbc) A11[(int64_t)r * lda + q] = a[r][q];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RS; ++u) a[tr + RG * u][tq] = x[u];
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < RS; ++u) {
+    const int r = tr + RG * u;
+    if (r < nbc && tq < nbc) A11[(int64_t)r * lda + tq] = x[u];
   }
   if (t < nbc) ipiv[(int64_t)b * n + kb + t] = kb + t;
   if (t == 0)
 #pragma unroll 1
     for (int j = 0; j < nbc; ++j)
       if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
-  // column c of U11^-1 (wave 0): y(i) for i = NB-1 .. 0, y(i) = (e_c(i) - sum_{k>i} U(i,k) y(k)) / U(i,i);
-  // column c of L11^-1 (wave 1): y(i) for i = 0 .. NB-1, y(i) = e_c(i) - sum_{k<i} L(i,k) y(k);
-  // each lane reads back only its own column of ys
-  const int w = t / 64, c = t & 63;
-  if (c < NB) {
-    if (w == 0) {
+  // U11^-1 (row groups 0-3) and L11^-1 (4-7): lane i holds row i of CB columns
+  constexpr int CB = NB / (RG / 2);  // columns per group (8)
+  const int i = tq, g = tr & (RG / 2 - 1);
+  double2 y[CB];
+#pragma unroll
+  for (int v = 0; v < CB; ++v) y[v] = make_double2(i == g * CB + v ? 1.0 : 0.0, 0.0);
+  if (tr < RG / 2) {
+    // back substitution: y(p) *= 1/U(p,p), then rows i < p subtract U(i,p) y(p)
+    const double2 rii = cdiv(make_double2(1.0, 0.0), a[i][i]);
 #pragma unroll 1
-      for (int i = NB - 1; i >= 0; --i) {
-        double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
-#pragma unroll 2
-        for (int k = i + 1; k < NB; ++k) s = csub(s, cmul(a[i][k], ys[0][k][c]));
-        const double2 u = cdiv(s, a[i][i]);
-        ys[0][i][c] = make_double2(i <= c ? u.x : 0.0, i <= c ? u.y : 0.0);
+    for (int p = NB - 1; p >= 0; --p) {
+      const double2 uip = a[i][p];
+      double2 yp[CB];
+#pragma unroll
+      for (int v = 0; v < CB; ++v) {
+        const double2 sc = cmul(y[v], rii);
+        y[v].x = (i == p) ? sc.x : y[v].x;
+        y[v].y = (i == p) ? sc.y : y[v].y;
+        yp[v] = make_double2(__shfl(y[v].x, p, NB), __shfl(y[v].y, p, NB));
       }
-    } else {
-#pragma unroll 1
-      for (int i = 0; i < NB; ++i) {
-        double2 s = make_double2(i == c ? 1.0 : 0.0, 0.0);
-#pragma unroll 2
-        for (int k = 0; k < i; ++k) s = csub(s, cmul(a[i][k], ys[1][k][c]));
-        ys[1][i][c] = make_double2(i >= c ? s.x : 0.0, i >= c ? s.y : 0.0);
+#pragma unroll
+      for (int v = 0; v < CB; ++v) {
+        const double2 upd = csub(y[v], cmul(uip, yp[v]));
+        y[v].x = (i < p) ? upd.x : y[v].x;
+        y[v].y = (i < p) ? upd.y : y[v].y;
       }
     }
-  }
-  __syncthreads();
-  for (int e = t; e < NB * NB; e += DLU_THREADS) {
-    const int i = e / NB, cc = e % NB;
-    uinv[((int64_t)b * NB + i) * NB + cc] = ys[0][i][cc];
-    linv[((int64_t)b * NB + i) * NB + cc] = ys[1][i][cc];
+#pragma unroll
+    for (int v = 0; v < CB; ++v) {
+      const int c = g * CB + v;
+      uinv[((int64_t)b * NB + i) * NB + c] = make_double2(i <= c ? y[v].x : 0.0, i <= c ? y[v].y : 0.0);
+    }
+  } else {
+    // forward substitution with the unit lower factor
+#pragma unroll 1
+    for (int p = 0; p < NB - 1; ++p) {
+      const double2 lip = a[i][p];
+      double2 yp[CB];
+#pragma unroll
+      for (int v = 0; v < CB; ++v) yp[v] = make_double2(__shfl(y[v].x, p, NB), __shfl(y[v].y, p, NB));
+#pragma unroll
+      for (int v = 0; v < CB; ++v) {
+        const double2 upd = csub(y[v], cmul(lip, yp[v]));
+        y[v].x = (i > p) ? upd.x : y[v].x;
+        y[v].y = (i > p) ? upd.y : y[v].y;
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < CB; ++v) {
+      const int c = g * CB + v;
+      linv[((int64_t)b * NB + i) * NB + c] = make_double2(i >= c ? y[v].x : 0.0, i >= c ? y[v].y : 0.0);
+    }
   }
 }
 
