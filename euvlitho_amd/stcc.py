@@ -127,11 +127,18 @@ def zero_order_amplitude(cfg: StccConfig = StccConfig(), device="cuda:0"):
 
 
 def _bits_of(mask, ndiv):
-    """The package's mask array M[x][y'] -> mask.bin bytes (pixel (x, ndiv-1-y'))."""
+    """The package's mask array M[x][y'] -> mask.bin bytes (pixel (x, ndiv-1-y')):
+    mask.bin row ndiv-1-y' packs M[x][y'] over x, so the bits are packed down
+    the first axis (eight x rows at a time, contiguous) and only the packed
+    array, an eighth of the size, is transposed."""
     m = np.asarray(mask)
-    if m.shape != (ndiv, ndiv):
-        raise ValueError(f"mask must be {ndiv} x {ndiv}")
-    return np.packbits(np.flipud(m.T).astype(np.uint8).reshape(-1))
+    if m.shape != (ndiv, ndiv) or ndiv % 8:
+        raise ValueError(f"mask must be {ndiv} x {ndiv} with ndiv a multiple of 8")
+    u = (m != 0).view(np.uint8).reshape(ndiv // 8, 8, ndiv)
+    packed = np.zeros((ndiv // 8, ndiv), dtype=np.uint8)
+    for bit in range(8):
+        packed |= u[:, bit] << np.uint8(7 - bit)
+    return np.ascontiguousarray(packed[:, ::-1].T).reshape(-1)
 
 
 def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig = StccConfig(), amplitudes=None,
@@ -150,38 +157,36 @@ def intensity(mask, polar="X", a0xx=None, axxx=None, ayxx=None, cfg: StccConfig 
     dev = torch.device(device)
     bits = torch.from_numpy(_bits_of(mask, cfg.ndiv)).to(dev)
     fm = torch.empty((2 * lpx + 1, 2 * lpy + 1), dtype=torch.complex128, device=dev)
-    own = handle is None
-    lib = _lib.load()
-    h = ctypes.c_void_p() if own else handle
-    if own:
-        _lib.check(lib.euvl_create(ctypes.byref(h)), "euvl_create")
-    try:
+    if handle is None:
+        # the per-thread handle of this package: its device workspace (the
+        # three TCCs, ~170 MB at the reference size) is kept between calls
+        from .cnn import _handle
+        lib, h = _handle(dev)
+    else:
+        lib, h = _lib.load(), handle
         _lib.check(lib.euvl_set_stream(h, ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)),
                    "euvl_set_stream")
-        if timing is not None:
-            ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
-            ev[0].record()
-        _lib.check(lib.euvl_mask_spectrum(h, _lib.ptr(bits), cfg.ndiv, cfg.ndiv, 1, lpx, lpy, _lib.ptr(fm)),
-                   "euvl_mask_spectrum")
-        m3d = [None if a is None else torch.as_tensor(np.asarray(a, dtype=np.complex128)).to(dev).contiguous()
-               for a in (a0xx, axxx, ayxx)]
-        if any(a is not None and tuple(a.shape) != tuple(fm.shape) for a in m3d):
-            raise ValueError(f"M3D arrays must be {tuple(fm.shape)}")
-        pupil = np.ascontiguousarray(cfg.pupil())
-        src = np.ascontiguousarray(cfg.source())
-        prm = _lib.StccParams(cfg.k, cfg.kx0, cfg.ky0, cfg.z0, float(cfg.ndiv), float(cfg.ndiv), cfg.na, cfg.mx,
-                              cfg.my, lpx, lpy, cfg.xdiv)
-        img = torch.empty((cfg.xdiv, cfg.xdiv), dtype=torch.float64, device=dev)
-        cplx = lambda z: (ctypes.c_double * 2)(complex(z).real, complex(z).imag)
-        p = lambda t: _lib.ptr(t) if t is not None else None
-        _lib.check(lib.euvl_stcc_image(h, polar.encode(), ctypes.byref(prm), _lib.ptr(fm), cplx(ab), cplx(vc),
-                                       cplx(ph), p(m3d[0]), p(m3d[1]), p(m3d[2]), len(pupil), pupil.ctypes.data,
-                                       len(src), src.ctypes.data, _lib.ptr(img)), "euvl_stcc_image")
-        if timing is not None:
-            ev[1].record()
-            ev[1].synchronize()
-            timing["device_ms"] = ev[0].elapsed_time(ev[1])
-        return img.cpu().numpy()
-    finally:
-        if own:
-            lib.euvl_destroy(h)
+    if timing is not None:
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        ev[0].record()
+    _lib.check(lib.euvl_mask_spectrum(h, _lib.ptr(bits), cfg.ndiv, cfg.ndiv, 1, lpx, lpy, _lib.ptr(fm)),
+               "euvl_mask_spectrum")
+    m3d = [None if a is None else torch.as_tensor(np.asarray(a, dtype=np.complex128)).to(dev).contiguous()
+           for a in (a0xx, axxx, ayxx)]
+    if any(a is not None and tuple(a.shape) != tuple(fm.shape) for a in m3d):
+        raise ValueError(f"M3D arrays must be {tuple(fm.shape)}")
+    pupil = np.ascontiguousarray(cfg.pupil())
+    src = np.ascontiguousarray(cfg.source())
+    prm = _lib.StccParams(cfg.k, cfg.kx0, cfg.ky0, cfg.z0, float(cfg.ndiv), float(cfg.ndiv), cfg.na, cfg.mx,
+                          cfg.my, lpx, lpy, cfg.xdiv)
+    img = torch.empty((cfg.xdiv, cfg.xdiv), dtype=torch.float64, device=dev)
+    cplx = lambda z: (ctypes.c_double * 2)(complex(z).real, complex(z).imag)
+    p = lambda t: _lib.ptr(t) if t is not None else None
+    _lib.check(lib.euvl_stcc_image(h, polar.encode(), ctypes.byref(prm), _lib.ptr(fm), cplx(ab), cplx(vc),
+                                   cplx(ph), p(m3d[0]), p(m3d[1]), p(m3d[2]), len(pupil), pupil.ctypes.data,
+                                   len(src), src.ctypes.data, _lib.ptr(img)), "euvl_stcc_image")
+    if timing is not None:
+        ev[1].record()
+        ev[1].synchronize()
+        timing["device_ms"] = ev[0].elapsed_time(ev[1])
+    return img.cpu().numpy()
