@@ -16,6 +16,7 @@
 //   R = C diag(alru (uu-ub)) - i G2 W1,  U0 = diag(klm)(C W1 - i F diag(alru (uu-ub))) + R
 // so neither the eigenvectors, nor S, nor a square-root branch is needed.
 #include <cmath>
+#include <algorithm>
 #include <complex>
 #include <cstdlib>
 #include <cstring>
@@ -36,43 +37,84 @@ __device__ inline int tw_index(int64_t i, int l, int N) {
   return N - (int)((-i * l) % N);
 }
 
-// T[c][i][mm] = sum_j m(i, j) tw_y[idx(j, m)], m = mm - mw; y row j is bitmap row NDIVY-1-j
-__global__ void mask_dft_y_kernel(const uint8_t* bits, int nx, int ny, int mw, const double2* twy, double2* T,
-                                  int n_clips) {
+// T[c][i][mm] = sum_j m(i, j) tw_y[idx(j, m)], m = mm - mw; y row j is bitmap row NDIVY-1-j.
+// A real-by-complex product (bits 0/1 times the twiddle table) on FP64 MFMAs:
+// a workgroup of 4 waves covers 64 x rows and 64 orders over one of S splits
+// of the y range; lane l feeds bit (i0 + l%16, j0 + l/16) as A and the
+// table entry of (j0 + l/16, order l%16) as B (re and im accumulated
+// separately).  Split s writes partial sums P[s][c][i][mm], summed in split
+// order by mask_dft_reduce_kernel (deterministic).
+constexpr int DFT_TILE = 64;
+__global__ __launch_bounds__(256) void mask_dft_y_kernel(const uint8_t* __restrict__ bits, int nx, int ny, int mw,
+                                                         const double2* __restrict__ twy, double2* __restrict__ P,
+                                                         int n_clips, int S, int ks) {
   const int nm = 2 * mw + 1;
-  const int64_t total = (int64_t)n_clips * nx * nm;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e / ((int64_t)nx * nm));
-    const int64_t r = e - (int64_t)c * nx * nm;
-    const int mm = (int)(r % nm);
-    const int i = (int)(r / nm);
-    const int m = mm - mw;
-    const uint8_t* b = bits + (int64_t)c * nx * ny / 8;
-    double2 s = make_double2(0.0, 0.0);
-    for (int j = 0; j < ny; ++j) {
-      if (mask_bit(b, (int64_t)nx * (ny - 1 - j) + i)) s = cadd(s, twy[tw_index(j, m, ny)]);
+  const int s = blockIdx.z % S, c = blockIdx.z / S;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int i = blockIdx.x * DFT_TILE + w * 16 + li;
+  const int m0 = blockIdx.y * DFT_TILE;
+  const uint8_t* b = bits + (int64_t)c * nx * ny / 8;
+  double4_t re[4], im[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    re[t] = (double4_t){0, 0, 0, 0};
+    im[t] = (double4_t){0, 0, 0, 0};
+  }
+  const int j0 = s * ks, j1 = min(ny, j0 + ks);
+  for (int k0 = j0; k0 < j1; k0 += 4) {
+    const int j = k0 + lk;
+    const bool jin = j < j1;
+    const double a = (jin && i < nx) ? (double)mask_bit(b, (int64_t)nx * (ny - 1 - j) + i) : 0.0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int mm = m0 + t * 16 + li;
+      const double2 tw = (jin && mm < nm) ? twy[tw_index(j, mm - mw, ny)] : make_double2(0.0, 0.0);
+      re[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tw.x, re[t], 0, 0, 0);
+      im[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, tw.y, im[t], 0, 0, 0);
     }
-    T[e] = s;  // layout [c][i][mm]
+  }
+  // accumulator element r of lane l: row 4r + l/16 (x), column l%16 (order)
+  double2* out = P + ((int64_t)s * n_clips + c) * nx * nm;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ii = blockIdx.x * DFT_TILE + w * 16 + 4 * r + lk, mm = m0 + t * 16 + li;
+      if (ii < nx && mm < nm) out[(int64_t)ii * nm + mm] = make_double2(re[t][r], im[t][r]);
+    }
+}
+
+__global__ void mask_dft_reduce_kernel(const double2* __restrict__ P, int S, int64_t per, double2* __restrict__ T) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = P[e];
+    for (int s = 1; s < S; ++s) acc = cadd(acc, P[(int64_t)s * per + e]);
+    T[e] = acc;
   }
 }
 
-// out[c][l][mm] = (1/(nx ny)) sum_i T[c][i][mm] tw_x[idx(i, l)]
-__global__ void mask_dft_x_kernel(const double2* T, int nx, int ny, int lw, int mw, const double2* twx, double2* out,
-                                  int n_clips) {
+// out[c][l][mm] = (1/(nx ny)) sum_i T[c][i][mm] tw_x[idx(i, l)]: one
+// workgroup per output, a strided sum per thread and a tree in LDS
+__global__ __launch_bounds__(256) void mask_dft_x_kernel(const double2* __restrict__ T, int nx, int ny, int lw,
+                                                         int mw, const double2* __restrict__ twx,
+                                                         double2* __restrict__ out, int n_clips) {
   const int nm = 2 * mw + 1, nl = 2 * lw + 1;
-  const int64_t total = (int64_t)n_clips * nl * nm;
-  const double inv = 1.0 / ((double)nx * (double)ny);
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int c = (int)(e / ((int64_t)nl * nm));
-    const int64_t r = e - (int64_t)c * nl * nm;
-    const int mm = (int)(r % nm);
-    const int l = (int)(r / nm) - lw;
-    const double2* t = T + (int64_t)c * nx * nm + mm;
-    double2 s = make_double2(0.0, 0.0);
-    for (int i = 0; i < nx; ++i) s = cadd(s, cmul(t[(int64_t)i * nm], twx[tw_index(i, l, nx)]));
-    // the reference divides by FDIVY after the y sum and by FDIVX after the x sum
-    out[e] = cscale(s, inv);
+  const int64_t e = blockIdx.x;
+  const int c = (int)(e / ((int64_t)nl * nm));
+  const int64_t r = e - (int64_t)c * nl * nm;
+  const int mm = (int)(r % nm);
+  const int l = (int)(r / nm) - lw;
+  const double2* t = T + (int64_t)c * nx * nm + mm;
+  double2 s = make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < nx; i += blockDim.x) s = cadd(s, cmul(t[(int64_t)i * nm], twx[tw_index(i, l, nx)]));
+  __shared__ double2 red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = cadd(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
   }
+  // the reference divides by FDIVY after the y sum and by FDIVX after the x sum
+  if (threadIdx.x == 0) out[e] = cscale(red[0], 1.0 / ((double)nx * (double)ny));
 }
 
 struct ModeVecs {
@@ -254,19 +296,36 @@ int fill(euvl_handle h, int64_t count, double2* out, double2 v) {
   return EUVL_OK;
 }
 
+// splits of the y range of mask_dft_y_kernel: enough workgroups to fill the GPU
+int dft_splits(int nx, int ny, int mw, int n_clips) {
+  const int64_t tiles = (int64_t)((nx + DFT_TILE - 1) / DFT_TILE) * ((2 * mw + 1 + DFT_TILE - 1) / DFT_TILE) * n_clips;
+  int S = (int)std::min<int64_t>(16, std::max<int64_t>(1, 1024 / tiles));
+  while (S > 1 && (int64_t)S * 64 > ny) --S;  // at least 64 rows per split
+  return S;
+}
+
 int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw, double2* out,
-                  double2* scratch /* n_clips*nx*(2mw+1) */, double2* tw /* nx+1 + ny+1 */) {
+                  double2* scratch /* n_clips*nx*(2mw+1) */, double2* part /* S*n_clips*nx*(2mw+1), S > 1 */,
+                  double2* tw /* nx+1 + ny+1 */) {
   std::vector<std::complex<double>> t(nx + ny + 2);
   const std::complex<double> zi(0.0, 1.0);
   for (int i = 0; i <= nx; ++i) t[i] = std::exp(-2.0 * PI * zi * double(i) / double(nx));       // exponential()
   for (int i = 0; i <= ny; ++i) t[nx + 1 + i] = std::exp(-2.0 * PI * zi * double(i) / double(ny));
   EUVL_HIP_CHECK(hipMemcpyAsync(tw, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice, h->stream));
-  const int64_t n1 = (int64_t)n_clips * nx * (2 * mw + 1);
-  hipLaunchKernelGGL(mask_dft_y_kernel, grid1(n1), dim3(256), 0, h->stream, bits, nx, ny, mw, tw + nx + 1, scratch,
-                     n_clips);
+  const int nm = 2 * mw + 1;
+  const int S = dft_splits(nx, ny, mw, n_clips);
+  const int ks = ((ny + S - 1) / S + 3) / 4 * 4;
+  const dim3 g1((nx + DFT_TILE - 1) / DFT_TILE, (nm + DFT_TILE - 1) / DFT_TILE, S * n_clips);
+  hipLaunchKernelGGL(mask_dft_y_kernel, g1, dim3(256), 0, h->stream, bits, nx, ny, mw, tw + nx + 1,
+                     S > 1 ? part : scratch, n_clips, S, ks);
   EUVL_LAUNCH_CHECK();
-  const int64_t n2 = (int64_t)n_clips * (2 * lw + 1) * (2 * mw + 1);
-  hipLaunchKernelGGL(mask_dft_x_kernel, grid1(n2), dim3(256), 0, h->stream, scratch, nx, ny, lw, mw, tw, out,
+  if (S > 1) {
+    const int64_t per = (int64_t)n_clips * nx * nm;
+    hipLaunchKernelGGL(mask_dft_reduce_kernel, grid1(per), dim3(256), 0, h->stream, part, S, per, scratch);
+    EUVL_LAUNCH_CHECK();
+  }
+  const int64_t n2 = (int64_t)n_clips * (2 * lw + 1) * nm;
+  hipLaunchKernelGGL(mask_dft_x_kernel, dim3((unsigned)n2), dim3(256), 0, h->stream, scratch, nx, ny, lw, mw, tw, out,
                      n_clips);
   EUVL_LAUNCH_CHECK();
   // the host table must outlive the copy
@@ -277,17 +336,20 @@ int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clip
 
 extern "C" int euvl_mask_spectrum(euvl_handle h, const uint8_t* d_bits, int ndivx, int ndivy, int n_clips, int lw,
                                   int mw, double* d_out) {
-  if (!h || !d_bits || !d_out || ndivx <= 0 || ndivy <= 0 || n_clips <= 0 || lw < 0 || mw < 0 ||
-      ((int64_t)ndivx * ndivy) % 8 != 0) {
+  if (!h || !d_bits || !d_out || ndivx <= 0 || ndivy <= 0 || n_clips <= 0 || n_clips > 65535 || lw < 0 || mw < 0 ||
+      ((int64_t)ndivx * ndivy) % 8 != 0 || (int64_t)n_clips * (2 * lw + 1) * (2 * mw + 1) >= ((int64_t)1 << 31)) {
     euvl::set_error("euvl_mask_spectrum: bad argument");
     return EUVL_ERR_ARG;
   }
   const size_t scratch = sizeof(double2) * (size_t)n_clips * ndivx * (2 * mw + 1);
+  const int S = euvl::dft_splits(ndivx, ndivy, mw, n_clips);
+  const size_t part = S > 1 ? S * scratch : 0;
   const size_t tw = sizeof(double2) * (size_t)(ndivx + ndivy + 2);
-  EUVL_TRY(euvl::ws_reserve(h, scratch + tw + 1024));
+  EUVL_TRY(euvl::ws_reserve(h, scratch + part + tw + 1024));
   double2* s = (double2*)euvl::ws_take(h, scratch);
+  double2* pp = part ? (double2*)euvl::ws_take(h, part) : nullptr;
   double2* t = (double2*)euvl::ws_take(h, tw);
-  return euvl::mask_spectrum(h, d_bits, ndivx, ndivy, n_clips, lw, mw, (double2*)d_out, s, t);
+  return euvl::mask_spectrum(h, d_bits, ndivx, ndivy, n_clips, lw, mw, (double2*)d_out, s, pp, t);
 }
 
 extern "C" int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n, const double* d_kx,

@@ -164,28 +164,41 @@ __device__ inline double2 twiddle(long long n, int X) {
   return make_double2(c, s);
 }
 
+// twiddle tables: tw_x[il][x] = e^{2 pi i (il - 2 lpx) x / X} (wl rows), then
+// tw_y[im][y] = e^{2 pi i (im - 2 lpy) y / X} (wm rows): each factor once,
+// not once per image pixel
+__global__ void stcc_twiddle_kernel(int lpx, int lpy, int X, double2* __restrict__ tw) {
+  const int wl = 4 * lpx + 1, wm = 4 * lpy + 1;
+  const int64_t total = (int64_t)(wl + wm) * X;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(e / X), c = (int)(e % X);
+    tw[e] = (r < wl) ? twiddle((long long)(r - 2 * lpx) * c, X) : twiddle((long long)(r - wl - 2 * lpy) * c, X);
+  }
+}
+
 // G[dl][y] = sum_dm T[dl][dm] e^{2 pi i dm y / X}
-__global__ void stcc_image_y_kernel(int lpx, int lpy, int X, const double2* __restrict__ T, double2* __restrict__ G) {
+__global__ void stcc_image_y_kernel(int lpx, int lpy, int X, const double2* __restrict__ T,
+                                    const double2* __restrict__ tw_y, double2* __restrict__ G) {
   const int wl = 4 * lpx + 1, wm = 4 * lpy + 1;
   const int64_t total = (int64_t)wl * X;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int il = (int)(e / X), y = (int)(e % X);
     double2 acc = make_double2(0.0, 0.0);
-    for (int im = 0; im < wm; ++im)
-      acc = cadd(acc, cmul(T[(int64_t)il * wm + im], twiddle((long long)(im - 2 * lpy) * y, X)));
+    for (int im = 0; im < wm; ++im) acc = cadd(acc, cmul(T[(int64_t)il * wm + im], tw_y[(int64_t)im * X + y]));
     G[e] = acc;
   }
 }
 
 // img[x][y] = Re sum_dl G[dl][y] e^{2 pi i dl x / X}
-__global__ void stcc_image_x_kernel(int lpx, int X, const double2* __restrict__ G, double* __restrict__ img) {
+__global__ void stcc_image_x_kernel(int lpx, int X, const double2* __restrict__ G, const double2* __restrict__ tw_x,
+                                    double* __restrict__ img) {
   const int wl = 4 * lpx + 1;
   const int64_t total = (int64_t)X * X;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int x = (int)(e / X), y = (int)(e % X);
     double acc = 0.0;
     for (int il = 0; il < wl; ++il) {
-      const double2 g = G[(int64_t)il * X + y], t = twiddle((long long)(il - 2 * lpx) * x, X);
+      const double2 g = G[(int64_t)il * X + y], t = tw_x[(int64_t)il * X + x];
       acc += g.x * t.x - g.y * t.y;
     }
     img[e] = acc;
@@ -231,7 +244,8 @@ extern "C" int euvl_stcc_image(euvl_handle h, char polar, const euvl_stcc_params
   const size_t bE = sizeof(double2) * 12 * ncut, bP = sizeof(double2) * (size_t)ncut * n_src;
   const size_t bT = sizeof(double2) * 3 * (size_t)ncut * ncut, bW = sizeof(double2) * (size_t)wl * wm;
   const size_t bG = sizeof(double2) * (size_t)wl * p->xdiv;
-  EUVL_TRY(euvl::ws_reserve(h, up.host.size() + bE + 4 * bP + bT + bW + bG + 8 * 256));
+  const size_t bTw = sizeof(double2) * (size_t)(wl + wm) * p->xdiv;
+  EUVL_TRY(euvl::ws_reserve(h, up.host.size() + bE + 4 * bP + bT + bW + bG + bTw + 9 * 256));
   char* blk = (char*)euvl::ws_take(h, up.host.size());
   double2* E = (double2*)euvl::ws_take(h, bE);
   double2* P = (double2*)euvl::ws_take(h, bP);
@@ -239,6 +253,7 @@ extern "C" int euvl_stcc_image(euvl_handle h, char polar, const euvl_stcc_params
   double2* T = (double2*)euvl::ws_take(h, bT);
   double2* TW = (double2*)euvl::ws_take(h, bW);
   double2* G = (double2*)euvl::ws_take(h, bG);
+  double2* TWID = (double2*)euvl::ws_take(h, bTw);
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
   const int* d_pup = (const int*)(blk + o_pup);
   hipLaunchKernelGGL(stcc_fields_kernel, dim3((ncut + 127) / 128), dim3(128), 0, h->stream, a, ncut, d_pup,
@@ -257,11 +272,14 @@ extern "C" int euvl_stcc_image(euvl_handle h, char polar, const euvl_stcc_params
   hipLaunchKernelGGL(stcc_tccee_kernel, dim3(wl * wm), dim3(256), 0, h->stream, ncut, lpx, lpy, d_pup,
                      (const int*)(blk + o_idx), T, E, TW);
   EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(stcc_twiddle_kernel, grid1((int64_t)(wl + wm) * p->xdiv), dim3(256), 0, h->stream, lpx, lpy,
+                     p->xdiv, TWID);
+  EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(stcc_image_y_kernel, grid1((int64_t)wl * p->xdiv), dim3(256), 0, h->stream, lpx, lpy, p->xdiv,
-                     TW, G);
+                     TW, TWID + (size_t)wl * p->xdiv, G);
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(stcc_image_x_kernel, grid1((int64_t)p->xdiv * p->xdiv), dim3(256), 0, h->stream, lpx, p->xdiv,
-                     G, d_img);
+                     G, TWID, d_img);
   EUVL_LAUNCH_CHECK();
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;

@@ -165,17 +165,20 @@ def oracle_problem(cfg):
     return p, G.derived(p)
 
 
-def test_mask_spectrum_vs_oracle(lib, handle):
+@pytest.mark.parametrize("nx,ny,L,M,nc", [(256, 128, 6, 4, 2), (200, 136, 7, 9, 3), (1024, 1024, 20, 20, 1)])
+def test_mask_spectrum_vs_oracle(lib, handle, nx, ny, L, M, nc):
+    """The mask spectrum (y sums on MFMAs over splits of the y range, then the
+    x sums) against the oracle's maskamp, including sizes that are not
+    multiples of the 64 x 64 tile and several splits."""
     from euvlitho_amd.masks import manhattan, pack_bits
     from oracle import em as OE
-    say("mask spectrum 256x128")
-    nx, ny, L, M = 256, 128, 6, 4
-    masks = [manhattan(nx, ny, 0.5, s) for s in range(2)]
+    say(f"mask spectrum {nx}x{ny} L={L} M={M} clips={nc}")
+    masks = [manhattan(nx, ny, 0.5, s) for s in range(nc)]
     bits = dev(np.frombuffer(b"".join(pack_bits(m) for m in masks), np.uint8).copy())
-    out = torch.empty((2, 4 * L + 1, 4 * M + 1), dtype=torch.complex128, device="cuda")
-    assert lib.euvl_mask_spectrum(handle, P(bits), nx, ny, 2, 2 * L, 2 * M, P(out)) == 0, err(lib)
+    out = torch.empty((nc, 4 * L + 1, 4 * M + 1), dtype=torch.complex128, device="cuda")
+    assert lib.euvl_mask_spectrum(handle, P(bits), nx, ny, nc, 2 * L, 2 * M, P(out)) == 0, err(lib)
     torch.cuda.synchronize()
-    for c in range(2):
+    for c in range(nc):
         ref = OE.maskamp(masks[c], nx, ny, nx, ny, L, M, 0.0, 1.0)
         assert np.abs(out.cpu().numpy()[c] - ref).max() < 1e-14
 
