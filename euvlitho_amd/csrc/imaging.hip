@@ -6,6 +6,7 @@
 // spectrum grid (orders |l| <= lpx, |m| <= lpy), and the unnormalised inverse
 // DFT onto the nx x ny image is evaluated separably (y first, then x) with
 // |Ex|^2 + |Ey|^2 + |Ez|^2 summed over source points inside the x pass.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -82,26 +83,76 @@ __global__ void abbe_ypass_kernel(int n_sc, int nl, int nm, int lpy, int ny, con
 }
 
 // img[x][y] += w * sum_{s,c} | sum_l exp(2 pi i (l x mod nx)/nx) G[s][c][l][y] |^2
-__global__ __launch_bounds__(256) void abbe_xpass_kernel(int n_sc, int nl, int lpx, int nx, int ny, const double2* G,
-                                                         const double2* twx, double w, double* img) {
-  const int y = blockIdx.x * blockDim.x + threadIdx.x;
-  const int x = blockIdx.y;
-  if (y >= ny) return;
-  extern __shared__ double2 sh_tw[];  // nl twiddles for this x
-  for (int l = threadIdx.x; l < nl; l += blockDim.x) {
-    const int ll = l - lpx;
-    const int idx = (int)((((int64_t)ll * x) % nx + nx) % nx);
-    sh_tw[l] = twx[idx];
+// img[x][y] += w sum_sc |sum_l tw_x(l x) G[sc][l][y]|^2, in two steps: a
+// workgroup (one wave) takes 64 y values and 16 x values of one split of the
+// source/component range, so each G value read serves 16 x (the sum over l
+// runs in the same order as a per-pixel loop); partials P[s][x][y] are then
+// added in split order (deterministic).
+constexpr int XP_X = 16, XP_Y = 64;
+__global__ __launch_bounds__(XP_Y) void abbe_xpass_kernel(int n_sc, int nl, int lpx, int nx, int ny,
+                                                          const double2* __restrict__ G,
+                                                          const double2* __restrict__ twx, int scs,
+                                                          double* __restrict__ P) {
+  const int y = blockIdx.x * XP_Y + threadIdx.x;
+  const int x0 = blockIdx.y * XP_X;
+  const int s = blockIdx.z;
+  extern __shared__ double2 sh_tw[];  // [XP_X][nl] twiddles of this block's x values
+  for (int e = threadIdx.x; e < XP_X * nl; e += XP_Y) {
+    const int xb = e / nl, ll = e % nl - lpx;
+    const int64_t x = x0 + xb;
+    sh_tw[e] = twx[(int)(((ll * x) % nx + nx) % nx)];
   }
   __syncthreads();
-  double acc = 0.0;
-  for (int sc = 0; sc < n_sc; ++sc) {
+  if (y >= ny) return;
+  double acc[XP_X];
+#pragma unroll
+  for (int xb = 0; xb < XP_X; ++xb) acc[xb] = 0.0;
+  const int sc1 = min(n_sc, (s + 1) * scs);
+  for (int sc = s * scs; sc < sc1; ++sc) {
     const double2* g = G + (int64_t)sc * nl * ny + y;
-    double2 f = make_double2(0.0, 0.0);
-    for (int l = 0; l < nl; ++l) f = cadd(f, cmul(sh_tw[l], g[(int64_t)l * ny]));
-    acc += f.x * f.x + f.y * f.y;
+    double2 f[XP_X];
+#pragma unroll
+    for (int xb = 0; xb < XP_X; ++xb) f[xb] = make_double2(0.0, 0.0);
+    for (int l = 0; l < nl; ++l) {
+      const double2 gv = g[(int64_t)l * ny];
+#pragma unroll
+      for (int xb = 0; xb < XP_X; ++xb) f[xb] = cadd(f[xb], cmul(sh_tw[xb * nl + l], gv));
+    }
+#pragma unroll
+    for (int xb = 0; xb < XP_X; ++xb) acc[xb] += f[xb].x * f[xb].x + f[xb].y * f[xb].y;
   }
-  img[(int64_t)x * ny + y] += w * acc;
+#pragma unroll
+  for (int xb = 0; xb < XP_X; ++xb)
+    if (x0 + xb < nx) P[((int64_t)s * nx + x0 + xb) * ny + y] = acc[xb];
+}
+
+__global__ void abbe_xreduce_kernel(int S, int64_t per, const double* __restrict__ P, double w,
+                                    double* __restrict__ img) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+    double acc = P[e];
+    for (int s = 1; s < S; ++s) acc += P[(int64_t)s * per + e];
+    img[e] += w * acc;
+  }
+}
+
+// splits of the source/component range: about 1024 waves in all
+int xpass_splits(int n_sc, int nx, int ny) {
+  const int64_t blocks = (int64_t)((ny + XP_Y - 1) / XP_Y) * ((nx + XP_X - 1) / XP_X);
+  const int64_t want = (1024 + blocks - 1) / blocks;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::min(n_sc, 32)));
+}
+
+int xpass(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const double2* G, const double2* twx, double w,
+          double* img, double* P) {
+  const int S0 = xpass_splits(n_sc, nx, ny);
+  const int scs = (n_sc + S0 - 1) / S0, S = (n_sc + scs - 1) / scs;
+  hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + XP_Y - 1) / XP_Y, (nx + XP_X - 1) / XP_X, S), dim3(XP_Y),
+                     sizeof(double2) * XP_X * nl, h->stream, n_sc, nl, lpx, nx, ny, G, twx, scs, P);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(abbe_xreduce_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, S, (int64_t)nx * ny, P,
+                     w, img);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
 }
 
 // M3D fit of one clip, one thread per pupil point (m3d.cpp:329-467).
@@ -302,10 +353,12 @@ extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* 
   if (chunk > n_src) chunk = n_src;
   const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
   const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
-  EUVL_TRY(ws_reserve(h, up.host.size() + fbytes + gbytes + 4096));
+  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
+  EUVL_TRY(ws_reserve(h, up.host.size() + fbytes + gbytes + pbytes + 4096));
   char* blk = (char*)ws_take(h, up.host.size());
   double2* F = (double2*)ws_take(h, fbytes);
   double2* G = (double2*)ws_take(h, gbytes);
+  double* Pxp = (double*)ws_take(h, pbytes);
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
   for (int s0 = 0; s0 < n_src; s0 += chunk) {
     const int ns = std::min(chunk, n_src - s0);
@@ -317,9 +370,7 @@ extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* 
     hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl, nm,
                        lpy, ny, F, (const double2*)(blk + o_twy), G);
     EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + 255) / 256, nx), dim3(256), sizeof(double2) * nl, h->stream,
-                       ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), weight, d_img);
-    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(xpass(h, ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), weight, d_img, Pxp));
   }
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
@@ -433,11 +484,13 @@ extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const 
   if (chunk > n_src) chunk = n_src;
   const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
   const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
-  EUVL_TRY(ws_reserve(h, up.host.size() + 2 * fbytes + gbytes + 4096));
+  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
+  EUVL_TRY(ws_reserve(h, up.host.size() + 2 * fbytes + gbytes + pbytes + 4096));
   char* blk = (char*)ws_take(h, up.host.size());
   double2* Fl = (double2*)ws_take(h, fbytes);
   double2* Ft = (double2*)ws_take(h, fbytes);
   double2* G = (double2*)ws_take(h, gbytes);
+  double* Pxp = (double*)ws_take(h, pbytes);
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, d_intlinear);
   EUVL_LAUNCH_CHECK();
@@ -456,10 +509,8 @@ extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const 
       hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl, nm,
                          lpy, ny, which == 0 ? Fl : Ft, (const double2*)(blk + o_twy), G);
       EUVL_LAUNCH_CHECK();
-      hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + 255) / 256, nx), dim3(256), sizeof(double2) * nl, h->stream,
-                         ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), w,
-                         which == 0 ? d_intlinear : d_intft);
-      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(xpass(h, ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), w,
+                     which == 0 ? d_intlinear : d_intft, Pxp));
     }
   }
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
