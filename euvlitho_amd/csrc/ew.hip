@@ -1,5 +1,7 @@
 // Element-wise and reduction kernels on batches of n x n complex matrices
 // (contiguous, stride n*n per batch entry) used by the EM solve.
+#include <algorithm>
+
 #include "common.h"
 #include "ew.h"
 
@@ -7,43 +9,49 @@ namespace {
 
 __device__ inline int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// The element-wise kernels run on grid (blocks, batch) with 32-bit indices
+// inside one n x n matrix (n*n < 2^31, checked by the callers): the row and
+// column of an element come from a 32-bit division, not a 64-bit one.
+
 // out = a1*X1 + a2*X2 + a3*X3 + g*I  (X2/X3 may be null)
-__global__ void lincomb_kernel(int n, int64_t total, double2* out, double2 a1, const double2* X1, double2 a2,
-                               const double2* X2, double2 a3, const double2* X3, double2 g) {
-  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
+__global__ void lincomb_kernel(int n, double2* out, double2 a1, const double2* X1, double2 a2, const double2* X2,
+                               double2 a3, const double2* X3, double2 g) {
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
+  const int64_t base = (int64_t)blockIdx.y * per;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
+    const int64_t e = base + r;
     double2 v = cmul(a1, X1[e]);
     if (X2) v = cadd(v, cmul(a2, X2[e]));
     if (X3) v = cadd(v, cmul(a3, X3[e]));
-    const int64_t r = e % ((int64_t)n * n);
-    if (r / n == r % n) v = cadd(v, g);
+    if (r % ((uint32_t)n + 1) == 0) v = cadd(v, g);  // the diagonal of a row-major n x n block
     out[e] = v;
   }
 }
 
 // out[b] = X[b] * s[b] (per-batch complex scalar from device memory) + g*I
-__global__ void bscale_kernel(int n, int64_t per, int batch, double2* out, const double2* X, const double2* s,
-                              double2 g) {
-  const int64_t total = per * batch;
-  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int b = (int)(e / per);
-    const int64_t r = e - (int64_t)b * per;
-    double2 v = cmul(X[e], s[b]);
-    if (r / n == r % n) v = cadd(v, g);
-    out[e] = v;
+__global__ void bscale_kernel(int n, double2* out, const double2* X, const double2* s, double2 g) {
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
+  const int b = blockIdx.y;
+  const int64_t base = (int64_t)b * per;
+  const double2 sb = s[b];
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
+    double2 v = cmul(X[base + r], sb);
+    if (r % ((uint32_t)n + 1) == 0) v = cadd(v, g);
+    out[base + r] = v;
   }
 }
 
 // X[b](i,j) *= col[b*n + j] (column scaling) and/or row[b*n + i]
-__global__ void diagscale_kernel(int n, int64_t total, double2* out, const double2* X, const double2* row,
-                                 const double2* col) {
-  const int64_t per = (int64_t)n * n;
-  for (int64_t e = gidx(); e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = e / per, r = e - b * per;
-    const int i = (int)(r / n), j = (int)(r % n);
-    double2 v = X[e];
-    if (row) v = cmul(row[b * n + i], v);
-    if (col) v = cmul(v, col[b * n + j]);
-    out[e] = v;
+__global__ void diagscale_kernel(int n, double2* out, const double2* X, const double2* row, const double2* col) {
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
+  const int b = blockIdx.y;
+  const int64_t base = (int64_t)b * per;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
+    const uint32_t i = r / (uint32_t)n, j = r - i * (uint32_t)n;
+    double2 v = X[base + r];
+    if (row) v = cmul(row[(int64_t)b * n + i], v);
+    if (col) v = cmul(v, col[(int64_t)b * n + j]);
+    out[base + r] = v;
   }
 }
 
@@ -67,33 +75,51 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+// max over a 256-thread block, then one atomic per block: thousands of
+// atomics on one address serialise (~40 ns each)
+__device__ inline void block_max_to(double* addr, double v) {
+  __shared__ double sh[4];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = sh[0];
+    for (int w = 1; w < 4; ++w) m = (sh[w] > m || sh[w] != sh[w]) ? sh[w] : m;
+    atomic_max_nonneg(addr, m);
+  }
+}
+
 // kind 0: max |X - I| (elementwise |re| + |im|); grid (blocks, batch)
 __global__ __launch_bounds__(256) void reduce_maxdev_kernel(int n, const double2* X, double* red) {
   const int b = blockIdx.y;
-  const int64_t per = (int64_t)n * n;
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
   X += (int64_t)b * per;
   double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < per; e += gridDim.x * blockDim.x) {
     double2 v = X[e];
-    if (e / n == e % n) v.x -= 1.0;
+    if (e % ((uint32_t)n + 1) == 0) v.x -= 1.0;
     const double a = fabs(v.x) + fabs(v.y);
     acc = (a > acc || a != a) ? a : acc;
   }
-  acc = wave_max(acc);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(red + b, acc);
+  block_max_to(red + b, acc);
 }
 
 // kind 1: infinity norm (max row sum of |x|), one wave per row; grid (ceil(n/4), batch)
 __global__ __launch_bounds__(256) void reduce_infnorm_kernel(int n, const double2* X, double* red) {
   const int b = blockIdx.y;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= n) return;
-  const double2* x = X + (int64_t)b * n * n + (int64_t)row * n;
-  double s = 0.0;
-  for (int j = lane; j < n; j += 64) s += hypot(x[j].x, x[j].y);
-  s = wave_sum(s);
-  if (lane == 0) atomic_max_nonneg(red + b, s);
+  double best = 0.0;
+  // one wave per row, rows strided over the grid's waves
+  for (int row = blockIdx.x * 4 + (threadIdx.x >> 6); row < n; row += gridDim.x * 4) {
+    const double2* x = X + (int64_t)b * n * n + (int64_t)row * n;
+    double s = 0.0;
+    // |x| as sqrt(re^2 + im^2): the norm only sizes the scaling steps, and
+    // the entries are far from the range where hypot's rescaling matters
+    for (int j = lane; j < n; j += 64) s += sqrt(fma(x[j].x, x[j].x, x[j].y * x[j].y));
+    s = wave_sum(s);
+    best = (s > best || s != s) ? s : best;
+  }
+  block_max_to(red + b, best);
 }
 
 // kind 2: mean of Re diag, one block per batch entry
@@ -117,15 +143,15 @@ __global__ __launch_bounds__(256) void reduce_maxabs_kernel(int m, int ncols, in
                                                             const double2* X, double* red) {
   const int b = blockIdx.y;
   X += (int64_t)b * sb;
-  const int64_t per = (int64_t)m * ncols;
   double acc = 0.0;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < per; e += (int64_t)gridDim.x * blockDim.x) {
-    const double2 v = X[(e / ncols) * ld + e % ncols];
-    const double a = fabs(v.x) + fabs(v.y);
-    acc = (a > acc || a != a) ? a : acc;
-  }
-  acc = wave_max(acc);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(red + b, acc);
+  // rows over the grid, columns over the threads: no per-element division
+  for (int r = blockIdx.x; r < m; r += gridDim.x)
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+      const double2 v = X[(int64_t)r * ld + c];
+      const double a = fabs(v.x) + fabs(v.y);
+      acc = (a > acc || a != a) ? a : acc;
+    }
+  block_max_to(red + b, acc);
 }
 
 __global__ void zero_kernel(double* p, int n) {
@@ -143,30 +169,47 @@ inline dim3 ew_grid(int64_t total) {
   return dim3((unsigned)b);
 }
 
+// (blocks over one n x n matrix, batch); about 4096 workgroups in all
+inline dim3 ew_grid2(int64_t per, int batch) {
+  int64_t b = (per + 255) / 256, cap = std::max<int64_t>(1, 4096 / batch);
+  if (b > cap) b = cap;
+  if (b < 1) b = 1;
+  return dim3((unsigned)b, (unsigned)batch);
+}
+
+int check_ew(int n, int batch) {
+  if (n <= 0 || (int64_t)n * n >= ((int64_t)1 << 31) || batch <= 0 || batch > 65535) {
+    euvl::set_error("element-wise kernel: n = %d, batch = %d out of range", n, batch);
+    return EUVL_ERR_ARG;
+  }
+  return EUVL_OK;
+}
+
 }  // namespace
 
 namespace euvl {
 
 int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
                const double2* X2, double2 a3, const double2* X3, double2 g) {
-  const int64_t total = (int64_t)n * n * batch;
-  hipLaunchKernelGGL(lincomb_kernel, ew_grid(total), dim3(256), 0, h->stream, n, total, out, a1, X1, a2, X2, a3,
-                     X3, g);
+  EUVL_TRY(check_ew(n, batch));
+  hipLaunchKernelGGL(lincomb_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, a1, X1, a2,
+                     X2, a3, X3, g);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }
 
 int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* d_s, double2 g) {
-  const int64_t per = (int64_t)n * n;
-  hipLaunchKernelGGL(bscale_kernel, ew_grid(per * batch), dim3(256), 0, h->stream, n, per, batch, out, X, d_s, g);
+  EUVL_TRY(check_ew(n, batch));
+  hipLaunchKernelGGL(bscale_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, X, d_s, g);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }
 
 int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
                  const double2* col) {
-  const int64_t total = (int64_t)n * n * batch;
-  hipLaunchKernelGGL(diagscale_kernel, ew_grid(total), dim3(256), 0, h->stream, n, total, out, X, row, col);
+  EUVL_TRY(check_ew(n, batch));
+  hipLaunchKernelGGL(diagscale_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, X, row,
+                     col);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }
@@ -186,11 +229,12 @@ int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* 
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, h->stream, d_red, batch);
   EUVL_LAUNCH_CHECK();
   if (kind == 0) {
+    EUVL_TRY(check_ew(n, batch));
     int64_t blocks = ((int64_t)n * n + 255) / 256;
-    if (blocks > 1024) blocks = 1024;
+    if (blocks > 256) blocks = 256;
     hipLaunchKernelGGL(reduce_maxdev_kernel, dim3((unsigned)blocks, batch), dim3(256), 0, h->stream, n, X, d_red);
   } else {
-    hipLaunchKernelGGL(reduce_infnorm_kernel, dim3((n + 3) / 4, batch), dim3(256), 0, h->stream, n, X, d_red);
+    hipLaunchKernelGGL(reduce_infnorm_kernel, dim3((unsigned)std::min(256, (n + 3) / 4), batch), dim3(256), 0, h->stream, n, X, d_red);
   }
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
@@ -207,8 +251,8 @@ int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, 
 int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X, double* d_slot) {
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, h->stream, d_slot, batch);
   EUVL_LAUNCH_CHECK();
-  int64_t blocks = ((int64_t)m * ncols + 255) / 256;
-  if (blocks > 1024) blocks = 1024;
+  int64_t blocks = m;
+  if (blocks > 256) blocks = 256;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(reduce_maxabs_kernel, dim3((unsigned)blocks, batch), dim3(256), 0, h->stream, m, ncols, ld, sb, X,
                      d_slot);

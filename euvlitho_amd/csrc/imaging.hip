@@ -135,11 +135,12 @@ __global__ void abbe_xreduce_kernel(int S, int64_t per, const double* __restrict
   }
 }
 
-// splits of the source/component range: about 1024 waves in all
+// splits of the source/component range: about 4096 waves in all (four per
+// SIMD, so that the G loads of one wave overlap another's arithmetic)
 int xpass_splits(int n_sc, int nx, int ny) {
   const int64_t blocks = (int64_t)((ny + XP_Y - 1) / XP_Y) * ((nx + XP_X - 1) / XP_X);
-  const int64_t want = (1024 + blocks - 1) / blocks;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::min(n_sc, 32)));
+  const int64_t want = (4096 + blocks - 1) / blocks;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(want, std::min(n_sc, 64)));
 }
 
 int xpass(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const double2* G, const double2* twx, double w,
