@@ -20,7 +20,7 @@ class EuvlError(RuntimeError):
 
 # euvl_set_precision bits (include/euvl.h)
 PREC_FP32_MATFUN, PREC_FP32_COUPLING, PREC_FP32_LU, PREC_FP32_GEMM = 1, 2, 4, 8
-PREC_GEMM_4M = 16  # FP64 complex products with 4 real MFMAs (default: 3, the zgemm3m form)
+PREC_GEMM_4M = 16  # complex products with 4 real MFMAs, FP64 and FP32 (default: 3, the zgemm3m form)
 PREC_FP64 = 0
 PREC_MIXED = PREC_FP32_MATFUN | PREC_FP32_COUPLING | PREC_FP32_LU
 

@@ -22,7 +22,9 @@
 //        3M form (F64M3, the default) or the 4M form (F64, EUVL_PREC_GEMM_4M);
 //   F32: operands rounded to FP32 when staged into LDS, v_mfma_f32_16x16x4_f32
 //        with FP32 accumulation over K, FP64 epilogue (the precision form:
-//        twice the MFMA rate of FP64 on MI355X, half the LDS and registers).
+//        twice the MFMA rate of FP64 on MI355X, half the LDS and registers),
+//        in the 3M form (F32M3, the default; P3 - P1 - P2 is taken in FP64 in
+//        the epilogue) or the 4M form (F32, EUVL_PREC_GEMM_4M).
 // Operand layout of both 16x16x4 MFMAs: lane l holds A[l%16][l/16] and
 // B[l/16][l%16].  Accumulator element r of lane l is D[4r + l/16][l%16] for
 // f64 and D[4(l/16) + r][l%16] for f32 (both measured on gfx950).
@@ -69,6 +71,10 @@ struct F32 {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int row(int r, int lk) { return 4 * lk + r; }
+};
+
+struct F32M3 : F32 {
+  static constexpr bool three = true;
 };
 
 // FAST: unchecked loads for interior tiles (edge tiles shifted back inside C
@@ -360,10 +366,12 @@ int pick_variant(int kind, int m, int n, int k, int batch) {
 //   1681 x 4: 102/99/100/77/96;  4096: 130/124/127/124/114;
 //   8705: 123/119/123/108/110;  2048 x 4: 128/121/125/121/113;  420 x 4: 40/29/30/17/24
 // (before the FAST form, 1681 x 4: 79/88/89/78/65 and 4096: 102/124/110/124/86).
-int pick_variant32(int m, int n, int k, int batch) {
+// The 3M form of the 64x64 tile: 1681 x 4: 116, 4096: 154, 841 x 4: 96,
+// 1681 x 171 x 1681 x 4: 65 (4M: 100, 129, 80, 57).
+int pick_variant32(int kind, int m, int n, int k, int batch) {
   if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 4);
   if (g_variant32 >= 0) return g_variant32;
-  const int t = tuned_variant(1, m, n, k, batch);
+  const int t = tuned_variant(kind, m, n, k, batch);
   return t >= 0 ? t : 0;
 }
 
@@ -383,18 +391,22 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
   };
   const auto c = span(C, sc, batch, m, ldc, n);
   const auto a = span(A, sa, batch, m, lda, k), b = span(B, sb, batch, k, ldb, n);
-  const bool overlap = k > 0 && ((a.first < c.second && c.first < a.second) || (b.first < c.second && c.first < b.second));
+  const bool overlap =
+      k > 0 && ((a.first < c.second && c.first < a.second) || (b.first < c.second && c.first < b.second));
   const int shift = (m >= BM && n >= BN && !overlap) ? 1 : 0;
   hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB, FAST>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles,
                      shift);
 }
 
-// kind: 0 = FP64 4M, 1 = FP32 compute, 2 = FP64 3M
-int gemm_kind(euvl_handle h) {
-  if (h->prec_mask & h->prec_scope) return 1;
-  return (h->prec_mask & EUVL_PREC_GEMM_4M) ? 0 : 2;
+// kind: 0 = FP64 4M, 1 = FP32 compute 4M, 2 = FP64 3M, 3 = FP32 compute 3M
+int gemm_kind(euvl_handle h, bool fp32) {
+  const bool m4 = (h->prec_mask & EUVL_PREC_GEMM_4M) != 0;
+  if (fp32) return m4 ? 1 : 3;
+  return m4 ? 0 : 2;
 }
+int gemm_kind(euvl_handle h) { return gemm_kind(h, (h->prec_mask & h->prec_scope) != 0); }
+bool is_fp32_kind(int kind) { return kind == 1 || kind == 3; }
 
 void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2 alpha, const double2* A,
                     int64_t lda, int64_t sa, const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C,
@@ -404,9 +416,18 @@ void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2
     switch (v) {
       case 0: launch<F64M3, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
       case 2: launch<F64M3, 2, 4, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
-      case 3: launch<F64M3, 4, 4, 2, 1, 16, 1>(EUVL_GEMM_ARGS); break;  // 128x64: 3 accumulator sets of 2x2 spill at 1024 threads
+      // 128x64: three accumulator sets of 2x2 tiles spill at 1024 threads
+      case 3: launch<F64M3, 4, 4, 2, 1, 16, 1>(EUVL_GEMM_ARGS); break;
       case 4: launch<F64M3, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
       default: launch<F64M3, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
+    }
+  } else if (kind == 3) {
+    switch (v) {
+      case 1: launch<F32M3, 2, 4, 2, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      case 2: launch<F32M3, 4, 2, 2, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      case 3: launch<F32M3, 4, 4, 2, 2, 16, 1, false>(EUVL_GEMM_ARGS); break;
+      case 4: launch<F32M3, 2, 2, 4, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      default: launch<F32M3, 2, 2, 2, 2, 16, 4, true>(EUVL_GEMM_ARGS); break;
     }
   } else if (kind == 1) {
     switch (v) {
@@ -454,14 +475,15 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev], h->stream));
   }
   const int kind = gemm_kind(h);
-  launch_variant(h, kind, kind == 1 ? pick_variant32(m, n, k, batch) : pick_variant(kind, m, n, k, batch), m, n, k,
-                 alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch);
+  const int v = is_fp32_kind(kind) ? pick_variant32(kind, m, n, k, batch) : pick_variant(kind, m, n, k, batch);
+  launch_variant(h, kind, v, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch);
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev + 1], h->stream));
     const double kk = k > 0 ? k : 0, nb = batch;
     const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
-    const double bytes = 16.0 * (m * kk * (sa ? nb : 1.0) + kk * n * (sb ? nb : 1.0) + (double)m * n * nb * (rd_c ? 2 : 1));
+    const double bytes =
+        16.0 * (m * kk * (sa ? nb : 1.0) + kk * n * (sb ? nb : 1.0) + (double)m * n * nb * (rd_c ? 2 : 1));
     h->ev_used.push_back({ev, 8.0 * (double)m * (double)n * kk * nb, bytes});
   }
   return EUVL_OK;
@@ -527,8 +549,8 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
     euvl::set_error("euvl_autotune_gemm: allocation failed");
     return EUVL_ERR_NOMEM;
   }
-  // FP64 variants are tuned in the product form the handle's mask selects
-  const int kind = fp32 ? 1 : ((h->prec_mask & EUVL_PREC_GEMM_4M) ? 0 : 2);
+  // variants are tuned in the product form the handle's mask selects
+  const int kind = gemm_kind(h, fp32 != 0);
   int rc = EUVL_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double best_ms = 1e300;

@@ -75,12 +75,12 @@ int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 #define EUVL_PREC_FP32_COUPLING 2
 #define EUVL_PREC_FP32_LU 4
 #define EUVL_PREC_FP32_GEMM 8
-/* FP64 products (those not moved to FP32) form each complex product with 3
- * real MFMAs by default (the zgemm3m form: Re = ArBr - AiBi,
- * Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi), a quarter fewer MFMAs than the
- * textbook 4 with a rounding error bounded like theirs in norm
- * ((|Ar| + |Ai|)(|Br| + |Bi|) in place of |A||B|).  EUVL_PREC_GEMM_4M selects
- * the 4-multiplication form. */
+/* Complex products, FP64 or FP32-compute, are formed with 3 real MFMAs by
+ * default (the zgemm3m form: Re = ArBr - AiBi,
+ * Im = (Ar + Ai)(Br + Bi) - ArBr - AiBi, the differences taken in FP64), a
+ * quarter fewer MFMAs than the textbook 4 with a rounding error bounded like
+ * theirs in norm ((|Ar| + |Ai|)(|Br| + |Bi|) in place of |A||B|).
+ * EUVL_PREC_GEMM_4M selects the 4-multiplication form in both precisions. */
 #define EUVL_PREC_GEMM_4M 16
 int euvl_set_precision(euvl_handle h, int mask);
 

@@ -43,18 +43,24 @@ def test_zgemm_fp32_compute(lib, m, n, k, batch):
     h = ctypes.c_void_p()
     assert lib.euvl_create(ctypes.byref(h)) == 0
     try:
-        assert lib.euvl_set_precision(h, _lib.PREC_FP32_GEMM) == 0
         al, be, ga = (ctypes.c_double * 2)(0.5, -1.0), (ctypes.c_double * 2)(0.25, 0.5), (ctypes.c_double * 2)(2, 0)
-        rc = lib.euvl_zgemm_batched(h, m, n, k, al, ctypes.c_void_p(dA.data_ptr()), k, m * k,
-                                    ctypes.c_void_p(dB.data_ptr()), n, k * n, be, ctypes.c_void_p(dC.data_ptr()), n,
-                                    m * n, ga, batch)
-        assert rc == 0, lib.euvl_last_error().decode()
-        torch.cuda.synchronize()
         ref = (0.5 - 1j) * A @ B + (0.25 + 0.5j) * C0 + 2.0 * np.eye(m, n)[None]
-        e = np.abs(dC.cpu().numpy() - ref).max() / np.abs(ref).max()
-        say(f"  max err / max |C| = {e:.2e}")
-        assert e < 2e-5
-        assert e > 1e-12  # it really ran in FP32
+        got = {}
+        # the 3M form (default) and the 4M form, both with FP32 products
+        for name, mask in (("3m", _lib.PREC_FP32_GEMM), ("4m", _lib.PREC_FP32_GEMM | _lib.PREC_GEMM_4M)):
+            assert lib.euvl_set_precision(h, mask) == 0
+            dC.copy_(torch.from_numpy(C0))
+            rc = lib.euvl_zgemm_batched(h, m, n, k, al, ctypes.c_void_p(dA.data_ptr()), k, m * k,
+                                        ctypes.c_void_p(dB.data_ptr()), n, k * n, be, ctypes.c_void_p(dC.data_ptr()),
+                                        n, m * n, ga, batch)
+            assert rc == 0, lib.euvl_last_error().decode()
+            torch.cuda.synchronize()
+            got[name] = dC.cpu().numpy()
+            e = np.abs(got[name] - ref).max() / np.abs(ref).max()
+            say(f"  {name}: max err / max |C| = {e:.2e}")
+            assert e < 2e-5
+            assert e > 1e-12  # it really ran in FP32
+        assert np.abs(got["3m"] - got["4m"]).max() > 0.0  # two different kernels ran
         # the mask selects: FP64 again after resetting it
         assert lib.euvl_set_precision(h, 0) == 0
         dC.copy_(torch.from_numpy(C0))
