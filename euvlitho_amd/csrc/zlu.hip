@@ -244,18 +244,20 @@ __global__ __launch_bounds__(256) void tslu_apply_kernel(double2* __restrict__ A
 // No column step of the factorisation runs over the panel's full height, so
 // a panel costs a few short launches instead of 32 dependent pivot searches.
 // The block is small and factored once per launch, so the kernel keeps it
-// in registers of 256 threads (thread (r, c) holds rows r, r+8, r+16, r+24 of
-// column c) with short loops: a fully unrolled single-lane-per-row form runs to
+// in registers of 512 threads (thread (r, c) holds rows r and r+16 of column
+// c) with short loops: a fully unrolled single-lane-per-row form runs to
 // ~20k instructions and is bound by instruction fetch.  The inverses are
-// row-parallel substitutions (lane i of a 32-lane group holds row i of eight
-// columns, the finished row is broadcast at each step).
-constexpr int DLU_THREADS = 256;
+// row-parallel substitutions (lane i of a 32-lane group holds row i of four
+// columns, the finished row is broadcast at each step); four waves form
+// U11^-1 while the other four form L11^-1.  512 threads measured 34 us per
+// block, 256 threads 44 us and 1024 threads 37 us.
+constexpr int DLU_THREADS = 512;
 __global__ __launch_bounds__(DLU_THREADS) void diag_lu_kernel(double2* __restrict__ A, int64_t lda, int64_t sa,
                                                               int kb, int nbc, double2* __restrict__ uinv,
                                                               double2* __restrict__ linv, int* __restrict__ ipiv,
                                                               int n, int* __restrict__ info) {
-  constexpr int RG = DLU_THREADS / NB;  // row groups (8)
-  constexpr int RS = NB / RG;           // rows per thread (4)
+  constexpr int RG = DLU_THREADS / NB;  // row groups (16)
+  constexpr int RS = NB / RG;           // rows per thread (2)
   __shared__ double2 a[NB][NB + 1];
   // step j publishes row j and column j into buffers of parity j & 1, so one
   // barrier per step orders them (step j+1 writes the other buffers)
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(DLU_THREADS) void diag_lu_kernel(double2* __restric
     for (int j = 0; j < nbc; ++j)
       if (a[j][j].x == 0.0 && a[j][j].y == 0.0 && info[b] == 0) info[b] = kb + j + 1;
   // U11^-1 (row groups 0-3) and L11^-1 (4-7): lane i holds row i of CB columns
-  constexpr int CB = NB / (RG / 2);  // columns per group (8)
+  constexpr int CB = NB / (RG / 2);  // columns per group (4)
   const int i = tq, g = tr & (RG / 2 - 1);
   double2 y[CB];
 #pragma unroll
