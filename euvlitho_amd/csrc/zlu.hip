@@ -423,9 +423,12 @@ __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, in
       if (i > p) x = csub(x, cmul(sT[i][p], xp));
     }
   } else {
+    // each lane's diagonal reciprocal up front: no division on the
+    // substitution's dependent chain
+    const double2 rii = (i < nbc) ? cdiv(make_double2(1.0, 0.0), sT[i][i]) : make_double2(0.0, 0.0);
 #pragma unroll 1
     for (int p = nbc - 1; p >= 0; --p) {
-      if (i == p) x = cdiv(x, sT[p][p]);
+      if (i == p) x = cmul(x, rii);
       const double2 xp = make_double2(__shfl(x.x, p, NB), __shfl(x.y, p, NB));
       if (i < p) x = csub(x, cmul(sT[i][p], xp));
     }
