@@ -89,6 +89,7 @@ struct euvl_handle_s {
     int ev;        // index of the start event (the end event follows)
     double flops;  // 8 m n k batch
     double bytes;  // A, B read once, C written (and read when beta != 0)
+    bool fp32;     // products on FP32 MFMA
   };
   std::vector<GemmRecord> ev_used;
   size_t ev_next = 0;

@@ -484,7 +484,7 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
     const double bytes =
         16.0 * (m * kk * (sa ? nb : 1.0) + kk * n * (sb ? nb : 1.0) + (double)m * n * nb * (rd_c ? 2 : 1));
-    h->ev_used.push_back({ev, 8.0 * (double)m * (double)n * kk * nb, bytes});
+    h->ev_used.push_back({ev, 8.0 * (double)m * (double)n * kk * nb, bytes, is_fp32_kind(kind)});
   }
   return EUVL_OK;
 }
@@ -514,22 +514,23 @@ extern "C" int euvl_gemm_stats(euvl_handle h, int enable, double* out) {
   if (!h) return EUVL_ERR_ARG;
   if (out) {
     EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-    double ms = 0.0, flops = 0.0, bytes = 0.0;
+    // out[0..3]: every recorded launch; out[4..7] (when enable & 2): those on FP32 MFMA
+    double sum[2][4] = {};
     for (auto& u : h->ev_used) {
       float t = 0.0f;
       EUVL_HIP_CHECK(hipEventElapsedTime(&t, h->ev_pool[u.ev], h->ev_pool[u.ev + 1]));
-      ms += t;
-      flops += u.flops;
-      bytes += u.bytes;
+      for (int g = 0; g < (u.fp32 ? 2 : 1); ++g) {
+        sum[g][0] += 1.0;
+        sum[g][1] += t;
+        sum[g][2] += u.flops;
+        sum[g][3] += u.bytes;
+      }
     }
-    out[0] = (double)h->ev_used.size();
-    out[1] = ms;
-    out[2] = flops;
-    out[3] = bytes;
+    for (int i = 0; i < ((enable & 2) ? 8 : 4); ++i) out[i] = sum[i / 4][i % 4];
   }
   h->ev_used.clear();
   h->ev_next = 0;
-  h->prof = enable != 0;
+  h->prof = (enable & 1) != 0;
   return EUVL_OK;
 }
 

@@ -56,8 +56,9 @@ int euvl_release_workspace(euvl_handle h);
  * milliseconds, algorithmic flops (8 m n k per complex GEMM), algorithmic
  * bytes (A and B read once, C written, and read too when beta != 0)} of the
  * launches recorded since the last call (out holds 4 doubles); then clears
- * them and enables (1) or disables (0) recording.  No reference counterpart
- * (instrumentation). */
+ * them and enables (enable & 1) or disables recording.  With enable & 2, out
+ * holds 8 doubles: the same four figures follow for the launches whose
+ * products ran on FP32 MFMA.  No reference counterpart (instrumentation). */
 int euvl_gemm_stats(euvl_handle h, int enable, double* out);
 
 /* Compute precision of the products inside the EM solve (the precision

@@ -46,6 +46,7 @@ def test_zgemm_fp32_compute(lib, m, n, k, batch):
         al, be, ga = (ctypes.c_double * 2)(0.5, -1.0), (ctypes.c_double * 2)(0.25, 0.5), (ctypes.c_double * 2)(2, 0)
         ref = (0.5 - 1j) * A @ B + (0.25 + 0.5j) * C0 + 2.0 * np.eye(m, n)[None]
         got = {}
+        assert lib.euvl_gemm_stats(h, 1, None) == 0  # record the three launches below
         # the 3M form (default) and the 4M form, both with FP32 products
         for name, mask in (("3m", _lib.PREC_FP32_GEMM), ("4m", _lib.PREC_FP32_GEMM | _lib.PREC_GEMM_4M)):
             assert lib.euvl_set_precision(h, mask) == 0
@@ -70,6 +71,13 @@ def test_zgemm_fp32_compute(lib, m, n, k, batch):
         assert rc == 0
         torch.cuda.synchronize()
         assert np.abs(dC.cpu().numpy() - ref).max() / np.abs(ref).max() < 1e-13
+        # launch records: all three, then the two whose products ran on FP32 MFMA
+        st = (ctypes.c_double * 8)()
+        assert lib.euvl_gemm_stats(h, 2, st) == 0
+        fl = 8.0 * m * n * k * batch
+        assert (st[0], st[2], st[4], st[6]) == (3.0, 3 * fl, 2.0, 2 * fl)
+        assert st[1] > st[5] > 0.0
+        assert lib.euvl_gemm_stats(h, 0, st) == 0 and st[0] == 0.0  # recording was off
         assert lib.euvl_set_precision(h, 64) != 0  # unknown bit
     finally:
         lib.euvl_destroy(h)
