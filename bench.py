@@ -76,11 +76,12 @@ def run_served(n_req, slots):
         srv.close()
 
 
-def measured_traffic():
-    """HBM bytes per GEMM launch of this workload, measured with rocprofv3 PMC
-    counters (FETCH_SIZE, WRITE_SIZE, scaled by a plain copy of known size) and
-    kept as a summary under profiles/; empty when absent."""
-    path = os.path.join(ROOT, "profiles", "zgemm_mfma_c2_fp64_traffic.json")
+def measured_traffic(form="fp64"):
+    """HBM bytes per GEMM launch of this workload (form "fp64" or "mixed"),
+    measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE, scaled by a
+    plain copy of known size) and kept as a summary under profiles/; empty
+    when absent."""
+    path = os.path.join(ROOT, "profiles", f"zgemm_mfma_c2_{form}_traffic.json")
     if not os.path.exists(path):
         return {}
     with open(path) as f:
@@ -399,7 +400,7 @@ def main():
         eng.close()
         return dict(lat=lat, wall=wall, tcc_ms=tcc_ms, outs=outs, launches=stats[0], gemm_ms=stats[1],
                     gemm_flops=stats[2], gemm_bytes=stats[3], f32_launches=stats[4], f32_ms=stats[5],
-                    f32_flops=stats[6], tuned=tuned, em=em)
+                    f32_flops=stats[6], f32_bytes=stats[7], tuned=tuned, em=em)
 
     f64 = run_form(_lib.PREC_FP64)
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
@@ -414,6 +415,7 @@ def main():
         (A64, a064, I64, S64), (A32, a032, I32, S32) = f64["outs"], mix["outs"]
         # the FP32-product launches only (the form keeps some products in FP64)
         mix_ach = mix["f32_flops"] / (mix["f32_ms"] * 1e-3) / 1e12 if mix["f32_ms"] > 0 else 0.0
+        mix_traffic = measured_traffic("mixed")
         mixed = {
             "value": args.steps / mix["wall"], "unit": "clips/s", "ms_per_step": 1000.0 * mix["wall"] / args.steps,
             "p50_ms": 1000.0 * float(np.percentile(mix["lat"], 50)),
@@ -428,6 +430,9 @@ def main():
                          "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": mix_ach / FP32_PEAK_TFLOPS,
                          "avg_launch_us": 1000.0 * mix["f32_ms"] / max(mix["f32_launches"], 1),
                          "launches_per_clip": int(mix["f32_launches"]),
+                         "traffic": mix_traffic.get("traffic_bytes_per_launch"),
+                         "traffic_source": mix_traffic.get("source"),
+                         "algorithmic_bytes_per_launch": mix["f32_bytes"] / max(mix["f32_launches"], 1),
                          "fp64_launches_per_clip": int(mix["launches"] - mix["f32_launches"]),
                          "gemm_ms_per_clip": mix["gemm_ms"], "f32_gemm_ms_per_clip": mix["f32_ms"],
                          "autotuned_variant": mix["tuned"]["fp32"][0],
