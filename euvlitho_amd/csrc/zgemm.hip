@@ -353,11 +353,18 @@ int tuned_variant(int kind, int m, int n, int k, int batch) {
   return it == g_tuned.end() ? -1 : it->second;
 }
 
+// Untuned FP64 3M products whose 64x64 grid has fewer than two workgroups
+// per CU take the 64x32 tile (variant 6).  On MI355X (us, 64x64 -> 64x32):
+// 1681 x 171 x 1681 x 4: 371 -> 306; the LU panels 1649 x 32 x 32 x 4 and
+// 32 x 1649 x 32 x 4: 9 -> 6; 1617 x 171 x 32 x 4: 13 -> 12.  Full-size
+// products stay on the 64x64 tile (1681 x 4: 2062 vs 2083).
 int pick_variant(int kind, int m, int n, int k, int batch) {
-  if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 4);
+  if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 6);
   if (getenv("EUVL_GEMM_TILE")) return g_variant;
   const int t = tuned_variant(kind, m, n, k, batch);
-  return t >= 0 ? t : g_variant;
+  if (t >= 0) return t;
+  const int64_t grid64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * batch;
+  return (kind == 2 && grid64 < 512) ? 6 : g_variant;
 }
 
 // FP32 compute.  With the FAST form the 64x64 tile (five workgroups per CU)
@@ -419,6 +426,8 @@ void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2
       // 128x64: three accumulator sets of 2x2 tiles spill at 1024 threads
       case 3: launch<F64M3, 4, 4, 2, 1, 16, 1>(EUVL_GEMM_ARGS); break;
       case 4: launch<F64M3, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      // 64x32 tile, for grids too small to fill the chip with 64x64 tiles
+      case 6: launch<F64M3, 2, 2, 2, 1, 16, 4>(EUVL_GEMM_ARGS); break;
       default: launch<F64M3, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
     }
   } else if (kind == 3) {
