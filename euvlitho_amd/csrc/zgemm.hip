@@ -376,10 +376,12 @@ int pick_variant(int kind, int m, int n, int k, int batch) {
 // The 3M form of the 64x64 tile: 1681 x 4: 116, 4096: 154, 841 x 4: 96,
 // 1681 x 171 x 1681 x 4: 65 (4M: 100, 129, 80, 57).
 int pick_variant32(int kind, int m, int n, int k, int batch) {
-  if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 4);
+  if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 6);
   if (g_variant32 >= 0) return g_variant32;
   const int t = tuned_variant(kind, m, n, k, batch);
-  return t >= 0 ? t : 0;
+  if (t >= 0) return t;
+  const int64_t grid64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * batch;
+  return (kind == 3 && grid64 < 512) ? 6 : 0;  // the 64x32 tile on small grids
 }
 
 template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST = P::fast_loads>
@@ -436,6 +438,8 @@ void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2
       case 2: launch<F32M3, 4, 2, 2, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
       case 3: launch<F32M3, 4, 4, 2, 2, 16, 1, false>(EUVL_GEMM_ARGS); break;
       case 4: launch<F32M3, 2, 2, 4, 2, 16, 2, true>(EUVL_GEMM_ARGS); break;
+      // 64x32 tile, for grids too small to fill the chip with 64x64 tiles
+      case 6: launch<F32M3, 2, 2, 2, 1, 16, 4, true>(EUVL_GEMM_ARGS); break;
       default: launch<F32M3, 2, 2, 2, 2, 16, 4, true>(EUVL_GEMM_ARGS); break;
     }
   } else if (kind == 1) {
