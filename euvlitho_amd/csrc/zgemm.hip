@@ -33,6 +33,8 @@
 // the linear workgroup id is remapped so that every XCD works on one
 // contiguous range of tiles, walked in groups of 8 tile rows (shared A row
 // panels and B column panels stay in one L2).
+#include <algorithm>
+#include <cstdint>
 #include <cstdlib>
 #include <map>
 #include <mutex>
@@ -324,15 +326,35 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 // 0 = 2x2 waves, 64x64; 1 = 2x4 waves, 64x128; 2 = 4x2 waves, 128x64;
 // 3 = 4x4 waves, 128x128; 4 = 2x2 waves of 4x2 tiles, 128x64.
 // EUVL_GEMM_TILE32 overrides the choice by size below.
-int g_variant = -1, g_variant32 = -2;  // -2: environment not read yet, -1: by size
+// per product kind: -2 = environment not read yet, -1 = unset or not a variant of the kind
+int g_env_variant[4] = {-2, -2, -2, -2};
 
-int env_variant(const char* name, int dflt, int hi) {
-  const char* e = getenv(name);
-  int v = e ? atoi(e) : dflt;
-  return (v < 0 || v > hi) ? dflt : v;
+// Tile shape (rows BM x columns BN of C per workgroup) of variant v of a
+// product kind; false for a variant number the kind does not have.  Variant
+// 6 (64x32) exists for the 3M kinds only.
+bool tile_shape(int kind, int v, int& bm, int& bn) {
+  switch (v) {
+    case 0: bm = 64; bn = 64; return true;
+    case 1: bm = 64; bn = (kind == 1 || kind == 3) ? 128 : 64; return true;
+    case 2: bm = (kind == 1 || kind == 3) ? 128 : 64; bn = (kind == 1 || kind == 3) ? 64 : 128; return true;
+    case 3: bm = 128; bn = (kind == 2) ? 64 : 128; return true;
+    case 4: bm = 128; bn = 64; return true;
+    case 6: bm = 64; bn = 32; return kind == 2 || kind == 3;
+    default: return false;
+  }
 }
 
-constexpr int N_VARIANTS = 5;  // per precision
+constexpr int MAX_VARIANT = 6;  // out_ms of euvl_autotune_gemm holds MAX_VARIANT + 1 entries
+
+// EUVL_GEMM_TILE (FP64 kinds) / EUVL_GEMM_TILE32 (FP32-compute kinds): a
+// variant number the active kind has, else ignored (-1).
+int env_variant(const char* name, int kind) {
+  const char* e = getenv(name);
+  if (!e) return -1;
+  int bm, bn;
+  const int v = atoi(e);
+  return tile_shape(kind, v, bm, bn) ? v : -1;
+}
 
 // Autotuned choices (euvl_autotune_gemm), per precision and size bucket: m, n
 // and k rounded up to a multiple of 64 below 256 and of 128 above, batch
@@ -359,12 +381,12 @@ int tuned_variant(int kind, int m, int n, int k, int batch) {
 // 32 x 1649 x 32 x 4: 9 -> 6; 1617 x 171 x 32 x 4: 13 -> 12.  Full-size
 // products stay on the 64x64 tile (1681 x 4: 2062 vs 2083).
 int pick_variant(int kind, int m, int n, int k, int batch) {
-  if (g_variant < 0) g_variant = env_variant("EUVL_GEMM_TILE", 1, 6);
-  if (getenv("EUVL_GEMM_TILE")) return g_variant;
+  if (g_env_variant[kind] == -2) g_env_variant[kind] = env_variant("EUVL_GEMM_TILE", kind);
+  if (g_env_variant[kind] >= 0) return g_env_variant[kind];
   const int t = tuned_variant(kind, m, n, k, batch);
   if (t >= 0) return t;
   const int64_t grid64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * batch;
-  return (kind == 2 && grid64 < 512) ? 6 : g_variant;
+  return (kind == 2 && grid64 < 512) ? 6 : 1;
 }
 
 // FP32 compute.  With the FAST form the 64x64 tile (five workgroups per CU)
@@ -376,12 +398,74 @@ int pick_variant(int kind, int m, int n, int k, int batch) {
 // The 3M form of the 64x64 tile: 1681 x 4: 116, 4096: 154, 841 x 4: 96,
 // 1681 x 171 x 1681 x 4: 65 (4M: 100, 129, 80, 57).
 int pick_variant32(int kind, int m, int n, int k, int batch) {
-  if (g_variant32 == -2) g_variant32 = env_variant("EUVL_GEMM_TILE32", -1, 6);
-  if (g_variant32 >= 0) return g_variant32;
+  if (g_env_variant[kind] == -2) g_env_variant[kind] = env_variant("EUVL_GEMM_TILE32", kind);
+  if (g_env_variant[kind] >= 0) return g_env_variant[kind];
   const int t = tuned_variant(kind, m, n, k, batch);
   if (t >= 0) return t;
   const int64_t grid64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * batch;
   return (kind == 3 && grid64 < 512) ? 6 : 0;  // the 64x32 tile on small grids
+}
+
+// Whether C shares elements with A or B.  Each pair of batch entries is
+// tested: disjoint byte ranges never overlap; submatrices of one row-major
+// matrix (same leading dimension) overlap only if their row and column ranges
+// both intersect, so blocks of one matrix (an LU's trailing update reads the
+// panel beside the block it writes) are told apart exactly; anything else
+// whose ranges meet counts as an overlap.
+struct Overlap {
+  bool a = false, b = false;
+};
+
+bool blocks_overlap(const double2* x, int64_t ldx, int rx, int cx, const double2* c, int64_t ldc, int rc, int cc) {
+  const int64_t ex = (int64_t)(rx - 1) * ldx + cx, ec = (int64_t)(rc - 1) * ldc + cc;  // extents in elements
+  const int64_t off = c - x;  // element offset of C from X
+  if (off >= ex || -off >= ec) return false;
+  if (ldx != ldc || cx > ldx || cc > ldc) return true;
+  const int64_t ld = ldx;
+  const int64_t drow = off >= 0 ? off / ld : -((-off + ld - 1) / ld);
+  const int64_t dcol = off - drow * ld;  // 0 <= dcol < ld
+  auto meet = [&](int64_t r0, int64_t c0, int64_t c1) {  // C rows [r0, r0+rc), cols [c0, c1) in X coordinates
+    return r0 < rx && r0 + rc > 0 && c0 < cx && c1 > 0 && c1 > c0;
+  };
+  if (meet(drow, dcol, std::min<int64_t>(dcol + cc, ld))) return true;
+  return dcol + cc > ld && meet(drow + 1, 0, dcol + cc - ld);  // the part of each C row past column ld
+}
+
+Overlap overlap_of(int m, int n, int k, const double2* A, int64_t lda, int64_t sa, const double2* B,
+                   int64_t ldb, int64_t sb, const double2* C, int64_t ldc, int64_t sc, int batch) {
+  Overlap o;
+  if (k <= 0 || m <= 0 || n <= 0) return o;
+  const int na = sa ? batch : 1, nb = sb ? batch : 1;
+  const bool exact = (int64_t)batch * std::max(na, nb) <= 4096;
+  for (int bc = 0; bc < batch && !(o.a && o.b); ++bc) {
+    const double2* cb = C + bc * sc;
+    if (!exact) {
+      o.a = o.a || blocks_overlap(A, 1, 1, (int)std::min<int64_t>((na - 1) * sa + (int64_t)(m - 1) * lda + k, INT32_MAX),
+                                  cb, 1, 1, (int)std::min<int64_t>((int64_t)(m - 1) * ldc + n, INT32_MAX));
+      o.b = o.b || blocks_overlap(B, 1, 1, (int)std::min<int64_t>((nb - 1) * sb + (int64_t)(k - 1) * ldb + n, INT32_MAX),
+                                  cb, 1, 1, (int)std::min<int64_t>((int64_t)(m - 1) * ldc + n, INT32_MAX));
+      continue;
+    }
+    for (int ba = 0; ba < na && !o.a; ++ba) o.a = blocks_overlap(A + ba * sa, lda, m, k, cb, ldc, m, n);
+    for (int bb = 0; bb < nb && !o.b; ++bb) o.b = blocks_overlap(B + bb * sb, ldb, k, n, cb, ldc, m, n);
+  }
+  return o;
+}
+
+// An in-place product is race-free only when every workgroup reads nothing
+// another one writes: C overlapping A needs tiles that span all n columns, C
+// overlapping B tiles that span all m rows.  Returns the chosen variant when
+// it qualifies, else the first variant of the kind that does, else -1.
+int safe_variant(int kind, int v, int m, int n, const Overlap& o) {
+  auto fits = [&](int vv) {
+    int bm, bn;
+    return tile_shape(kind, vv, bm, bn) && (!o.a || n <= bn) && (!o.b || m <= bm);
+  };
+  if (!o.a && !o.b) return v;
+  if (fits(v)) return v;
+  for (int vv : {1, 0, 2, 4, 3, 6})
+    if (fits(vv)) return vv;
+  return -1;
 }
 
 template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST = P::fast_loads>
@@ -393,16 +477,8 @@ void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A,
   const int tiles = tiles_m * tiles_n;
   // edge tiles may be shifted back inside C unless C overlaps an operand (an
   // in-place product: a shifted tile would read rows another tile writes)
-  auto span = [](const double2* p, int64_t stride, int nb, int rows, int64_t ld, int cols) {
-    const char* lo = (const char*)p;
-    const int64_t n = (nb - 1) * stride + (int64_t)(rows - 1) * ld + cols;
-    return std::make_pair(lo, lo + 16 * n);
-  };
-  const auto c = span(C, sc, batch, m, ldc, n);
-  const auto a = span(A, sa, batch, m, lda, k), b = span(B, sb, batch, k, ldb, n);
-  const bool overlap =
-      k > 0 && ((a.first < c.second && c.first < a.second) || (b.first < c.second && c.first < b.second));
-  const int shift = (m >= BM && n >= BN && !overlap) ? 1 : 0;
+  const Overlap o = overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch);
+  const int shift = (m >= BM && n >= BN && !o.a && !o.b) ? 1 : 0;
   hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB, FAST>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles,
                      shift);
@@ -488,7 +564,14 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev], h->stream));
   }
   const int kind = gemm_kind(h);
-  const int v = is_fp32_kind(kind) ? pick_variant32(kind, m, n, k, batch) : pick_variant(kind, m, n, k, batch);
+  int v = is_fp32_kind(kind) ? pick_variant32(kind, m, n, k, batch) : pick_variant(kind, m, n, k, batch);
+  v = safe_variant(kind, v, m, n, overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch));
+  if (v < 0) {
+    if (ev >= 0) h->ev_next -= 2;
+    set_error("zgemm: C overlaps an operand and no tile spans it (m %d, n %d): in-place products need n <= 128 "
+              "(C on A) or m <= 128 (C on B)", m, n);
+    return EUVL_ERR_ARG;
+  }
   launch_variant(h, kind, v, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch);
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
@@ -579,7 +662,12 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
       break;
     }
     const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-    for (int v = 0; v < N_VARIANTS && rc == EUVL_OK; ++v) {
+    for (int v = 0; v <= MAX_VARIANT && rc == EUVL_OK; ++v) {
+      int bm, bn;
+      if (!tile_shape(kind, v, bm, bn)) {
+        if (out_ms) out_ms[v] = -1.0;
+        continue;
+      }
       launch_variant(h, kind, v, m, n, k, one, A, k, (int64_t)m * k, B, n, (int64_t)k * n, zero, C, n,
                      (int64_t)m * n, zero, batch);
       const int reps = 3;

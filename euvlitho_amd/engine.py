@@ -101,7 +101,7 @@ class Engine:
         out = {}
         for fp32 in ((0, 1) if self.precision else (0,)):
             v = ctypes.c_int()
-            ms = (ctypes.c_double * 5)()
+            ms = (ctypes.c_double * 7)()
             self._bind_stream()
             _lib.check(self.lib.euvl_autotune_gemm(self.h, n, n, n, b, fp32, ctypes.byref(v), ms),
                        "euvl_autotune_gemm")

@@ -4,7 +4,9 @@ Runs the FP64 oracle (one N = 8705 eigenproblem plus the two N = 1009 blank
 solves) and compares with the reference fixtures ref_inputxx.csv (m3d.cpp
 output) and ref_emint.csv.gz (intensity.cpp output), both written by the
 reference with 6 significant digits.  Writes ref_default_e2e.json beside this
-file.  Usage: python tests/golden/ref_default_e2e.py  (about an hour on 8 cores).
+file and the oracle's arrays (a0, ax, ay, image) to ref_default_oracle.npz in
+the current directory.  Usage: python tests/golden/ref_default_e2e.py
+(about 12 minutes on 8 cores).
 """
 import gzip
 import json
@@ -27,7 +29,7 @@ def compare_inputxx(rows_ref, a0, ax, ay, xflag):
     return rel
 
 
-def run(golden=HERE, out=os.path.join(HERE, "ref_default_e2e.json")):
+def run(golden=HERE, out=os.path.join(HERE, "ref_default_e2e.json"), arrays="ref_default_oracle.npz"):
     t0 = time.time()
     p = P.REF_DEFAULT
     d = G.derived(p)
@@ -59,8 +61,8 @@ def run(golden=HERE, out=os.path.join(HERE, "ref_default_e2e.json")):
     )
     with open(out, "w") as f:
         json.dump(rep, f, indent=1)
-    np.savez_compressed(os.path.join(os.path.dirname(os.path.dirname(HERE)), "_work", "ref_default_oracle.npz"),
-                        a0=a0, ax=ax, ay=ay, img=img)
+    if arrays:
+        np.savez_compressed(arrays, a0=a0, ax=ax, ay=ay, img=img)
     return rep
 
 

@@ -252,6 +252,50 @@ def test_m3d_vs_oracle(ndiv, K):
         assert e < 1e-9
 
 
+def test_m3d_fit_recovers_linear_data_exactly():
+    """euvl_m3d_fit on amplitudes built as phase (a0 + ax ls + ay ms) per pupil
+    point (m3d.cpp:426-467): fit(Ax) - fit(0) returns (a0, ax, ay) where
+    snum >= 8 and a0 where snum < 8, to 1e-12; on random amplitudes it equals
+    numpy.linalg.lstsq on [1, ls, ms] (and the mean) to 1e-12."""
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from tests import m3d_synth as S
+    say("m3d fit on synthetic linear data, 512 nm, K=10")
+    cfg = c1_problem(10, 512)
+    eng = Engine(cfg)
+    bits = eng.upload_mask(manhattan(512, 512, 0.5, 4))
+    vc, _ = eng.blank_amplitudes()
+    phase = vc / np.where(vc == 0, 1, np.abs(vc))
+    lin, min_, snum = cfg.pupil
+    nsx, nsy = cfg.nsource
+    lsx, lsy = cfg.lsmax
+    geo = (lin, min_, eng.lidx, eng.midx, lsx, lsy, nsx, nsy, lambda l_, m_: bool(cfg.in_na(l_, m_)))
+    big = snum >= 8
+    assert big.any() and (~big).any()
+    rng = np.random.default_rng(9)
+    n = len(lin)
+    a = crand(rng, n)
+    x = np.where(big, crand(rng, n), 0)
+    y = np.where(big, crand(rng, n), 0)
+
+    def fit(Ax):
+        return [t.cpu().numpy() for t in eng.m3d(bits, Ax0=dev(Ax))]
+
+    zero = fit(np.zeros((nsx, nsy, len(eng.lidx)), complex))
+    got = fit(S.linear_amplitudes(*geo, phase, a, x, y))
+    for g, z, t in zip(got, zero, (a, x, y)):
+        e = np.abs((g - z) - t).max() / np.abs(t).max()
+        say(f"  exact recovery: max err {e:.2e}")
+        assert e < 1e-12
+    Ar = crand(rng, nsx, nsy, len(eng.lidx))
+    got = fit(Ar)
+    want = S.lstsq_fit(lin, min_, snum, eng.lidx, eng.midx, lsx, lsy, nsx, nsy, geo[-1], phase, Ar)
+    for g, z, t in zip(got, zero, want):
+        e = np.abs((g - z) - t).max() / np.abs(want[0]).max()
+        say(f"  vs lstsq: max err {e:.2e}")
+        assert e < 1e-12
+
+
 def test_abbe_c1_vs_oracle():
     from euvlitho_amd.engine import Engine
     from euvlitho_amd.masks import manhattan

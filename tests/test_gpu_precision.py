@@ -199,19 +199,26 @@ def test_autotune_keeps_results(lib):
     say(f"  tuned {tuned}")
     for key in ("fp64", "fp32"):
         v, ms = tuned[key]
-        assert 0 <= v < 5 and all(t > 0 for t in ms) and ms[v] == min(ms)
+        timed = [t for t in ms if t > 0]
+        # the 3M forms (the default) have variants 0-4 and 6; number 5 is not a variant
+        assert len(ms) == 7 and ms[5] == -1.0 and len(timed) == 6
+        assert v in (0, 1, 2, 3, 4, 6) and ms[v] == min(timed)
     after = eng.amp_s(bits, "X").cpu().numpy()
     e = np.abs(after - before).max() / np.abs(before).max()
     say(f"  amplitudes after tuning vs before: {e:.1e}")
     assert e < 1e-12
 
 
-@pytest.mark.parametrize("m,kn,batch", [(200, 48, 2), (1681, 64, 1), (130, 20, 3)])
+@pytest.mark.parametrize("m,kn,batch", [(200, 48, 2), (1681, 64, 1), (130, 20, 3), (200, 32, 2), (300, 100, 1)])
 def test_zgemm_in_place_rows(lib, m, kn, batch):
-    """C = A B with C the same storage as A (k = n <= one tile column, as the
-    LU's panel products use it): every workgroup reads only the rows it
-    overwrites, in both FP64 forms, and edge tiles are not moved onto rows
-    that another workgroup writes (1e-13 of the exact product)."""
+    """C = A B with C the same storage as A (k = n, as the LU's panel products
+    use it): the launcher keeps or picks a tile that spans all n columns, so
+    every workgroup reads only the rows it overwrites, and edge tiles are not
+    moved onto rows that another workgroup writes.  Small grids otherwise take
+    the 64 x 32 tile: n = 32 keeps it, n = 48 and 64 move to a 64-column tile,
+    n = 100 to a 128-column one.  Checked in the FP64 forms (3M, 4M) and the
+    FP32-compute 3M form, to 1e-13 (FP64) and 1e-5 (FP32) of the exact
+    product."""
     from euvlitho_amd import _lib
     say(f"in-place zgemm m={m} k=n={kn} batch={batch}")
     rng = np.random.default_rng(m + kn)
@@ -221,7 +228,7 @@ def test_zgemm_in_place_rows(lib, m, kn, batch):
     h = ctypes.c_void_p()
     assert lib.euvl_create(ctypes.byref(h)) == 0
     try:
-        for mask in (_lib.PREC_FP64, _lib.PREC_GEMM_4M):
+        for mask, bar in ((_lib.PREC_FP64, 1e-13), (_lib.PREC_GEMM_4M, 1e-13), (_lib.PREC_FP32_GEMM, 1e-5)):
             dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
             assert lib.euvl_set_precision(h, mask) == 0
             one, zero = (ctypes.c_double * 2)(1.0, 0.0), (ctypes.c_double * 2)(0.0, 0.0)
@@ -232,6 +239,29 @@ def test_zgemm_in_place_rows(lib, m, kn, batch):
             torch.cuda.synchronize()
             e = np.abs(dA.cpu().numpy() - ref).max() / np.abs(ref).max()
             say(f"  prec {mask}: rel err {e:.2e}")
-            assert e < 1e-13
+            assert e < bar
+    finally:
+        lib.euvl_destroy(h)
+
+
+def test_zgemm_in_place_too_wide_is_refused(lib):
+    """An in-place product wider than every tile (C on A with n = 200) would
+    race between workgroups: it is refused with EUVL_ERR_ARG and C is left
+    untouched."""
+    m, kn = 96, 200
+    rng = np.random.default_rng(7)
+    A = rng.standard_normal((m, kn)) + 1j * rng.standard_normal((m, kn))
+    B = rng.standard_normal((kn, kn)) + 1j * rng.standard_normal((kn, kn))
+    h = ctypes.c_void_p()
+    assert lib.euvl_create(ctypes.byref(h)) == 0
+    try:
+        dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+        one, zero = (ctypes.c_double * 2)(1.0, 0.0), (ctypes.c_double * 2)(0.0, 0.0)
+        rc = lib.euvl_zgemm_batched(h, m, kn, kn, one, ctypes.c_void_p(dA.data_ptr()), kn, 0,
+                                    ctypes.c_void_p(dB.data_ptr()), kn, 0, zero,
+                                    ctypes.c_void_p(dA.data_ptr()), kn, 0, None, 1)
+        assert rc == -1 and b"overlaps" in lib.euvl_last_error()
+        torch.cuda.synchronize()
+        assert np.array_equal(dA.cpu().numpy(), A)
     finally:
         lib.euvl_destroy(h)

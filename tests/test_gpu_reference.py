@@ -2,16 +2,22 @@
 outputs: m3d/inputxx.csv (M3D parameters, N = 8705 modes) and abbe/emint.csv
 (rigorous Abbe image, 512 x 512).
 
-Agreement level: the FP64 oracle (eigendecomposition route, LAPACK) and this
-GPU path (matrix-function route) agree with each other to ~1e-12 on this
-problem, and both reproduce the fixtures to 0.8 % (Re a0), 0.5 % (Im a0),
-0.4 % (ax), 3.6 % (ay, the smallest row) of each row's maximum and 2.5 % of
-the image peak, against ~1e-6 expected from the fixtures' 6 significant
-digits.  Every physical parameter perturbation tried (incidence angle,
-absorber index and thickness, pair count, defocus plane) moves the result
-away from the fixtures by much more, so the committed parameters are the
-right ones; the residual is an open parity question.  The bars below are the
-measured agreement plus margin, so that a regression shows.
+The fixtures carry 6 significant digits, so the bar is 2e-5 of each row's
+maximum (and of the image peak).  This build does not meet it: the FP64 oracle
+(eigendecomposition route, LAPACK) and this GPU path (matrix-function route)
+agree with each other to ~1e-12 on this problem, and both miss the fixtures by
+0.83 % (Re a0), 0.46 % (Im a0), 0.35 % / 0.26 % (ax), 3.5 % / 3.6 % (ay) of
+the row maxima and 2.5 % of the image peak.  The test is marked as an
+expected failure (strict), so the gap shows in every report, and an
+agreement reached by a later fix turns it into an unexpected pass.
+
+What the residual looks like: divided by the thin-mask shadow phase, the
+a0 difference is the spectrum of a feature at the mask's y wrap (pattern rows
+2047/0), full width in x (the l = 0 column carries most of it) and, off that
+column, 75 % correlated with the mask's own boundary row; a single-row model
+at the wrap explains 65 % of its energy.  Variants ruled out on this GPU path:
+the commented-out 0.5 nm grid of the reference drivers, a truncated or
+zero-padded mask file, flips, transposes, shifts and incidence-plane changes.
 """
 import gzip
 import os
@@ -23,8 +29,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-ROW_BARS = [1.2e-2, 7e-3, 5e-3, 4e-3, 4.5e-2, 4.5e-2]
-IMAGE_BAR = 3e-2
+# 6 significant digits in the fixtures
+ROW_BAR = 2e-5
+IMAGE_BAR = 2e-5
+MEASURED = ("inputxx rows miss by 8.3e-3 / 4.6e-3 / 3.5e-3 / 2.6e-3 / 3.5e-2 / 3.6e-2 of the row maxima and "
+            "emint.csv by 2.5e-2 of peak (oracle and GPU agree to 1e-12); the cause is open")
 
 
 def read_rows(path):
@@ -37,6 +46,7 @@ def read_rows(path):
     return rows
 
 
+@pytest.mark.xfail(strict=True, reason=MEASURED)
 def test_reference_default_matches_fixtures():
     from euvlitho_amd.config import reference_default
     from euvlitho_amd.engine import Engine
@@ -53,11 +63,12 @@ def test_reference_default_matches_fixtures():
     got = [a0.real, a0.imag, ax[x].real, ax[x].imag, ay[x].real, ay[x].imag]
     ref = read_rows(os.path.join(GOLDEN, "ref_inputxx.csv"))
     assert [len(r) for r in ref] == [len(g) for g in got]
-    for i, (g, r, bar) in enumerate(zip(got, ref, ROW_BARS)):
+    errs = []
+    for i, (g, r) in enumerate(zip(got, ref)):
         e = np.abs(g - r).max() / np.abs(r).max()
         c = np.corrcoef(g, r)[0, 1]
         print(f"  inputxx row {i}: max err / row max = {e:.3e}, correlation {c:.6f}", flush=True)
-        assert e < bar and c > 0.998
+        errs.append(e)
     img = eng.abbe(bits, Ax=Ax).cpu().numpy()
     with gzip.open(os.path.join(GOLDEN, "ref_emint.csv.gz"), "rt") as f:
         rows = [np.array([float(v) for v in line.strip().split(",") if v]) for line in f if line.strip()]
@@ -66,4 +77,4 @@ def test_reference_default_matches_fixtures():
     e = np.abs(img - ref_img).max() / ref_img.max()
     c = np.corrcoef(img.ravel(), ref_img.ravel())[0, 1]
     print(f"  emint max err / peak = {e:.3e}, correlation {c:.6f}", flush=True)
-    assert e < IMAGE_BAR and c > 0.9995
+    assert max(errs) < ROW_BAR and e < IMAGE_BAR

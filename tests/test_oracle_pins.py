@@ -225,6 +225,67 @@ def test_x_mirror_symmetry():
 
 
 # ------------------------------------------------------------------- imaging
+def _fit_setup(seed=5):
+    from tests import m3d_synth as S
+    p, d = small(512, 6)
+    li, mi, L, M = G.order_set(p, "rect", K=6)
+    lin, min_, snum = G.pupil_points(p, d)
+    rng = np.random.default_rng(seed)
+    nsx, nsy = d["nsourceX"], d["nsourceY"]
+    vc = rng.normal(size=(nsx, nsy)) + 1j * rng.normal(size=(nsx, nsy))
+    ab = 0.3 * (rng.normal(size=(nsx, nsy)) + 1j * rng.normal(size=(nsx, nsy)))
+    m = rand_mask(512, 512, 17)
+    in_na = lambda l_, m_: G.in_na(l_, m_, p, d)
+    geo = (lin, min_, li, mi, d["lsmaxX"], d["lsmaxY"], nsx, nsy, in_na)
+    return S, p, d, li, mi, lin, min_, snum, vc, ab, m, geo, rng
+
+
+def test_m3d_fit_recovers_linear_data_exactly():
+    """Ax built as phase (a0 + ax ls + ay ms) per pupil point: the fit returns
+    (a0, ax, ay) where snum >= 8 (3x3 normal equations by Cramer's rule,
+    m3d.cpp:451-457) and a0 where snum < 8 (the mean, m3d.cpp:459-464), to
+    rounding.  fit(0) carries the thin-mask term and is subtracted."""
+    S, p, d, li, mi, lin, min_, snum, vc, ab, m, geo, rng = _fit_setup()
+    n = len(lin)
+    big = snum >= 8
+    assert big.any() and (~big).any()  # both branches are exercised
+    # The valid sources of a pupil point are symmetric under the point
+    # reflection that swaps the two NA disks, so sum(ls) = sum(ms) = 0 exactly
+    # and the cofactor terms carrying cx or cy never reach an output; the
+    # cross moment sum(ls ms) is not zero, so its terms are exercised.
+    cxy = []
+    for lp, mp in zip(lin, min_):
+        v = np.array([s[2:] for s in S.valid_sources(int(lp), int(mp), *geo[4:])])
+        assert v[:, 0].sum() == 0 and v[:, 1].sum() == 0
+        cxy.append(abs((v[:, 0] * v[:, 1]).sum()))
+    assert max(cxy) > 0
+    a = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x = np.where(big, rng.normal(size=n) + 1j * rng.normal(size=n), 0)
+    y = np.where(big, rng.normal(size=n) + 1j * rng.normal(size=n), 0)
+    phase = vc / np.abs(vc)
+    Ax = S.linear_amplitudes(*geo, phase, a, x, y)
+    got = OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
+    zero = OA.m3d_params(p, d, m, li, mi, np.zeros_like(Ax), vc, ab)
+    for g, z, t in zip(got, zero, (a, x, y)):
+        assert np.abs((g - z) - t).max() < 1e-12 * np.abs(t).max()
+    assert np.abs(got[1][~big]).max() == 0 and np.abs(got[2][~big]).max() == 0
+
+
+def test_m3d_fit_equals_lstsq():
+    """On random amplitudes the fit equals numpy.linalg.lstsq on [1, ls, ms]
+    (snum >= 8) and the mean (snum < 8)."""
+    S, p, d, li, mi, lin, min_, snum, vc, ab, m, geo, rng = _fit_setup(6)
+    nsx, nsy = d["nsourceX"], d["nsourceY"]
+    Ax = rng.normal(size=(nsx, nsy, len(li))) + 1j * rng.normal(size=(nsx, nsy, len(li)))
+    phase = vc / np.abs(vc)
+    got = OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
+    zero = OA.m3d_params(p, d, m, li, mi, np.zeros_like(Ax), vc, ab)
+    want = S.lstsq_fit(lin, min_, snum, li, mi, d["lsmaxX"], d["lsmaxY"], nsx, nsy,
+                       lambda l_, m_: G.in_na(l_, m_, p, d), phase, Ax)
+    for g, z, t in zip(got, zero, want):
+        assert np.abs((g - z) - t).max() < 1e-12 * np.abs(want[0]).max()
+
+
 def test_abbe_grid_refinement_subsamples():
     p, d = small(256, 4)
     li, mi, L, M = G.order_set(p, "rect", K=4)
@@ -253,17 +314,16 @@ def test_abbe_blank_mask_uniform():
 
 # ------------------------------------------------------------ end to end (slow)
 @pytest.mark.slow
-def test_reference_default_end_to_end(golden):
+@pytest.mark.xfail(strict=True, reason="the oracle misses inputxx.csv by 0.26-3.6 % of the row maxima and "
+                                       "emint.csv by 2.5 % of peak (tests/golden/ref_default_e2e.json); open")
+def test_reference_default_end_to_end(golden, tmp_path):
     """Oracle at the reference-default optics vs m3d/inputxx.csv and
-    abbe/emint.csv (one N = 8705 eigenproblem; ~15 min on 6 cores).  Measured
-    agreement (tests/golden/ref_default_e2e.json): 0.8 % / 0.5 % / 0.4 % /
-    0.3 % / 3.5 % / 3.6 % of the row maxima, 2.5 % of the image peak; the
-    cause of the residual is open (see tests/test_gpu_reference.py)."""
+    abbe/emint.csv (one N = 8705 eigenproblem; ~15 min on 6 cores), at the
+    2e-5 bar that the fixtures' 6 significant digits support."""
     from tests.golden.ref_default_e2e import run
-    rep = run(golden)
-    bars = [1.2e-2, 7e-3, 5e-3, 4e-3, 4.5e-2, 4.5e-2]
-    assert all(e < b for e, b in zip(rep["inputxx_rel_per_row"], bars))
-    assert rep["emint_max_rel"] < 3e-2
+    rep = run(golden, out=str(tmp_path / "ref_default_e2e.json"), arrays=str(tmp_path / "ref_default_oracle.npz"))
+    assert all(e < 2e-5 for e in rep["inputxx_rel_per_row"])
+    assert rep["emint_max_rel"] < 2e-5
 
 
 # ----------------------------------------------- images from M3D parameters
