@@ -4,8 +4,8 @@ Workload (one step = one clip, FP64): a seeded 1024 x 1024 nm Manhattan clip
 (1 nm pixels, rectangles 16-64 nm, vacuum fraction 0.5), NA 0.55, 4x/8x,
 CRAO 5.355 deg, 60 nm absorber n = 0.947+0.031i on 40 x Mo/Si 2.8/4.2 nm with
 a 2.5 nm Ru cap, orders +-20 x +-20 (N = 1681 modes), X-dipole (sigma 0.5-0.9,
-60 deg) + annular (sigma 0.7-0.8) source on 2 x 2 lattice divisions
-(190 source points, 4 EM solves per clip), outputs: M3D parameters, the
+70 deg) + annular (sigma 0.7-0.8) source: 200 points of the 2 x 2 lattice
+divisions (4 EM solves per clip), outputs: M3D parameters, the
 rigorous Abbe image (256 x 256) and the STCC-SOCS image from the M3D
 parameters (24 kernels, 20 of them with the x/y slope terms, nout 64).  The
 SOCS kernels (TCC eigendecomposition on the 0.2 degree source mesh) are
@@ -13,9 +13,18 @@ mask-independent: built once, resident on the device, timed separately.
 
 Prints one JSON line: metric, value = clips/s of the plain FP64 form,
 roofline of the dominant kernel (the FP64 MFMA complex GEMM), cpu_baseline =
-the FP64 NumPy/LAPACK oracle on one clip of the same workload, and under
-"forms" the precision form (FP32 MFMA products, FP64 storage and the rest)
-timed on the same clips with its error against the FP64 form.
+the FP64 NumPy/LAPACK oracle on clips of the same workload, and under
+"forms" the precision form timed on the same clips with its error against
+the FP64 form, the batched and served forms over the uneven-clip mix, and
+the reference-default m3d.cpp path (one mask: the N = 8705 solve, the two
+N = 1009 blank solves and the fit).
+
+Roofline accounting of the EM solve: the algorithmic work is the
+reference algorithm's 164 N^3 flops per (clip, polarisation, offset) solve
+(5 ZGEMM + 3 inverses + one ZGEEV with right vectors at 100 N^3); achieved =
+that work over the summed duration of the GEMM launches of one clip (HIP
+events on the library's stream).  The GEMM kernel's own executed rate
+(8mnk per complex product) is reported beside it.
 """
 import argparse
 import json
@@ -29,6 +38,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+EM_FLOPS_PER_N3 = 164.0  # reference algorithm per (clip, polarisation, offset) solve
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense FP32 matrix
 SOCS_KERNELS, SOCS_XY = 24, 20
 
@@ -93,22 +103,25 @@ def workload_config():
     return preset("c2")
 
 
-def cpu_baseline(cfg, masks, threads):
-    """Oracle (NumPy + LAPACK, FP64) on one whole clip of the workload: the EM
-    solves of every source division, the Abbe accumulation over every source
-    point and the M3D fit.  The blank solves are done first, untimed, as the
-    GPU path keeps them resident; the SOCS image (24 + 5 x 20 small IFFTs,
-    well under 1 % of the clip on the CPU) is left out."""
+def cpu_clip(cfg, m):
+    """Oracle (NumPy + LAPACK, FP64) on one whole clip: the EM solves of every
+    source division, the Abbe accumulation over every source point and the
+    M3D fit; returns (seconds, (em, abbe, fit) seconds).  The blank solves are
+    done before, untimed, as the GPU path keeps them resident; the SOCS image
+    (24 + 5 x 20 small IFFTs, well under 1 % of the clip on the CPU) is left
+    out."""
     from oracle import abbe as OA, em as OE, geometry as G, problems as OP
     p = OP.hna(cfg.ndivx, cfg.orders[1], None)
     d = G.derived(p)
     li, mi, L, M = cfg.order_set()
     li, mi = li.astype(np.int64), mi.astype(np.int64)
-    m = masks[0]
     off = cfg.offsets()
     divs = cfg.source_points()
     nx, ny = cfg.image_shape
-    vc, ab = OA.blank_amplitudes(p, d)
+    key = (cfg.ndivx, cfg.orders)
+    if key not in _BLANK:
+        _BLANK[key] = OA.blank_amplitudes(p, d)
+    vc, ab = _BLANK[key]
     t0 = time.perf_counter()
     amps = [OE.amp_s("X", m, p, d, li, mi, L, M, o[0], o[1]) for o in off]
     t1 = time.perf_counter()
@@ -117,11 +130,79 @@ def cpu_baseline(cfg, masks, threads):
     t2 = time.perf_counter()
     OA.m3d_params(p, d, m, li, mi, amps[0], vc, ab)
     t3 = time.perf_counter()
-    n_src = sum(len(x[2]) for x in divs)
-    return dict(value=1.0 / (t3 - t0), unit="clips/s", cores=threads, kind="port",
-                sample=f"one whole clip (seed 0): oracle EM solves of {len(off)} divisions ({t1 - t0:.1f} s, "
-                       f"N={len(li)}), Abbe over {n_src} source points ({t2 - t1:.1f} s), M3D fit "
-                       f"({t3 - t2:.1f} s); {t3 - t0:.1f} s per clip; LAPACK threads = cores")
+    return t3 - t0, (t1 - t0, t2 - t1, t3 - t2)
+
+
+_BLANK = {}
+
+
+def cpu_baseline(cfg, masks, threads, reps=3):
+    """The oracle on `reps` clips of the workload (median), and on `reps`
+    clips of the 512 nm / +-10-order / 61-point problem beside it."""
+    from euvlitho_amd.config import preset
+    from euvlitho_amd.masks import manhattan
+    ts = [cpu_clip(cfg, masks[i % len(masks)]) for i in range(reps)]
+    t = float(np.median([x[0] for x in ts]))
+    em, ab, fit = ts[0][1]
+    c1 = preset("c1")
+    t1 = [cpu_clip(c1, manhattan(c1.ndivx, c1.ndivy, 0.5, s))[0] for s in range(reps)]
+    n_src = sum(len(x[2]) for x in cfg.source_points())
+    return dict(value=1.0 / t, unit="clips/s", cores=threads, kind="port",
+                sample=f"{reps} whole clips (seeds 0-{reps - 1}), median {t:.1f} s per clip: oracle EM solves of "
+                       f"{len(cfg.offsets())} divisions ({em:.1f} s, N={len(cfg.order_set()[0])}), Abbe over "
+                       f"{n_src} source points ({ab:.1f} s), M3D fit ({fit:.1f} s); LAPACK threads = cores",
+                runs_s=[x[0] for x in ts],
+                c1={"value": 1.0 / float(np.median(t1)), "unit": "clips/s", "runs_s": t1,
+                    "sample": "512 nm clip, +-10 orders (N=441), 61 annular points, 4 divisions, Abbe 256x256"})
+
+
+def ref_default_leg(reps=3):
+    """The reference's own m3d.cpp path on its mask (tests/golden/ref_mask.bin,
+    2048 nm, NA 0.33, 4x/4x): the two blank solves on the narrow ellipse set
+    (N = 1009), the wide solve on the diamond set (N = 8705) and the M3D fit,
+    per mask, FP64 -- the run that the reference's README quotes at about
+    one minute per mask.  Roofline: 164 N^3 per solve over the GEMM time of
+    the run (HIP events), and over its wall time."""
+    import ctypes
+    import torch
+    from euvlitho_amd import _lib
+    from euvlitho_amd.config import reference_default
+    from euvlitho_amd.engine import Engine, Resident
+    cfg = reference_default()
+    eng = Engine(cfg)
+    with open(os.path.join(ROOT, "tests", "golden", "ref_mask.bin"), "rb") as f:
+        bits = eng.upload_mask(f.read())
+
+    def run():
+        eng.resident = Resident()  # the reference solves the blank masks for every mask
+        Ax = eng.amp_s(bits, "X")
+        return eng.m3d(bits, Ax0=Ax[0])
+
+    run()
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    stats = (ctypes.c_double * 8)()
+    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
+    run()
+    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
+    N, Nb = len(eng.lidx), len(cfg.blank_order_set()[0])
+    eng.close()
+    wall = float(np.median(walls))
+    flops = EM_FLOPS_PER_N3 * (float(N) ** 3 + 2.0 * float(Nb) ** 3)
+    ach = flops / (stats[1] * 1e-3) / 1e12
+    return {"value": 1.0 / wall, "unit": "masks/s", "s_per_mask": wall, "runs_s": walls,
+            "config": f"reference default: 2048 nm mask (tests/golden/ref_mask.bin), N={N} wide solve + 2 blank "
+                      f"solves N={Nb} + M3D fit, X polarisation, FP64",
+            "published": "~1 min per mask (reference README.md:32, hardware not stated)",
+            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": ach,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                         "flops_counted": "164 N^3 per solve (3 solves)", "gemm_ms_per_mask": stats[1],
+                         "gemm_launches": int(stats[0]), "whole_run_tflops": flops / wall / 1e12}}
 
 
 def mask_producer(n_masks=64, ndiv=2048, reps=5, with_cpu=True):
@@ -321,7 +402,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only (profiling runs)")
-    ap.add_argument("--served", type=int, default=200,
+    ap.add_argument("--served", type=int, default=2000,
                     help="requests of the served Poisson stream (0 = skip; 2000 = the full stream)")
     ap.add_argument("--slots", type=int, default=4, help="streams of the served form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -330,6 +411,7 @@ def main():
     ap.add_argument("--no-cnn", action="store_true", help="skip the CNN predictor leg")
     ap.add_argument("--cnn-only", action="store_true", help="only run the CNN predictor leg")
     ap.add_argument("--no-stcc", action="store_true", help="skip the TCCEE image leg")
+    ap.add_argument("--no-ref-default", action="store_true", help="skip the reference-default m3d.cpp leg")
     ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -406,7 +488,11 @@ def main():
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
     ms = 1000.0 * f64["wall"] / args.steps
     launches, gemm_ms, gemm_flops, gemm_bytes = f64["launches"], f64["gemm_ms"], f64["gemm_flops"], f64["gemm_bytes"]
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    executed = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    n_solves = len(cfg.offsets())  # X polarisation, one solve per source division
+    N = len(cfg.order_set()[0])
+    algo_flops = EM_FLOPS_PER_N3 * float(N) ** 3 * n_solves
+    achieved = algo_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     traffic = measured_traffic()
     lat = f64["lat"]
     mixed = None
@@ -458,22 +544,28 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded Manhattan clips)",
         "config": {"workload": "plain FP64 drop-in, 1024x1024 nm clip, +-20 orders (N=1681), dipole+annular "
-                               "190 source pts on 2x2 divisions (4 EM solves), M3D params + Abbe 256x256 + "
+                               "200 source pts on 2x2 divisions (4 EM solves), M3D params + Abbe 256x256 + "
                                "24-kernel SOCS 256x256",
                    "form": "fp64", "clips": args.steps, "socs_kernel_build_ms": tcc_ms, "em_solve": f64["em"]},
         "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": achieved,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                     "flops_counted": "reference algorithm: 164 N^3 per (clip, polarisation, offset) solve "
+                                      f"x {n_solves} solves = {algo_flops:.3e} per clip, over the clip's GEMM time",
+                     "algorithmic_flops_per_launch": algo_flops / max(launches, 1),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
                      "traffic_source": traffic.get("source"),
                      "algorithmic_bytes_per_launch": gemm_bytes / max(launches, 1),
+                     "traffic_over_algorithmic": (traffic["traffic_bytes_per_launch"] / (gemm_bytes / launches)
+                                                  if traffic.get("traffic_bytes_per_launch") and launches else None),
                      "launches_per_clip": int(launches),
                      "avg_launch_us": 1000.0 * gemm_ms / max(launches, 1),
                      "gemm_ms_per_clip": gemm_ms, "gemm_share_of_step": gemm_ms / ms,
                      "autotuned_variant": f64["tuned"]["fp64"][0],
-                     # achieved counts 8mnk flops per complex product; the 3M form issues 6mnk on the MFMAs
-                     "flops_counted": "8mnk per complex GEMM",
+                     # the route's own GEMMs: 8mnk per complex product, of which the 3M form issues 6mnk
+                     "gemm_executed_8mnk_tflops": executed,
+                     "gemm_executed_8mnk_frac": executed / FP64_PEAK_TFLOPS,
                      "complex_form": "3M (3 real MFMAs per complex product)",
-                     "mfma_issue_frac": 0.75 * achieved / FP64_PEAK_TFLOPS},
+                     "mfma_issue_frac": 0.75 * executed / FP64_PEAK_TFLOPS},
         "forms": {"mixed": mixed} if mixed else {},
     }
     if args.served > 0:
@@ -484,6 +576,8 @@ def main():
         line["cnn_predictor"] = cnn_predictor(with_cpu=not args.no_cpu_baseline)
     if not args.no_stcc:
         line["tccee_image"] = stcc_leg(with_cpu=not args.no_cpu_baseline)
+    if not args.no_ref_default:
+        line["reference_default_m3d"] = ref_default_leg()
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
         line["cpu_baseline"] = cpu_baseline(cfg, masks, threads)

@@ -63,6 +63,10 @@ class Source:
     open_angle: float = 90.0
     extra: tuple = ()
     ndivs: int = 0  # 0: reference rule max(1, int(180/pi lambda/dx/0.5)) (intensity.cpp:90-91)
+    # > 0: keep exactly this many of the rule's lattice points (all divisions
+    # pooled), spread evenly in azimuth; the explicit point list of a workload
+    # whose point count is given (e.g. 61 or 200)
+    count: int = 0
 
     def members(self):
         return ((self.kind, self.sigma_out, self.sigma_in, self.open_angle),) + tuple(self.extra)
@@ -236,6 +240,30 @@ class Config:
                     elif t == 3:
                         ok |= ring & (np.abs(sxo) <= np.abs(syo) * math.tan(PI * oa / 180.0 / 2.0))
                 out.append((nsx, nsy, l[ok].astype(np.int32), m[ok].astype(np.int32)))
+        if self.source.count > 0:
+            out = self._thin(out, self.source.count)
+        return out
+
+    def _thin(self, divs, count):
+        """Exactly ``count`` of the pooled points: sorted by azimuth (then
+        radius) of their pupil position, every (total/count)-th is kept."""
+        nd = self.ndivs
+        pts = []
+        for q, (nsx, nsy, l0, m0) in enumerate(divs):
+            sx = (l0 + nsx / float(nd)) * self.mx / self.dx
+            sy = (m0 + nsy / float(nd)) * self.my / self.dy
+            for a, b, x, y in zip(l0, m0, sx, sy):
+                pts.append((math.atan2(y, x), x * x + y * y, q, int(a), int(b)))
+        if count > len(pts):
+            raise ValueError(f"source rule gives {len(pts)} points, fewer than the {count} asked for")
+        pts.sort()
+        keep = [pts[int((i + 0.5) * len(pts) / count)] for i in range(count)]
+        out = []
+        for q, (nsx, nsy, _, _) in enumerate(divs):
+            sel = sorted((a, b) for _, _, qq, a, b in keep if qq == q)
+            l0 = np.array([a for a, _ in sel], dtype=np.int32)
+            m0 = np.array([b for _, b in sel], dtype=np.int32)
+            out.append((nsx, nsy, l0, m0))
         return out
 
     def mesh_source(self, mesh_deg=0.2):
@@ -289,15 +317,33 @@ def high_na(ndiv, K, source, ndivs=0, image=(0, 0)):
                   image=image)
 
 
-ANNULAR_C1 = dict(kind=1, sigma_out=0.8, sigma_in=0.3)
-DIPOLE_ANNULAR_C2 = dict(kind=2, sigma_out=0.9, sigma_in=0.5, open_angle=60.0, extra=((1, 0.8, 0.7, 90.0),))
+ANNULAR = dict(kind=1, sigma_out=0.8, sigma_in=0.3)
+DIPOLE_ANNULAR = dict(kind=2, sigma_out=0.9, sigma_in=0.5, open_angle=70.0, extra=((1, 0.8, 0.7, 90.0),))
+# annular 0.3-0.8 at 61 points; X dipole 0.5-0.9 / 70 deg + annular 0.7-0.8 at 200 points
+ANNULAR_C1 = dict(ANNULAR, count=61)
+DIPOLE_ANNULAR_C2 = dict(DIPOLE_ANNULAR, count=200)
+
+
+def workload_mix():
+    """The problems of the uneven-clip workloads (batched and served forms):
+    pitch 256-1024 nm, order truncation +-8..+-20 growing with the pitch,
+    annular or dipole + annular sources of 61-200 points."""
+    return [
+        high_na(256, 8, dict(ANNULAR, count=61), ndivs=4, image=(64, 64)),
+        high_na(512, 12, dict(ANNULAR, count=80), ndivs=2, image=(128, 128)),
+        high_na(512, 12, dict(DIPOLE_ANNULAR, count=100), ndivs=3, image=(128, 128)),
+        high_na(768, 16, dict(ANNULAR, count=150), ndivs=2, image=(192, 192)),
+        preset("c2"),
+        high_na(1024, 20, dict(ANNULAR, count=61), ndivs=1, image=(256, 256)),
+    ]
 
 
 def preset(name):
     """Named problems: 'reference' (the reference drivers' own), 'c1' (512 nm
-    clip, +-10 orders, annular 0.3-0.8 on 2 x 2 divisions, 256 x 256 image) and
-    'c2' (1024 nm clip, +-20 orders, X-dipole 0.5-0.9 / 60 deg + annular
-    0.7-0.8 on 2 x 2 divisions, 256 x 256 image)."""
+    clip, +-10 orders, annular 0.3-0.8 at 61 points of the 2 x 2 division
+    lattice, 256 x 256 image) and 'c2' (1024 nm clip, +-20 orders, X dipole
+    0.5-0.9 / 70 deg + annular 0.7-0.8 at 200 points of the 2 x 2 division
+    lattice, 256 x 256 image)."""
     if name == "reference":
         return reference_default()
     if name == "c1":

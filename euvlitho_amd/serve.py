@@ -31,7 +31,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .config import Config, preset, high_na, ANNULAR_C1, DIPOLE_ANNULAR_C2
+from .config import Config, workload_mix
 from .engine import Engine, Resident
 
 MIB = 1 << 20
@@ -62,17 +62,10 @@ class ClipResult:
 
 
 def served_configs():
-    """The problem mix of the served stream: pitch 256-1024 nm with order
-    truncation +-8..+-20 growing with the pitch, annular or dipole+annular
-    sources on lattices of 61-200 points."""
-    return [
-        high_na(256, 8, ANNULAR_C1, ndivs=4, image=(64, 64)),
-        high_na(512, 12, ANNULAR_C1, ndivs=2, image=(128, 128)),
-        high_na(512, 12, DIPOLE_ANNULAR_C2, ndivs=3, image=(128, 128)),
-        high_na(768, 16, DIPOLE_ANNULAR_C2, ndivs=2, image=(192, 192)),
-        preset("c2"),
-        high_na(1024, 20, ANNULAR_C1, ndivs=1, image=(256, 256)),
-    ]
+    """The problem mix of the served stream (config.workload_mix): pitch
+    256-1024 nm with order truncation +-8..+-20 growing with the pitch,
+    annular or dipole+annular sources of 61-200 points."""
+    return workload_mix()
 
 
 def footprint(cfg: Config):
