@@ -44,6 +44,17 @@ class StccParams(ctypes.Structure):
     ]
 
 
+class PlanDesc(ctypes.Structure):
+    _fields_ = [
+        ("polar", c_char), ("p", ctypes.POINTER(EmParams)), ("h_lindex", P_int), ("h_mindex", P_int),
+        ("nrange", c_int), ("lmax", c_int), ("mmax", c_int), ("n_off", c_int), ("h_off", P_double),
+        ("n_clips", c_int), ("kx0", c_double), ("ky0", c_double), ("z0", c_double),
+        ("ncut", c_int), ("h_pupil3", P_int), ("lpmax_x", c_int), ("lpmax_y", c_int), ("h_vc", P_double),
+        ("h_ab", P_double), ("n_src", c_int), ("h_src", P_int), ("z", c_double), ("nx", c_int), ("ny", c_int),
+        ("weight", c_double),
+    ]
+
+
 # name -> (argtypes); every entry point returns int
 _SIGS = {
     "euvl_create": [ctypes.POINTER(c_void_p)],
@@ -92,6 +103,18 @@ _SIGS = {
     "euvl_cnn_linear": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p],
     "euvl_stcc_image": [c_void_p, c_char, ctypes.POINTER(StccParams), c_void_p, P_double, P_double, P_double, c_void_p,
                         c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p],
+    "euvl_plan_create": [c_void_p, ctypes.POINTER(PlanDesc), ctypes.POINTER(c_void_p)],
+    "euvl_plan_destroy": [c_void_p],
+    "euvl_plan_workspace_bytes": [c_void_p],
+    "euvl_plan_reserve": [c_void_p, c_int, ctypes.POINTER(c_void_p)],
+    "euvl_plan_run": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int],
+    "euvl_plan_decisions": [c_void_p, P_int, P_int],
+    "euvl_plan_set_decisions": [c_void_p, c_int, c_int],
+    "euvl_graph_begin": [c_void_p],
+    "euvl_graph_end": [c_void_p, ctypes.POINTER(c_void_p)],
+    "euvl_graph_launch": [c_void_p, c_void_p],
+    "euvl_graph_nodes": [c_void_p, ctypes.POINTER(c_int64)],
+    "euvl_graph_destroy": [c_void_p],
     "euvl_augment_batch": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
 }
@@ -112,7 +135,7 @@ def load():
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = ctypes.c_int64 if name == "euvl_workspace_bytes" else c_int
+        fn.restype = ctypes.c_int64 if name in ("euvl_workspace_bytes", "euvl_plan_workspace_bytes") else c_int
     lib.euvl_last_error.argtypes = []
     lib.euvl_last_error.restype = ctypes.c_char_p
     _lib = lib

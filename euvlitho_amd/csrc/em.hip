@@ -179,9 +179,11 @@ __global__ void mode_vectors_kernel(int n, int batch, char polar, double k, doub
 
 // D and G = diag(kt) sigma^ diag(kt) / k^2 from the indicator spectrum W
 // (window |dl| <= 2 lmax, |dm| <= 2 mmax).
+// Batch entry b reads the spectrum W + (b / per_clip) * w_stride (the
+// offsets of one clip share its spectrum).
 __global__ void toeplitz_kernel(int n, int batch, char polar, double k, double dx, double dy, const int* lidx,
-                                const int* midx, int lmax, int mmax, const double2* W, double2 ea, double2 ea_inv,
-                                double2 log_ea, ModeVecs v, double2* D, double2* G) {
+                                const int* midx, int lmax, int mmax, const double2* W, int per_clip, int64_t w_stride,
+                                double2 ea, double2 ea_inv, double2 log_ea, ModeVecs v, double2* D, double2* G) {
   const int64_t per = (int64_t)n * n;
   const int64_t total = per * batch;
   const int wm = 4 * mmax + 1;
@@ -190,7 +192,7 @@ __global__ void toeplitz_kernel(int n, int batch, char polar, double k, double d
     const int64_t r = e - (int64_t)b * per;
     const int i = (int)(r / n), j = (int)(r % n);
     const int dl = lidx[i] - lidx[j], dm = midx[i] - midx[j];
-    const double2 mh = W[(int64_t)(dl + 2 * lmax) * wm + (dm + 2 * mmax)];
+    const double2 mh = W[(int64_t)(b / per_clip) * w_stride + (int64_t)(dl + 2 * lmax) * wm + (dm + 2 * mmax)];
     const bool z = (dl == 0 && dm == 0);
     // eps^ = ea delta + (1-ea) m^ ; log-eps^ = log(ea)(delta - m^) ; sigma^ = ea^-1 delta + (1-ea^-1) m^
     double2 eps = cmul(make_double2(1.0 - ea.x, -ea.y), mh);
@@ -304,14 +306,22 @@ int dft_splits(int nx, int ny, int mw, int n_clips) {
   return S;
 }
 
-int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw, double2* out,
-                  double2* scratch /* n_clips*nx*(2mw+1) */, double2* part /* S*n_clips*nx*(2mw+1), S > 1 */,
-                  double2* tw /* nx+1 + ny+1 */) {
-  std::vector<std::complex<double>> t(nx + ny + 2);
+// twiddle tables of exponential() (header.h:1637-1648): nx + 1 entries for x, then ny + 1 for y
+void dft_twiddles(int nx, int ny, std::vector<double2>& t) {
+  t.resize(nx + ny + 2);
   const std::complex<double> zi(0.0, 1.0);
-  for (int i = 0; i <= nx; ++i) t[i] = std::exp(-2.0 * PI * zi * double(i) / double(nx));       // exponential()
-  for (int i = 0; i <= ny; ++i) t[nx + 1 + i] = std::exp(-2.0 * PI * zi * double(i) / double(ny));
-  EUVL_HIP_CHECK(hipMemcpyAsync(tw, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice, h->stream));
+  for (int i = 0; i <= nx; ++i) {
+    const std::complex<double> e = std::exp(-2.0 * PI * zi * double(i) / double(nx));
+    t[i] = make_double2(e.real(), e.imag());
+  }
+  for (int i = 0; i <= ny; ++i) {
+    const std::complex<double> e = std::exp(-2.0 * PI * zi * double(i) / double(ny));
+    t[nx + 1 + i] = make_double2(e.real(), e.imag());
+  }
+}
+
+int mask_spectrum_enqueue(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
+                          double2* out, double2* scratch, double2* part, const double2* tw) {
   const int nm = 2 * mw + 1;
   const int S = dft_splits(nx, ny, mw, n_clips);
   const int ks = ((ny + S - 1) / S + 3) / 4 * 4;
@@ -328,6 +338,16 @@ int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clip
   hipLaunchKernelGGL(mask_dft_x_kernel, dim3((unsigned)n2), dim3(256), 0, h->stream, scratch, nx, ny, lw, mw, tw, out,
                      n_clips);
   EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw, double2* out,
+                  double2* scratch /* n_clips*nx*(2mw+1) */, double2* part /* S*n_clips*nx*(2mw+1), S > 1 */,
+                  double2* tw /* nx+1 + ny+1 */) {
+  std::vector<double2> t;
+  dft_twiddles(nx, ny, t);
+  EUVL_HIP_CHECK(hipMemcpyAsync(tw, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(mask_spectrum_enqueue(h, bits, nx, ny, n_clips, lw, mw, out, scratch, part, tw));
   // the host table must outlive the copy
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
@@ -388,6 +408,294 @@ extern "C" int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* 
   return EUVL_OK;
 }
 
+namespace euvl {
+
+// Host tables of one EM solve (order set, incidence columns), replicated over
+// the batch entries: entry b = clip * n_off + offset.
+void build_solve_tables(const euvl_em_params* p, const int* lindex, const int* mindex, int N, int lmax, int mmax,
+                        int n_off, const double* sx0, const double* sy0, int n_clips, SolveHost& t) {
+  const int nsy = 2 * p->lsmax_y + 1;
+  t.inc_idx.clear();
+  t.inc_slot.clear();
+  std::vector<int> inc_ls, inc_ms;
+  std::vector<int> pos((2 * lmax + 1) * (2 * mmax + 1), -1);
+  for (int i = 0; i < N; ++i) pos[(lindex[i] + lmax) * (2 * mmax + 1) + (mindex[i] + mmax)] = i;
+  // lattice incidences inside the NA (header.h:1394-1398) and their modes
+  for (int ls = -p->lsmax_x; ls <= p->lsmax_x; ++ls)
+    for (int ms = -p->lsmax_y; ms <= p->lsmax_y; ++ms) {
+      const double a = ls * p->mx / p->dx, b = ms * p->my / p->dy, c = p->na / p->wavelength;
+      if (a * a + b * b > c * c) continue;
+      if (std::abs(ls) > lmax || std::abs(ms) > mmax) continue;
+      const int i = pos[(ls + lmax) * (2 * mmax + 1) + (ms + mmax)];
+      if (i < 0) continue;  // no such mode: AS = 0 and Ax stays 0 (header.h:1409-1415)
+      t.inc_idx.push_back(i);
+      t.inc_slot.push_back((ls + p->lsmax_x) * nsy + (ms + p->lsmax_y));
+      inc_ls.push_back(ls);
+      inc_ms.push_back(ms);
+    }
+  const int ninc = (int)t.inc_idx.size(), E = n_off * n_clips;
+  t.sx.resize(E);
+  t.sy.resize(E);
+  t.kz.assign((size_t)E * std::max(ninc, 1), 0.0);
+  for (int b = 0; b < E; ++b) {
+    const int o = b % n_off;
+    t.sx[b] = sx0[o];
+    t.sy[b] = sy0[o];
+    for (int c = 0; c < ninc; ++c) {
+      const double kx = sx0[o] + inc_ls[c] * 2.0 * PI / p->dx;
+      const double ky = sy0[o] + inc_ms[c] * 2.0 * PI / p->dy;
+      t.kz[(size_t)b * ninc + c] = std::sqrt(p->k * p->k - kx * kx - ky * ky);
+    }
+  }
+}
+
+// Uploads the tables into `blk` (device) and fills the device pointers.
+size_t solve_tables_bytes(const euvl_em_params* p, int N, const SolveHost& t) {
+  size_t top = 0;
+  auto add = [&](size_t bytes) { top = ((top + 255) & ~size_t(255)) + bytes; };
+  add(sizeof(int) * N);
+  add(sizeof(int) * N);
+  add(sizeof(double) * t.sx.size());
+  add(sizeof(double) * t.sy.size());
+  add(sizeof(double2) * p->n_layers);
+  add(sizeof(double) * p->n_layers);
+  add(sizeof(int) * std::max<size_t>(t.inc_idx.size(), 1));
+  add(sizeof(int) * std::max<size_t>(t.inc_slot.size(), 1));
+  add(sizeof(double) * t.kz.size());
+  return top;
+}
+
+// `up` receives the packed host copy; the caller keeps it alive until the
+// stream has been synchronised (the copy may still read it).
+int upload_solve_tables(euvl_handle h, const euvl_em_params* p, const int* lindex, const int* mindex, int N,
+                        const SolveHost& t, Upload& up, char* blk, SolveDev& d) {
+  const size_t o_l = up.add(lindex, sizeof(int) * N);
+  const size_t o_m = up.add(mindex, sizeof(int) * N);
+  const size_t o_sx = up.add(t.sx.data(), sizeof(double) * t.sx.size());
+  const size_t o_sy = up.add(t.sy.data(), sizeof(double) * t.sy.size());
+  const size_t o_eps = up.add(p->h_layer_eps, sizeof(double2) * p->n_layers);
+  const size_t o_d = up.add(p->h_layer_d, sizeof(double) * p->n_layers);
+  const size_t o_idx = up.add(t.inc_idx.data(), sizeof(int) * std::max<size_t>(t.inc_idx.size(), 1));
+  const size_t o_slot = up.add(t.inc_slot.data(), sizeof(int) * std::max<size_t>(t.inc_slot.size(), 1));
+  const size_t o_kz = up.add(t.kz.data(), sizeof(double) * t.kz.size());
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  d.lidx = (const int*)(blk + o_l);
+  d.midx = (const int*)(blk + o_m);
+  d.sx0 = (const double*)(blk + o_sx);
+  d.sy0 = (const double*)(blk + o_sy);
+  d.leps = (const double2*)(blk + o_eps);
+  d.ld = (const double*)(blk + o_d);
+  d.inc_idx = (const int*)(blk + o_idx);
+  d.inc_slot = (const int*)(blk + o_slot);
+  d.kz = (const double*)(blk + o_kz);
+  d.ninc = (int)t.inc_idx.size();
+  return EUVL_OK;
+}
+
+size_t solve_workspace_bytes(int N, int E, int ninc) {
+  const size_t mat = sizeof(double2) * (size_t)N * N * E;
+  const size_t vec = sizeof(double2) * (size_t)N * E;
+  const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * E;
+  return 13 * mat + 8 * vec + 2 * rhs + sizeof(int) * (size_t)N * E + sizeof(double) * 8 * (size_t)E +
+         sizeof(int) * (size_t)E + 64 * 1024;
+}
+
+// The EM solve of E = n_clips x n_off batch entries (ampS, header.h:1208-1466)
+// on tables already on the device.  ctl.fixed: the doubling count, the Neumann
+// length and the unpivoted top solve are fixed ahead of time and checked on the
+// device (ctl.d_flags); nothing is read back, so the call can be captured in a
+// graph.  Otherwise the decisions are taken from norms read back to the host,
+// with the pivoted fallback of the top solve.  Takes its matrices from the
+// workspace (reserved by the caller).
+int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDev& t, int N, int lmax, int mmax,
+               const double2* spec, int per_clip, int64_t spec_stride, int E, double2* Ax, SolveCtl& ctl) {
+  const int B = E;
+  const double k = p->k;
+  const int nsx = 2 * p->lsmax_x + 1, nsy = 2 * p->lsmax_y + 1;
+  const int ninc = t.ninc;
+  const size_t mat = sizeof(double2) * (size_t)N * N * B;
+  const size_t vec = sizeof(double2) * (size_t)N * B;
+  const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * B;
+  ModeVecs v;
+  v.kt = (double*)ws_take(h, vec);
+  v.kxy2 = (double*)ws_take(h, vec);
+  v.klm = (double2*)ws_take(h, vec);
+  v.uu = (double2*)ws_take(h, vec);
+  v.ub = (double2*)ws_take(h, vec);
+  v.alru = (double2*)ws_take(h, vec);
+  v.b2 = (double2*)ws_take(h, vec);
+  double2* Mx[7];
+  for (int i = 0; i < 7; ++i) Mx[i] = (double2*)ws_take(h, mat);
+  double2* Brhs = (double2*)ws_take(h, rhs);
+  double2* FGc = (double2*)ws_take(h, rhs);
+  int* ipiv = (int*)ws_take(h, sizeof(int) * (size_t)N * B);
+  double* red = (double*)ws_take(h, sizeof(double) * 8 * (size_t)B);  // device-side check slots
+  int* info = (int*)ws_take(h, sizeof(int) * (size_t)B);
+  for (int i = 0; i < 7; ++i)
+    if (!Mx[i] || !Brhs || !FGc || !ipiv || !red || !info || !v.b2) {
+      set_error("euvl_amp_s: workspace carve failed");
+      return EUVL_ERR_NOMEM;
+    }
+  if (ctl.fixed && !ctl.d_flags) {
+    set_error("euvl_amp_s: a fixed-decision solve needs a flag array");
+    return EUVL_ERR_ARG;
+  }
+  const int64_t sm = (int64_t)N * N;
+
+  hipLaunchKernelGGL(mode_vectors_kernel, grid1((int64_t)N * B), dim3(256), 0, h->stream, N, B, polar, k, p->dx,
+                     p->dy, t.lidx, t.midx, t.sx0, t.sy0, p->n_layers, t.leps, t.ld, v, (double*)nullptr,
+                     (double*)nullptr);
+  EUVL_LAUNCH_CHECK();
+  const std::complex<double> ea(p->eabs_re, p->eabs_im);
+  const std::complex<double> eai = 1.0 / ea, lea = std::log(ea);
+  double2* D = Mx[0];
+  double2* G = Mx[1];
+  hipLaunchKernelGGL(toeplitz_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, polar, k, p->dx, p->dy, t.lidx,
+                     t.midx, lmax, mmax, spec, per_clip, spec_stride, make_double2(ea.real(), ea.imag()),
+                     make_double2(eai.real(), eai.imag()), make_double2(lea.real(), lea.imag()), v, D, G);
+  EUVL_LAUNCH_CHECK();
+
+  // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (6 scratch matrices from the workspace)
+  double2* C = Mx[2];
+  double2* Fs = Mx[3];
+  double2* G2c = Mx[4];
+  int ndbl = 0;
+  EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl, ctl.fixed ? ctl.doublings : -1, red, ctl.d_flags,
+                    &ctl.alpha));
+
+  // products of the coupling and the top solve (the LU keeps its own scope)
+  PrecScope scope(h, EUVL_PREC_FP32_COUPLING);
+  // Minv = (I+G)(I+G^2)(I+G^4)... ~ (I - G)^-1 with M = i k (I - G); ||G|| is ~0.1 for these stacks.
+  double2* Gp = Mx[0];  // D no longer needed
+  double2* Fm = Mx[5];
+  double2* Fn = Mx[6];
+  int nfac = 1;
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, Gp, N, sm, ZERO, B));           // G^2
+  EUVL_TRY(ew_lincomb(h, N, B, Fm, ONE, G, ONE, Gp, ZERO, nullptr, ONE));                   // I + G + G^2
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, Gp, N, sm, ONE, Fm, N, sm, ZERO, B));           // + G^3
+  // stop once the remainder bound ||G^(2^(j+1))|| <= ||G^(2^j)||^2 is below 1e-16
+  // (1e-10 when these products run in FP32, well below their rounding)
+  const double neumann_tol = (h->prec_mask & EUVL_PREC_FP32_COUPLING) ? 1e-10 : 1e-16;
+  auto next_factor = [&]() -> int {
+    EUVL_TRY(zgemm(h, N, N, N, ONE, Gp, N, sm, Gp, N, sm, ZERO, G, N, sm, ZERO, B));        // next power
+    std::swap(G, Gp);
+    EUVL_TRY(ew_copy(h, sm * B, Fn, Fm));
+    EUVL_TRY(zgemm(h, N, N, N, ONE, Fm, N, sm, Gp, N, sm, ONE, Fn, N, sm, ZERO, B));        // F (I + G^(2^j))
+    std::swap(Fm, Fn);
+    ++nfac;
+    return EUVL_OK;
+  };
+  if (ctl.fixed) {
+    while (nfac < ctl.neumann) EUVL_TRY(next_factor());
+    EUVL_TRY(reduce(h, 1, N, B, Gp, red + 2 * B));
+    EUVL_TRY(flag_check(h, 1, B, red + 2 * B, nullptr, nullptr, 0.0, neumann_tol, FLAG_NEUMANN, ctl.d_flags));
+  } else {
+    if (B > 256) {
+      set_error("euvl_amp_s: batch %d above the 256 of the host read-back", B);
+      return EUVL_ERR_ARG;
+    }
+    for (;;) {
+      std::vector<double> nr(B);
+      EUVL_TRY(reduce_to_host(h, 1, N, B, Gp, nr.data()));
+      double worst = 0.0;
+      for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
+      if (worst * worst < neumann_tol) break;
+      if (!(worst < 0.5) || nfac >= 8) {
+        set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)|| = %.3e)", nfac, worst);
+        return EUVL_ERR_NOT_CONVERGED;
+      }
+      EUVL_TRY(next_factor());
+    }
+  }
+  // W1 = M^-1 diag(b2 (uu+ub)) in place over Fm
+  hipLaunchKernelGGL(w1_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, Fm, v);
+  EUVL_LAUNCH_CHECK();
+  // T1 = C W1 (into G), T2 = G2 W1 (into Gp)
+  double2* T1 = G;
+  double2* T2 = Gp;
+  EUVL_TRY(zgemm(h, N, N, N, ONE, C, N, sm, Fm, N, sm, ZERO, T1, N, sm, ZERO, B));
+  EUVL_TRY(zgemm(h, N, N, N, ONE, G2c, N, sm, Fm, N, sm, ZERO, T2, N, sm, ZERO, B));
+  hipLaunchKernelGGL(u0r_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, C, Fs, T1, T2, v);
+  EUVL_LAUNCH_CHECK();
+  double2* R = T2;
+  double2* U0 = T1;
+  double2* U0c = Mx[2];  // C is no longer needed: a copy of U0 for the residual check
+  double2* Bres = Mx[3]; // nor F: the right-hand sides kept for the check
+  const int64_t nsxy = (int64_t)nsx * nsy;
+  EUVL_TRY(fill(h, nsxy * N * B, Ax, ZERO));
+  int lu_path = 0;
+  if (ninc > 0) {
+    const int64_t sr = (int64_t)N * ninc;
+    const double tol = (h->prec_mask & EUVL_PREC_FP32_LU) ? 1e-5 : 1e-12;
+    auto singular = [&](bool report) -> int {
+      int* hi = (int*)h->h_pinned;
+      EUVL_HIP_CHECK(hipMemcpyAsync(hi, info, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+      for (int b = 0; b < B; ++b)
+        if (hi[b] != 0) {
+          if (report) set_error("euvl_amp_s: U0 singular (offset %d, column %d)", b, hi[b] - 1);
+          return EUVL_ERR_SINGULAR;
+        }
+      return EUVL_OK;
+    };
+    // U0 is well conditioned (cond ~10-20 for these stacks) and its unpivoted
+    // elimination is stable (growth ~1.1), so the LU without row interchanges
+    // (GEMM panels, no pivot searches) comes first; its solve is verified by
+    // the residual ||U0 X - E|| / (||U0|| ||X||) and redone with partial
+    // pivoting if that is not at rounding level (adaptive), or flagged (fixed).
+    EUVL_TRY(ew_copy(h, sm * B, U0c, U0));
+    EUVL_TRY(zgetrf_nopiv(h, N, U0, N, sm, ipiv, B, info));
+    bool ok = true;
+    if (ctl.fixed)
+      EUVL_TRY(flag_info(h, B, info, FLAG_ZERO_PIVOT, ctl.d_flags));
+    else
+      ok = singular(false) == EUVL_OK;
+    if (ok) {
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
+      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(ew_copy(h, sr * B, Bres, Brhs));
+      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+      {
+        PrecScope fp64(h, 0);  // the check itself always in FP64
+        EUVL_TRY(zgemm(h, N, ninc, N, ONE, U0c, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
+      }
+      double* dr = red + 3 * B;
+      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Bres, dr));
+      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Brhs, dr + B));
+      EUVL_TRY(reduce(h, 1, N, B, U0c, dr + 2 * B));
+      if (ctl.fixed) {
+        EUVL_TRY(flag_check(h, 2, B, dr, dr + B, dr + 2 * B, 0.0, tol, FLAG_LU_RESIDUAL, ctl.d_flags));
+      } else {
+        EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, dr, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, h->stream));
+        EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+        for (int b = 0; b < B; ++b) {
+          const double res = h->h_pinned[b], xm = h->h_pinned[B + b], nrm = h->h_pinned[2 * B + b];
+          if (!(res <= tol * nrm * xm)) ok = false;
+        }
+      }
+    }
+    if (!ok) {
+      lu_path = 1;
+      EUVL_TRY(ew_copy(h, sm * B, U0, U0c));
+      EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, info));
+      EUVL_TRY(singular(true));
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
+      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+    }
+    EUVL_TRY(zgemm(h, N, ninc, N, ONE, R, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));
+    hipLaunchKernelGGL(ax_scatter_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx,
+                       t.inc_slot, t.kz, FGc, v.klm, nsxy * N, Ax);
+    EUVL_LAUNCH_CHECK();
+  }
+  ctl.doublings_used = ndbl;
+  ctl.neumann_used = nfac;
+  ctl.lu_path = lu_path;
+  return EUVL_OK;
+}
+
+}  // namespace euvl
+
 extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, const int* h_lindex,
                           const int* h_mindex, int nrange, int lmax, int mmax, const double* d_spec, int n_off,
                           const double* h_sx0, const double* h_sy0, double* d_Ax, int* h_info) {
@@ -404,209 +712,22 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
       return EUVL_ERR_ARG;
     }
   const int N = nrange, B = n_off;
-  const double k = p->k;
-  // lattice incidences inside the NA (header.h:1394-1398) and their modes
-  const int nsx = 2 * p->lsmax_x + 1, nsy = 2 * p->lsmax_y + 1;
-  std::vector<int> inc_idx, inc_slot, inc_ls, inc_ms;
-  {
-    std::vector<int> pos((2 * lmax + 1) * (2 * mmax + 1), -1);
-    for (int i = 0; i < N; ++i) pos[(h_lindex[i] + lmax) * (2 * mmax + 1) + (h_mindex[i] + mmax)] = i;
-    for (int ls = -p->lsmax_x; ls <= p->lsmax_x; ++ls)
-      for (int ms = -p->lsmax_y; ms <= p->lsmax_y; ++ms) {
-        const double a = ls * p->mx / p->dx, b = ms * p->my / p->dy, c = p->na / p->wavelength;
-        if (a * a + b * b > c * c) continue;
-        if (std::abs(ls) > lmax || std::abs(ms) > mmax) continue;
-        const int i = pos[(ls + lmax) * (2 * mmax + 1) + (ms + mmax)];
-        if (i < 0) continue;  // no such mode: AS = 0 and Ax stays 0 (header.h:1409-1415)
-        inc_idx.push_back(i);
-        inc_slot.push_back((ls + p->lsmax_x) * nsy + (ms + p->lsmax_y));
-        inc_ls.push_back(ls);
-        inc_ms.push_back(ms);
-      }
-  }
-  const int ninc = (int)inc_idx.size();
-  std::vector<double> kz((size_t)B * std::max(ninc, 1));
-  for (int b = 0; b < B; ++b)
-    for (int c = 0; c < ninc; ++c) {
-      const double kx = h_sx0[b] + inc_ls[c] * 2.0 * PI / p->dx;
-      const double ky = h_sy0[b] + inc_ms[c] * 2.0 * PI / p->dy;
-      kz[(size_t)b * ninc + c] = std::sqrt(k * k - kx * kx - ky * ky);
-    }
-
-  Upload up;
-  const size_t o_l = up.add(h_lindex, sizeof(int) * N);
-  const size_t o_m = up.add(h_mindex, sizeof(int) * N);
-  const size_t o_sx = up.add(h_sx0, sizeof(double) * B);
-  const size_t o_sy = up.add(h_sy0, sizeof(double) * B);
-  const size_t o_eps = up.add(p->h_layer_eps, sizeof(double2) * p->n_layers);
-  const size_t o_d = up.add(p->h_layer_d, sizeof(double) * p->n_layers);
-  const size_t o_idx = up.add(inc_idx.data(), sizeof(int) * std::max(ninc, 1));
-  const size_t o_slot = up.add(inc_slot.data(), sizeof(int) * std::max(ninc, 1));
-  const size_t o_kz = up.add(kz.data(), sizeof(double) * kz.size());
-
-  const size_t mat = sizeof(double2) * (size_t)N * N * B;
-  const size_t vec = sizeof(double2) * (size_t)N * B;
-  const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * B;
-  const size_t need = up.host.size() + 13 * mat + 8 * vec + 2 * rhs + sizeof(int) * (size_t)N * B + 64 * 1024;
-  EUVL_TRY(ws_reserve(h, need));
-  char* blk = (char*)ws_take(h, up.host.size());
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
-  const int* d_l = (const int*)(blk + o_l);
-  const int* d_m = (const int*)(blk + o_m);
-  ModeVecs v;
-  v.kt = (double*)ws_take(h, vec);
-  v.kxy2 = (double*)ws_take(h, vec);
-  v.klm = (double2*)ws_take(h, vec);
-  v.uu = (double2*)ws_take(h, vec);
-  v.ub = (double2*)ws_take(h, vec);
-  v.alru = (double2*)ws_take(h, vec);
-  v.b2 = (double2*)ws_take(h, vec);
-  double2* Mx[7];
-  for (int i = 0; i < 7; ++i) Mx[i] = (double2*)ws_take(h, mat);
-  double2* Brhs = (double2*)ws_take(h, rhs);
-  double2* FGc = (double2*)ws_take(h, rhs);
-  int* ipiv = (int*)ws_take(h, sizeof(int) * (size_t)N * B);
-  for (int i = 0; i < 7; ++i)
-    if (!Mx[i] || !Brhs || !FGc || !ipiv) {
-      set_error("euvl_amp_s: workspace carve failed");
-      return EUVL_ERR_NOMEM;
-    }
-  const int64_t sm = (int64_t)N * N;
-
-  hipLaunchKernelGGL(mode_vectors_kernel, grid1((int64_t)N * B), dim3(256), 0, h->stream, N, B, polar, k, p->dx,
-                     p->dy, d_l, d_m, (const double*)(blk + o_sx), (const double*)(blk + o_sy), p->n_layers,
-                     (const double2*)(blk + o_eps), (const double*)(blk + o_d), v, (double*)nullptr,
-                     (double*)nullptr);
-  EUVL_LAUNCH_CHECK();
-  const std::complex<double> ea(p->eabs_re, p->eabs_im);
-  const std::complex<double> eai = 1.0 / ea, lea = std::log(ea);
-  double2* D = Mx[0];
-  double2* G = Mx[1];
-  hipLaunchKernelGGL(toeplitz_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, polar, k, p->dx, p->dy, d_l, d_m,
-                     lmax, mmax, (const double2*)d_spec, make_double2(ea.real(), ea.imag()),
-                     make_double2(eai.real(), eai.imag()), make_double2(lea.real(), lea.imag()), v, D, G);
-  EUVL_LAUNCH_CHECK();
-
-  // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (6 scratch matrices from the workspace)
-  double2* C = Mx[2];
-  double2* Fs = Mx[3];
-  double2* G2c = Mx[4];
-  int ndbl = 0;
-  EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl));
-
-  // products of the coupling and the top solve (the LU keeps its own scope)
-  PrecScope scope(h, EUVL_PREC_FP32_COUPLING);
-  // Minv = (I+G)(I+G^2)(I+G^4)... ~ (I - G)^-1 with M = i k (I - G); ||G|| is ~0.1 for these stacks.
-  double2* Gp = Mx[0];  // D no longer needed
-  double2* Fm = Mx[5];
-  double2* Fn = Mx[6];
-  int nfac = 1;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, Gp, N, sm, ZERO, B));           // G^2
-  EUVL_TRY(ew_lincomb(h, N, B, Fm, ONE, G, ONE, Gp, ZERO, nullptr, ONE));                   // I + G + G^2
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, Gp, N, sm, ONE, Fm, N, sm, ZERO, B));           // + G^3
-  // stop once the remainder bound ||G^(2^(j+1))|| <= ||G^(2^j)||^2 is below 1e-16
-  // (1e-10 when these products run in FP32, well below their rounding)
-  const double neumann_tol = (h->prec_mask & EUVL_PREC_FP32_COUPLING) ? 1e-10 : 1e-16;
-  for (;;) {
-    std::vector<double> nr(B);
-    EUVL_TRY(reduce_to_host(h, 1, N, B, Gp, nr.data()));
-    double worst = 0.0;
-    for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
-    if (worst * worst < neumann_tol) break;
-    if (!(worst < 0.5) || nfac >= 8) {
-      set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)|| = %.3e)", nfac, worst);
-      return EUVL_ERR_NOT_CONVERGED;
-    }
-    EUVL_TRY(zgemm(h, N, N, N, ONE, Gp, N, sm, Gp, N, sm, ZERO, G, N, sm, ZERO, B));        // next power
-    std::swap(G, Gp);
-    EUVL_TRY(ew_copy(h, sm * B, Fn, Fm));
-    EUVL_TRY(zgemm(h, N, N, N, ONE, Fm, N, sm, Gp, N, sm, ONE, Fn, N, sm, ZERO, B));        // F (I + G^(2^j))
-    std::swap(Fm, Fn);
-    ++nfac;
-  }
-  // W1 = M^-1 diag(b2 (uu+ub)) in place over Fm
-  hipLaunchKernelGGL(w1_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, Fm, v);
-  EUVL_LAUNCH_CHECK();
-  // T1 = C W1 (into G), T2 = G2 W1 (into Gp)
-  double2* T1 = G;
-  double2* T2 = Gp;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, C, N, sm, Fm, N, sm, ZERO, T1, N, sm, ZERO, B));
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G2c, N, sm, Fm, N, sm, ZERO, T2, N, sm, ZERO, B));
-  hipLaunchKernelGGL(u0r_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, C, Fs, T1, T2, v);
-  EUVL_LAUNCH_CHECK();
-  double2* R = T2;
-  double2* U0 = T1;
-  double2* U0c = Mx[2];  // C is no longer needed: a copy of U0 for the residual check
-  double2* Bres = Mx[3]; // nor F: the right-hand sides kept for the check
-  int* d_info = (int*)h->d_red;
-  const int64_t nsxy = (int64_t)nsx * nsy;
-  EUVL_TRY(fill(h, nsxy * N * B, (double2*)d_Ax, ZERO));
-  int lu_path = 0;
-  if (ninc > 0) {
-    const int64_t sr = (int64_t)N * ninc;
-    auto singular = [&](bool report) -> int {
-      int* hi = (int*)h->h_pinned;
-      EUVL_HIP_CHECK(hipMemcpyAsync(hi, d_info, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
-      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-      for (int b = 0; b < B; ++b)
-        if (hi[b] != 0) {
-          if (report) set_error("euvl_amp_s: U0 singular (offset %d, column %d)", b, hi[b] - 1);
-          return EUVL_ERR_SINGULAR;
-        }
-      return EUVL_OK;
-    };
-    // U0 is well conditioned (cond ~10-20 for these stacks) and its unpivoted
-    // elimination is stable (growth ~1.1), so the LU without row interchanges
-    // (GEMM panels, no pivot searches) comes first; its solve is verified by
-    // the residual ||U0 X - E|| / (||U0|| ||X||) and redone with partial
-    // pivoting if that is not at rounding level.
-    EUVL_TRY(ew_copy(h, sm * B, U0c, U0));
-    EUVL_TRY(zgetrf_nopiv(h, N, U0, N, sm, ipiv, B, d_info));
-    bool ok = singular(false) == EUVL_OK;
-    if (ok) {
-      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
-                         (const int*)(blk + o_idx), Brhs);
-      EUVL_LAUNCH_CHECK();
-      EUVL_TRY(ew_copy(h, sr * B, Bres, Brhs));
-      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
-      {
-        PrecScope fp64(h, 0);  // the check itself always in FP64
-        EUVL_TRY(zgemm(h, N, ninc, N, ONE, U0c, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
-      }
-      double* dr = h->d_red;
-      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Bres, dr));
-      EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Brhs, dr + 64));
-      EUVL_TRY(reduce(h, 1, N, B, U0c, dr + 128));
-      EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, dr, sizeof(double) * 192, hipMemcpyDeviceToHost, h->stream));
-      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-      const double tol = (h->prec_mask & EUVL_PREC_FP32_LU) ? 1e-5 : 1e-12;
-      for (int b = 0; b < B; ++b) {
-        const double res = h->h_pinned[b], xm = h->h_pinned[64 + b], nrm = h->h_pinned[128 + b];
-        if (!(res <= tol * nrm * xm)) ok = false;
-      }
-    }
-    if (!ok) {
-      lu_path = 1;
-      EUVL_TRY(ew_copy(h, sm * B, U0, U0c));
-      EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, d_info));
-      EUVL_TRY(singular(true));
-      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
-                         (const int*)(blk + o_idx), Brhs);
-      EUVL_LAUNCH_CHECK();
-      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
-    }
-    EUVL_TRY(zgemm(h, N, ninc, N, ONE, R, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));
-    hipLaunchKernelGGL(ax_scatter_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B,
-                       (const int*)(blk + o_idx), (const int*)(blk + o_slot), (const double*)(blk + o_kz), FGc,
-                       v.klm, nsxy * N, (double2*)d_Ax);
-    EUVL_LAUNCH_CHECK();
-  }
+  SolveHost th;
+  build_solve_tables(p, h_lindex, h_mindex, N, lmax, mmax, B, h_sx0, h_sy0, 1, th);
+  const size_t tb = solve_tables_bytes(p, N, th);
+  EUVL_TRY(ws_reserve(h, tb + solve_workspace_bytes(N, B, (int)th.inc_idx.size())));
+  char* blk = (char*)ws_take(h, tb);
+  SolveDev td;
+  Upload up;  // read by the copy until the synchronisation at the end
+  EUVL_TRY(upload_solve_tables(h, p, h_lindex, h_mindex, N, th, up, blk, td));
+  SolveCtl ctl;
+  EUVL_TRY(amp_s_core(h, polar, p, td, N, lmax, mmax, (const double2*)d_spec, B, 0, B, (double2*)d_Ax, ctl));
   if (h_info) {
     for (int b = 0; b < B; ++b) {
-      h_info[4 * b + 0] = lu_path;
-      h_info[4 * b + 1] = ndbl;
-      h_info[4 * b + 2] = nfac + 1;
-      h_info[4 * b + 3] = ninc;
+      h_info[4 * b + 0] = ctl.lu_path;
+      h_info[4 * b + 1] = ctl.doublings_used;
+      h_info[4 * b + 2] = ctl.neumann_used + 1;
+      h_info[4 * b + 3] = td.ninc;
     }
   }
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));

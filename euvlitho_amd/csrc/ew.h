@@ -1,6 +1,7 @@
 // Internal interfaces shared between the libeuvl translation units.
 #pragma once
 #include "common.h"
+#include "optics.h"
 
 namespace euvl {
 
@@ -31,9 +32,67 @@ int maxabs_to_host(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int 
 // whose unpivoted elimination is safe; callers verify what they solve with it.
 int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info);
 
+// Device-side checks of decisions fixed ahead of time (graph-capturable runs):
+// flags[b] |= bit where the check fails (see flag_kernel in ew.hip).
+int flag_check(euvl_handle h, int kind, int batch, const double* a, const double* c, const double* d, double s,
+               double theta, int bit, int* flags);
+int flag_info(euvl_handle h, int batch, const int* info, int bit, int* flags);
+int zero_ints(euvl_handle h, int count, int* p);
+
+// flag bits of the fixed-decision EM solve
+constexpr int FLAG_DOUBLINGS = 1;  // the fixed doubling count is below what ||D^2||, ||D^3|| ask for
+constexpr int FLAG_NEUMANN = 2;    // the fixed Neumann product has not reached its remainder bound
+constexpr int FLAG_LU_RESIDUAL = 4;  // the unpivoted top solve failed its residual check
+constexpr int FLAG_ZERO_PIVOT = 8;   // the unpivoted LU met a zero pivot
+
 // matrix functions (matfun.hip); buffers are batch x n x n contiguous.
 // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (G2 may be null); takes 6 workspace matrices.
+// fixed_s < 0: the doubling count is chosen from ||D^2||, ||D^3|| read back to the host (*doublings
+// receives it).  fixed_s >= 0: that count is used without a host read-back; the norms go to
+// d_red[0, 2 batch) and entries whose bound needs more doublings get FLAG_DOUBLINGS in d_flags.
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
-             int* doublings);
+             int* doublings, int fixed_s = -1, double* d_red = nullptr, int* d_flags = nullptr,
+             double* alpha_out = nullptr);
+// the theta of the doubling bound in the current precision scope
+double cos_sinc_theta(euvl_handle h);
+
+// The EM solve core (em.hip).
+struct SolveHost {  // host tables, entry b = clip * n_off + offset
+  std::vector<int> inc_idx, inc_slot;
+  std::vector<double> sx, sy, kz;
+};
+struct SolveDev {  // the same tables on the device
+  const int* lidx = nullptr;
+  const int* midx = nullptr;
+  const double* sx0 = nullptr;
+  const double* sy0 = nullptr;
+  const double2* leps = nullptr;
+  const double* ld = nullptr;
+  const int* inc_idx = nullptr;
+  const int* inc_slot = nullptr;
+  const double* kz = nullptr;
+  int ninc = 0;
+};
+struct SolveCtl {
+  bool fixed = false;  // decisions fixed ahead of time, checked on the device
+  int doublings = 0, neumann = 1;
+  int* d_flags = nullptr;  // per batch entry, FLAG_* bits (fixed mode)
+  int doublings_used = 0, neumann_used = 0, lu_path = 0;  // what ran
+  double alpha = 0.0;  // max(||D^2||^(1/2), ||D^3||^(1/3)) over the batch (adaptive runs)
+};
+void build_solve_tables(const euvl_em_params* p, const int* lindex, const int* mindex, int N, int lmax, int mmax,
+                        int n_off, const double* sx0, const double* sy0, int n_clips, SolveHost& t);
+size_t solve_tables_bytes(const euvl_em_params* p, int N, const SolveHost& t);
+int upload_solve_tables(euvl_handle h, const euvl_em_params* p, const int* lindex, const int* mindex, int N,
+                        const SolveHost& t, Upload& up, char* blk, SolveDev& d);
+size_t solve_workspace_bytes(int N, int E, int ninc);
+int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDev& t, int N, int lmax, int mmax,
+               const double2* spec, int per_clip, int64_t spec_stride, int E, double2* Ax, SolveCtl& ctl);
+// indicator spectrum of n_clips packed masks on a window |l| <= lw, |m| <= mw, with the
+// twiddle tables already on the device (tw: nx + 1 then ny + 1 entries); no synchronisation
+int mask_spectrum_enqueue(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
+                          double2* out, double2* scratch, double2* part, const double2* tw);
+int dft_splits(int nx, int ny, int mw, int n_clips);
+void dft_twiddles(int nx, int ny, std::vector<double2>& t);
 
 }  // namespace euvl

@@ -185,6 +185,36 @@ int check_ew(int n, int batch) {
   return EUVL_OK;
 }
 
+// flags[b] |= bit where a device-side check of a fixed decision fails:
+//   kind 0 (doubling count): s * max(sqrt(a[b]), cbrt(c[b])) > theta, or non-finite
+//   kind 1 (Neumann length): !(a[b] < 0.5) or a[b]^2 >= theta
+//   kind 2 (LU residual):    !(a[b] <= theta * c[b] * d[b])
+__global__ void flag_kernel(int kind, int batch, const double* a, const double* c, const double* d, double s,
+                            double theta, int bit, int* flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  bool bad;
+  if (kind == 0) {
+    const double al = fmax(sqrt(a[b]), cbrt(c[b]));
+    bad = !(s * al <= theta);
+  } else if (kind == 1) {
+    bad = !(a[b] < 0.5) || !(a[b] * a[b] < theta);
+  } else {
+    bad = !(a[b] <= theta * c[b] * d[b]);
+  }
+  if (bad) flags[b] |= bit;
+}
+
+// flags[b] |= bit where info[b] != 0 (a zero pivot)
+__global__ void flag_info_kernel(int batch, const int* info, int bit, int* flags) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch && info[b] != 0) flags[b] |= bit;
+}
+
+__global__ void zero_int_kernel2(int* p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
 }  // namespace
 
 namespace euvl {
@@ -236,6 +266,27 @@ int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* 
   } else {
     hipLaunchKernelGGL(reduce_infnorm_kernel, dim3((unsigned)std::min(256, (n + 3) / 4), batch), dim3(256), 0, h->stream, n, X, d_red);
   }
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int flag_check(euvl_handle h, int kind, int batch, const double* a, const double* c, const double* d, double s,
+               double theta, int bit, int* flags) {
+  hipLaunchKernelGGL(flag_kernel, dim3((batch + 255) / 256), dim3(256), 0, h->stream, kind, batch, a, c, d, s, theta,
+                     bit, flags);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int flag_info(euvl_handle h, int batch, const int* info, int bit, int* flags) {
+  hipLaunchKernelGGL(flag_info_kernel, dim3((batch + 255) / 256), dim3(256), 0, h->stream, batch, info, bit, flags);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int zero_ints(euvl_handle h, int count, int* p) {
+  if (count <= 0) return EUVL_OK;
+  hipLaunchKernelGGL(zero_int_kernel2, dim3(std::min(1024, (count + 255) / 256)), dim3(256), 0, h->stream, p, count);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

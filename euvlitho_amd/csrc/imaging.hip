@@ -13,17 +13,14 @@
 #include <vector>
 
 #include "ew.h"
+#include "imaging.h"
 #include "optics.h"
 
 namespace {
 
 
-struct AbbeSetup {
-  double k, dx, dy, na, kx0, ky0, z0, z;
-  int pol;  // 0: X (Ax = amp, Ay = 0), 1: Y (Ay = amp, Ax = 0), intensity.cpp:295-305
-  int mx, my, lsmx, lsmy, nsx, nsy, lmax, mmax, nrange;
-  int lpx, lpy;  // spectrum half-widths
-};
+using euvl::AbbeSetup;
+using euvl::M3dSetup;
 
 // Pupil fields: F[s][c][l+lpx][m+lpy] (c = x, y, z) for every source point s and pupil point i.
 __global__ void abbe_field_kernel(AbbeSetup a, const double2* Ax, const int* modemap, const double* off,
@@ -157,21 +154,23 @@ int xpass(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const double
 }
 
 // M3D fit of one clip, one thread per pupil point (m3d.cpp:329-467).
-struct M3dSetup {
-  double k, dx, dy, na, lambda, kx0, ky0, z0;
-  int mx, my, lsmx, lsmy, nsx, nsy, lpmx, lpmy, lmax, mmax, nrange;
-};
 
 __device__ inline bool in_na_d(const M3dSetup& a, int l, int m) {
   const double u = l * a.mx / a.dx, v = m * a.my / a.dy, c = a.na / a.lambda;
   return u * u + v * v <= c * c;
 }
 
+// blockIdx.y = clip: fmask, Ax and the outputs advance by their clip strides
 __global__ void m3d_fit_kernel(M3dSetup a, const double2* fmask, const double2* Ax, const int* modemap,
                                const double2* vc, const double2* ab, const int* pupil, int ncut, double2* a0,
-                               double2* ax, double2* ay) {
+                               double2* ax, double2* ay, int64_t f_stride, int64_t a_stride) {
   const int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= ncut) return;
+  fmask += blockIdx.y * f_stride;
+  Ax += blockIdx.y * a_stride;
+  a0 += (int64_t)blockIdx.y * ncut;
+  ax += (int64_t)blockIdx.y * ncut;
+  ay += (int64_t)blockIdx.y * ncut;
   const int lin = pupil[3 * n], min_ = pupil[3 * n + 1], xin = pupil[3 * n + 2];
   const int noy = 2 * a.lpmy + 1;
   const int cx = a.lsmx, cy = a.lsmy;
@@ -293,29 +292,32 @@ __global__ void zero_d_kernel(int64_t count, double* out) {
 
 }  // namespace
 
-extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* p, const double* d_Ax, int nrange,
-                               int lmax,
-                               int mmax, const int* h_modemap, int n_off, const double* h_off, int n_src,
-                               const int* h_src, int ncut, const int* h_pupil, double kx0, double ky0, double z0,
-                               double z, int nx, int ny, double weight, double* d_img) {
-  using namespace euvl;
-  if (!h || !p || !d_Ax || !h_modemap || n_off <= 0 || !h_off || n_src < 0 || ncut <= 0 || !h_pupil || nx <= 0 ||
-      ny <= 0 || !d_img || (polar != 'X' && polar != 'Y')) {
-    set_error("euvl_abbe_image: bad argument");
-    return EUVL_ERR_ARG;
+namespace euvl {
+
+void image_twiddles(int nx, int ny, std::vector<double2>& twx, std::vector<double2>& twy) {
+  const std::complex<double> zi(0.0, 1.0);
+  twx.resize(nx);
+  twy.resize(ny);
+  for (int i = 0; i < nx; ++i) {
+    const std::complex<double> e = std::exp(2.0 * PI * zi * double(i) / double(nx));
+    twx[i] = make_double2(e.real(), e.imag());
   }
-  if (n_src == 0) return EUVL_OK;
-  for (int s = 0; s < n_src; ++s)
-    if (h_src[3 * s] < 0 || h_src[3 * s] >= n_off) {
-      set_error("euvl_abbe_image: source %d names offset %d of %d", s, h_src[3 * s], n_off);
-      return EUVL_ERR_ARG;
-    }
-  int lpx = 0, lpy = 0;
+  for (int i = 0; i < ny; ++i) {
+    const std::complex<double> e = std::exp(2.0 * PI * zi * double(i) / double(ny));
+    twy[i] = make_double2(e.real(), e.imag());
+  }
+}
+
+void pupil_window(const int* pupil, int stride, int ncut, int& lpx, int& lpy) {
+  lpx = lpy = 0;
   for (int i = 0; i < ncut; ++i) {
-    lpx = std::max(lpx, std::abs(h_pupil[2 * i]));
-    lpy = std::max(lpy, std::abs(h_pupil[2 * i + 1]));
+    lpx = std::max(lpx, std::abs(pupil[stride * i]));
+    lpy = std::max(lpy, std::abs(pupil[stride * i + 1]));
   }
-  const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
+}
+
+AbbeSetup abbe_setup(char polar, const euvl_em_params* p, double kx0, double ky0, double z0, double z, int nrange,
+                     int lmax, int mmax, int lpx, int lpy) {
   AbbeSetup a{};
   a.k = p->k;
   a.dx = p->dx;
@@ -337,56 +339,11 @@ extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* 
   a.nrange = nrange;
   a.lpx = lpx;
   a.lpy = lpy;
-  std::vector<std::complex<double>> twx(nx), twy(ny);
-  const std::complex<double> zi(0.0, 1.0);
-  for (int i = 0; i < nx; ++i) twx[i] = std::exp(2.0 * PI * zi * double(i) / double(nx));
-  for (int i = 0; i < ny; ++i) twy[i] = std::exp(2.0 * PI * zi * double(i) / double(ny));
-  Upload up;
-  const size_t o_map = up.add(h_modemap, sizeof(int) * (2 * lmax + 1) * (2 * mmax + 1));
-  const size_t o_off = up.add(h_off, sizeof(double) * 4 * n_off);
-  const size_t o_src = up.add(h_src, sizeof(int) * 3 * n_src);
-  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * ncut);
-  const size_t o_twx = up.add(twx.data(), sizeof(double2) * nx);
-  const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
-  // process sources in chunks so the two spectrum buffers stay bounded (~256 MB)
-  const int64_t per_src = (int64_t)3 * nl * (nm + ny);
-  int chunk = (int)std::max<int64_t>(1, (int64_t)(16 << 20) / per_src);
-  if (chunk > n_src) chunk = n_src;
-  const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
-  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
-  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
-  EUVL_TRY(ws_reserve(h, up.host.size() + fbytes + gbytes + pbytes + 4096));
-  char* blk = (char*)ws_take(h, up.host.size());
-  double2* F = (double2*)ws_take(h, fbytes);
-  double2* G = (double2*)ws_take(h, gbytes);
-  double* Pxp = (double*)ws_take(h, pbytes);
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
-  for (int s0 = 0; s0 < n_src; s0 += chunk) {
-    const int ns = std::min(chunk, n_src - s0);
-    EUVL_TRY(fill(h, (int64_t)ns * 3 * nl * nm, F, make_double2(0.0, 0.0)));
-    hipLaunchKernelGGL(abbe_field_kernel, grid1((int64_t)ns * ncut), dim3(256), 0, h->stream, a, (const double2*)d_Ax,
-                       (const int*)(blk + o_map), (const double*)(blk + o_off), (const int*)(blk + o_src) + 3 * s0, ns,
-                       (const int*)(blk + o_pup), ncut, F);
-    EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl, nm,
-                       lpy, ny, F, (const double2*)(blk + o_twy), G);
-    EUVL_LAUNCH_CHECK();
-    EUVL_TRY(xpass(h, ns * 3, nl, lpx, nx, ny, G, (const double2*)(blk + o_twx), weight, d_img, Pxp));
-  }
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  return EUVL_OK;
+  return a;
 }
 
-extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
-                            const double* d_Ax, int nrange, int lmax, int mmax, const int* h_modemap,
-                            const double* h_vc, const double* h_ab, double kx0, double ky0, double z0, int ncut,
-                            const int* h_pupil, double* d_a0, double* d_ax, double* d_ay) {
-  using namespace euvl;
-  if (!h || !p || !d_fmask || !d_Ax || !h_modemap || !h_vc || !h_ab || ncut <= 0 || !h_pupil || !d_a0 || !d_ax ||
-      !d_ay) {
-    set_error("euvl_m3d_fit: bad argument");
-    return EUVL_ERR_ARG;
-  }
+M3dSetup m3d_setup(const euvl_em_params* p, double kx0, double ky0, double z0, int lpmax_x, int lpmax_y,
+                   int nrange, int lmax, int mmax) {
   M3dSetup a;
   a.k = p->k;
   a.dx = p->dx;
@@ -407,6 +364,132 @@ extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double
   a.lmax = lmax;
   a.mmax = mmax;
   a.nrange = nrange;
+  return a;
+}
+
+// source chunk of the Abbe accumulation: the two spectrum buffers stay bounded (~256 MB)
+int abbe_chunk(int n_src, int nl, int nm, int ny) {
+  const int64_t per_src = (int64_t)3 * nl * (nm + ny);
+  int chunk = (int)std::max<int64_t>(1, (int64_t)(16 << 20) / per_src);
+  return std::max(1, std::min(chunk, n_src));
+}
+
+size_t abbe_workspace_bytes(int n_src, int lpx, int lpy, int nx, int ny) {
+  const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
+  const int chunk = abbe_chunk(n_src, nl, nm, ny);
+  const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
+  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
+  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
+  return fbytes + gbytes + pbytes + 3 * 256;
+}
+
+// Abbe images of n_clips clips of one problem: clip c reads Ax + c ax_stride and
+// accumulates weight x |E|^2 into img + c img_stride (zeroed first when `zero`).
+// Takes its spectrum buffers from the workspace.
+int abbe_enqueue(euvl_handle h, const AbbeSetup& a, const double2* Ax, int64_t ax_stride, int n_clips,
+                 const ImagingDev& t, int n_src, int ncut, int nx, int ny, double weight, double* img,
+                 int64_t img_stride, bool zero) {
+  const int nl = 2 * a.lpx + 1, nm = 2 * a.lpy + 1;
+  const int chunk = abbe_chunk(n_src, nl, nm, ny);
+  const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
+  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
+  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
+  double2* F = (double2*)ws_take(h, fbytes);
+  double2* G = (double2*)ws_take(h, gbytes);
+  double* Pxp = (double*)ws_take(h, pbytes);
+  if (!F || !G || !Pxp) {
+    set_error("abbe image: workspace too small");
+    return EUVL_ERR_NOMEM;
+  }
+  for (int c = 0; c < n_clips; ++c) {
+    double* im = img + c * img_stride;
+    if (zero) {
+      hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, im);
+      EUVL_LAUNCH_CHECK();
+    }
+    for (int s0 = 0; s0 < n_src; s0 += chunk) {
+      const int ns = std::min(chunk, n_src - s0);
+      EUVL_TRY(fill(h, (int64_t)ns * 3 * nl * nm, F, make_double2(0.0, 0.0)));
+      hipLaunchKernelGGL(abbe_field_kernel, grid1((int64_t)ns * ncut), dim3(256), 0, h->stream, a,
+                         Ax + c * ax_stride, t.map, t.off, t.src + 3 * s0, ns, t.pup2, ncut, F);
+      EUVL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl,
+                         nm, a.lpy, ny, F, t.twy, G);
+      EUVL_LAUNCH_CHECK();
+      EUVL_TRY(xpass(h, ns * 3, nl, a.lpx, nx, ny, G, t.twx, weight, im, Pxp));
+    }
+  }
+  return EUVL_OK;
+}
+
+// M3D fits of n_clips clips (fmask + c f_stride: the clip's thin-mask window; Ax + c ax_stride:
+// its amplitudes at the central offset); outputs [clip][ncut]
+int m3d_enqueue(euvl_handle h, const M3dSetup& a, const double2* fmask, int64_t f_stride, const double2* Ax,
+                int64_t ax_stride, int n_clips, const ImagingDev& t, int ncut, double2* a0, double2* ax,
+                double2* ay) {
+  hipLaunchKernelGGL(m3d_fit_kernel, dim3((ncut + 127) / 128, n_clips), dim3(128), 0, h->stream, a, fmask, Ax, t.map,
+                     t.vc, t.ab, t.pup3, ncut, a0, ax, ay, f_stride, ax_stride);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+}  // namespace euvl
+
+extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* p, const double* d_Ax, int nrange,
+                               int lmax,
+                               int mmax, const int* h_modemap, int n_off, const double* h_off, int n_src,
+                               const int* h_src, int ncut, const int* h_pupil, double kx0, double ky0, double z0,
+                               double z, int nx, int ny, double weight, double* d_img) {
+  using namespace euvl;
+  if (!h || !p || !d_Ax || !h_modemap || n_off <= 0 || !h_off || n_src < 0 || ncut <= 0 || !h_pupil || nx <= 0 ||
+      ny <= 0 || !d_img || (polar != 'X' && polar != 'Y')) {
+    set_error("euvl_abbe_image: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  if (n_src == 0) return EUVL_OK;
+  for (int s = 0; s < n_src; ++s)
+    if (h_src[3 * s] < 0 || h_src[3 * s] >= n_off) {
+      set_error("euvl_abbe_image: source %d names offset %d of %d", s, h_src[3 * s], n_off);
+      return EUVL_ERR_ARG;
+    }
+  int lpx = 0, lpy = 0;
+  pupil_window(h_pupil, 2, ncut, lpx, lpy);
+  const AbbeSetup a = abbe_setup(polar, p, kx0, ky0, z0, z, nrange, lmax, mmax, lpx, lpy);
+  std::vector<double2> twx, twy;
+  image_twiddles(nx, ny, twx, twy);
+  Upload up;
+  const size_t o_map = up.add(h_modemap, sizeof(int) * (2 * lmax + 1) * (2 * mmax + 1));
+  const size_t o_off = up.add(h_off, sizeof(double) * 4 * n_off);
+  const size_t o_src = up.add(h_src, sizeof(int) * 3 * n_src);
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * ncut);
+  const size_t o_twx = up.add(twx.data(), sizeof(double2) * nx);
+  const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
+  EUVL_TRY(ws_reserve(h, up.host.size() + abbe_workspace_bytes(n_src, lpx, lpy, nx, ny) + 4096));
+  char* blk = (char*)ws_take(h, up.host.size());
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  ImagingDev t;
+  t.map = (const int*)(blk + o_map);
+  t.off = (const double*)(blk + o_off);
+  t.src = (const int*)(blk + o_src);
+  t.pup2 = (const int*)(blk + o_pup);
+  t.twx = (const double2*)(blk + o_twx);
+  t.twy = (const double2*)(blk + o_twy);
+  EUVL_TRY(abbe_enqueue(h, a, (const double2*)d_Ax, 0, 1, t, n_src, ncut, nx, ny, weight, d_img, 0, false));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
+}
+
+extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
+                            const double* d_Ax, int nrange, int lmax, int mmax, const int* h_modemap,
+                            const double* h_vc, const double* h_ab, double kx0, double ky0, double z0, int ncut,
+                            const int* h_pupil, double* d_a0, double* d_ax, double* d_ay) {
+  using namespace euvl;
+  if (!h || !p || !d_fmask || !d_Ax || !h_modemap || !h_vc || !h_ab || ncut <= 0 || !h_pupil || !d_a0 || !d_ax ||
+      !d_ay) {
+    set_error("euvl_m3d_fit: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  const M3dSetup a = m3d_setup(p, kx0, ky0, z0, lpmax_x, lpmax_y, nrange, lmax, mmax);
   for (int i = 0; i < ncut; ++i)
     if (std::abs(h_pupil[3 * i]) > lpmax_x || std::abs(h_pupil[3 * i + 1]) > lpmax_y) {
       set_error("euvl_m3d_fit: pupil point %d outside the thin-mask window", i);
@@ -420,11 +503,13 @@ extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double
   EUVL_TRY(ws_reserve(h, up.host.size() + 4096));
   char* blk = (char*)ws_take(h, up.host.size());
   EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
-  hipLaunchKernelGGL(m3d_fit_kernel, dim3((ncut + 127) / 128), dim3(128), 0, h->stream, a, (const double2*)d_fmask,
-                     (const double2*)d_Ax, (const int*)(blk + o_map), (const double2*)(blk + o_vc),
-                     (const double2*)(blk + o_ab), (const int*)(blk + o_pup), ncut, (double2*)d_a0, (double2*)d_ax,
-                     (double2*)d_ay);
-  EUVL_LAUNCH_CHECK();
+  ImagingDev t;
+  t.map = (const int*)(blk + o_map);
+  t.vc = (const double2*)(blk + o_vc);
+  t.ab = (const double2*)(blk + o_ab);
+  t.pup3 = (const int*)(blk + o_pup);
+  EUVL_TRY(m3d_enqueue(h, a, (const double2*)d_fmask, 0, (const double2*)d_Ax, 0, 1, t, ncut, (double2*)d_a0,
+                       (double2*)d_ax, (double2*)d_ay));
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
 }

@@ -17,8 +17,16 @@ inline double2 re(double x) { return make_double2(x, 0.0); }
 namespace euvl {
 
 // D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.
+double cos_sinc_theta(euvl_handle h) {
+  // With FP32 products (the precision form) the remainder only has to stay
+  // below FP32 rounding: h^2 alpha <= 8 bounds it by 8^10 / 20! = 4.4e-10.
+  PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
+  return (h->prec_mask & h->prec_scope) ? 8.0 : 1.5;
+}
+
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
-             int* doublings) {
+             int* doublings, int fixed_s, double* d_red, int* d_flags, double* alpha_out) {
+  const double theta = cos_sinc_theta(h);
   PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
   const size_t mat = sizeof(double2) * (size_t)n * n * batch;
   double2* s[6];
@@ -40,24 +48,35 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *T = s[5];
   EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
   EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
-  EUVL_TRY(reduce(h, 1, n, batch, X2, h->d_red));
-  EUVL_TRY(reduce(h, 1, n, batch, X3, h->d_red + 256));
-  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 512, hipMemcpyDeviceToHost, h->stream));
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  double alpha = 0.0;
-  for (int b = 0; b < batch; ++b) {
-    const double a2 = std::sqrt(h->h_pinned[b]), a3 = std::cbrt(h->h_pinned[256 + b]);
-    alpha = std::max(alpha, (std::isnan(a2) || std::isnan(a3)) ? INFINITY : std::max(a2, a3));
-  }
-  if (!std::isfinite(alpha)) {
-    set_error("cos_sinc: non-finite ||D^2|| or ||D^3||");
-    return EUVL_ERR_NOT_CONVERGED;
-  }
-  // With FP32 products (the precision form) the remainder only has to stay
-  // below FP32 rounding: h^2 alpha <= 8 bounds it by 8^10 / 20! = 4.4e-10.
-  const double theta = (h->prec_mask & h->prec_scope) ? 8.0 : 1.5;
   int sq = 0;
-  while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > theta && sq < 60) ++sq;
+  if (fixed_s >= 0) {
+    // the bound is checked on the device; no read-back
+    sq = fixed_s;
+    EUVL_TRY(reduce(h, 1, n, batch, X2, d_red));
+    EUVL_TRY(reduce(h, 1, n, batch, X3, d_red + batch));
+    const double hh = std::ldexp(d, -sq);
+    EUVL_TRY(flag_check(h, 0, batch, d_red, d_red + batch, nullptr, hh * hh, theta, FLAG_DOUBLINGS, d_flags));
+  } else {
+    if (batch > 256) {
+      set_error("cos_sinc: batch %d above the 256 of the host read-back", batch);
+      return EUVL_ERR_ARG;
+    }
+    EUVL_TRY(reduce(h, 1, n, batch, X2, h->d_red));
+    EUVL_TRY(reduce(h, 1, n, batch, X3, h->d_red + 256));
+    EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 512, hipMemcpyDeviceToHost, h->stream));
+    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+    double alpha = 0.0;
+    for (int b = 0; b < batch; ++b) {
+      const double a2 = std::sqrt(h->h_pinned[b]), a3 = std::cbrt(h->h_pinned[256 + b]);
+      alpha = std::max(alpha, (std::isnan(a2) || std::isnan(a3)) ? INFINITY : std::max(a2, a3));
+    }
+    if (!std::isfinite(alpha)) {
+      set_error("cos_sinc: non-finite ||D^2|| or ||D^3||");
+      return EUVL_ERR_NOT_CONVERGED;
+    }
+    while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > theta && sq < 60) ++sq;
+    if (alpha_out) *alpha_out = alpha;
+  }
   if (doublings) *doublings = sq;
   const double hh = std::ldexp(d, -sq);
   // q(x) = sum c_j x^j with x = h^2 D is evaluated as sum (c_j h^(2j)) D^j on the unscaled powers

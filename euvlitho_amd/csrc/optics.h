@@ -9,6 +9,21 @@
 #include <cstring>
 #include <vector>
 
+namespace euvl {
+// small host arrays packed (256-byte aligned) into one block, copied to the
+// device with a single transfer
+struct Upload {
+  std::vector<char> host;
+  size_t add(const void* p, size_t bytes) {
+    size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + bytes);
+    if (bytes) memcpy(host.data() + off, p, bytes);
+    return off;
+  }
+};
+}  // namespace euvl
+using euvl::Upload;
+
 namespace {
 
 const double PI = 3.14159265358979323846;
@@ -21,17 +36,7 @@ inline dim3 grid1(int64_t total) {
   return dim3((unsigned)b);
 }
 
-// small host arrays packed (256-byte aligned) into one block, copied to the
-// device with a single transfer
-struct Upload {
-  std::vector<char> host;
-  size_t add(const void* p, size_t bytes) {
-    size_t off = (host.size() + 255) & ~size_t(255);
-    host.resize(off + bytes);
-    if (bytes) memcpy(host.data() + off, p, bytes);
-    return off;
-  }
-};
+
 
 // Column `col` (0: x-polarised incidence, 1: y) of the 3x2 polarisation
 // rotation R = esp es^T + emp em^T of intensity.cpp:423-465

@@ -376,6 +376,77 @@ int euvl_stcc_image(euvl_handle h, char polar, const euvl_stcc_params* p, const 
                     const double* d_ay, int ncut, const int* h_pupil, int n_src, const double* h_src,
                     double* d_img);
 
+/* ---------------------------------------------------------------------------
+ * Batched plans and graphs.  The reference runs its drivers once per mask
+ * (the ndata loop of m3d.cpp:197-558; intensity.cpp:219-408 per mask file).
+ * A plan runs the same pipeline -- mask spectrum, EM solve (ampS,
+ * header.h:1208-1466), M3D fit (m3d.cpp:329-467) and the rigorous Abbe image
+ * (intensity.cpp:276-408) -- for n_clips masks of one problem in one pass of
+ * batched kernels (batch = clips x source divisions).  No reference
+ * counterpart as an interface: it batches the calls above.
+ * ------------------------------------------------------------------------- */
+typedef struct euvl_plan_s* euvl_plan;
+typedef struct euvl_graph_s* euvl_graph;
+
+typedef struct {
+  char polar;                  /* 'X' or 'Y' */
+  const euvl_em_params* p;     /* optics, stack, grid (copied) */
+  const int* h_lindex;         /* EM order set (nrange), |l| <= lmax, |m| <= mmax */
+  const int* h_mindex;
+  int nrange, lmax, mmax;
+  int n_off;                   /* source divisions */
+  const double* h_off;         /* (sx0, sy0, fx, fy) per division; the first is the central one */
+  int n_clips;                 /* masks per run */
+  double kx0, ky0, z0;
+  /* M3D fit (skipped when ncut = 0) */
+  int ncut;
+  const int* h_pupil3;         /* (linput, minput, xinput) per pupil point */
+  int lpmax_x, lpmax_y;        /* thin-mask window */
+  const double* h_vc;          /* blank amplitudes, nsx x nsy complex */
+  const double* h_ab;
+  /* Abbe image (skipped when n_src = 0; needs the pupil points above) */
+  int n_src;
+  const int* h_src;            /* (division, l0, m0) per source point */
+  double z;                    /* defocus */
+  int nx, ny;
+  double weight;               /* 1 / n_src for the mean over source points */
+} euvl_plan_desc;
+
+/* Makes a plan: all tables go to device memory owned by the plan. */
+int euvl_plan_create(euvl_handle h, const euvl_plan_desc* d, euvl_plan* out);
+int euvl_plan_destroy(euvl_plan pl);
+/* Workspace bytes a run needs. */
+int64_t euvl_plan_workspace_bytes(euvl_plan pl);
+/* Reserves the handle's workspace and LU scratch for the largest of the
+ * plans, so that later runs and their graphs never reallocate. */
+int euvl_plan_reserve(euvl_handle h, int n_plans, const euvl_plan* plans);
+/* One run over n_clips packed masks (d_bits, clip after clip).  Outputs:
+ * d_Ax [clip][division][nsx][nsy][nrange] complex; d_a0, d_ax, d_ay
+ * [clip][ncut] complex; d_img [clip][nx][ny] float64 (overwritten).
+ * fixed = 0: adaptive (host-read norms pick the doubling count and the
+ * Neumann length, the top solve falls back to partial pivoting), then the
+ * plan keeps those decisions (the doublings with a factor-2 margin on the
+ * norm bound).  fixed = 1: the plan's decisions are used as they are, checked
+ * on the device; d_flags [clip x division] receives 1 (doubling bound), 2
+ * (Neumann remainder), 4 (top-solve residual), 8 (zero pivot) where a check
+ * failed -- those clips must be run again adaptively.  A fixed run only
+ * enqueues work (no synchronisation, allocation or host copy) and may be
+ * captured; the caller synchronises before reading anything. */
+int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, double* d_a0, double* d_ax, double* d_ay,
+                  double* d_img, int* d_flags, int fixed);
+int euvl_plan_decisions(euvl_plan pl, int* doublings, int* neumann);
+int euvl_plan_set_decisions(euvl_plan pl, int doublings, int neumann);
+
+/* HIP graph of the work enqueued on the handle's stream between begin and
+ * end (fixed-mode plan runs; the stream must not be the null stream).  A
+ * graph keeps the workspace addresses of the capture: do not grow that
+ * handle's workspace while the graph is in use (euvl_plan_reserve first). */
+int euvl_graph_begin(euvl_handle h);
+int euvl_graph_end(euvl_handle h, euvl_graph* out);
+int euvl_graph_launch(euvl_graph g, euvl_handle h);
+int euvl_graph_nodes(euvl_graph g, int64_t* n);
+int euvl_graph_destroy(euvl_graph g);
+
 /* ---- training data (cnn/intensity/m3d/compress.py, the model scripts of cnn/cnn/model) ---- */
 
 /* The CNN data loader's augmentation (MaskampDataset.__getitem__,

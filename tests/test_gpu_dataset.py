@@ -65,9 +65,9 @@ def test_generate_equals_the_file_route(tmp_path):
     """generate (masks, EM + M3D and compmask on the GPU, no text files)
     writes the npy files that compress makes from the intermediate mask.bin,
     inputxx.csv and inputlm.csv it also writes."""
-    from euvlitho_amd.config import high_na, ANNULAR_C1
+    from euvlitho_amd.config import high_na, ANNULAR
     from euvlitho_amd.dataset import compress, generate
-    cfg = high_na(256, 6, ANNULAR_C1, ndivs=1, image=(64, 64))
+    cfg = high_na(256, 6, ANNULAR, ndivs=1, image=(64, 64))
     say("generate 2 clips at 256 nm")
     direct = tmp_path / "direct"
     generate(2, str(direct), cfg, seed=5, cd=24, gap=16, nout=64, write_intermediates=True)

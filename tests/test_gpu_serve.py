@@ -25,10 +25,10 @@ def need_gpu():
 
 
 def small_mix():
-    from euvlitho_amd.config import high_na, ANNULAR_C1, DIPOLE_ANNULAR_C2
-    return [high_na(256, 8, ANNULAR_C1, ndivs=2, image=(64, 64)),
-            high_na(512, 10, DIPOLE_ANNULAR_C2, ndivs=2, image=(128, 128)),
-            high_na(512, 12, ANNULAR_C1, ndivs=1, image=(128, 128))]
+    from euvlitho_amd.config import high_na, ANNULAR, DIPOLE_ANNULAR
+    return [high_na(256, 8, ANNULAR, ndivs=2, image=(64, 64)),
+            high_na(512, 10, DIPOLE_ANNULAR, ndivs=2, image=(128, 128)),
+            high_na(512, 12, ANNULAR, ndivs=1, image=(128, 128))]
 
 
 def test_results_same_alone_and_in_the_mix():
