@@ -156,6 +156,98 @@ def cpu_baseline(cfg, masks, threads, reps=3):
                     "sample": "512 nm clip, +-10 orders (N=441), 61 annular points, 4 divisions, Abbe 256x256"})
 
 
+def batched_clips(n_clips=64, seed=0):
+    """The uneven-clip workload: n_clips seeded Manhattan clips, each of a
+    problem drawn from the mix (pitch 256-1024 nm, orders +-8..+-20, 61-200
+    source points) with a vacuum fraction drawn from U(0.3, 0.7)."""
+    from euvlitho_amd.config import workload_mix
+    from euvlitho_amd.masks import manhattan
+    cfgs = workload_mix()
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_clips):
+        cfg = cfgs[int(rng.integers(len(cfgs)))]
+        out.append((cfg, manhattan(cfg.ndivx, cfg.ndivy, float(rng.uniform(0.3, 0.7)), int(rng.integers(1 << 30)))))
+    return out
+
+
+def run_batched(n_clips=64, reps=5, precision=None, with_eager=True):
+    """Batched form: the uneven-clip workload in batched plan runs captured
+    into one HIP graph (one launch per pass over all clips), FP64 unless a
+    precision mask is given.  Roofline: 164 N^3 per (clip, division) solve over
+    the pass's wall time.  Beside it: the same clips through the eager
+    per-clip path (Engine, host decisions), and the largest difference
+    between the two (M3D parameters of max|a0|, images of peak)."""
+    import torch
+    from euvlitho_amd import _lib
+    from euvlitho_amd.batch import BatchedEngine
+    from euvlitho_amd.engine import Engine
+    prec = _lib.PREC_FP64 if precision is None else precision
+    clips = batched_clips(n_clips)
+    be = BatchedEngine(precision=prec)
+    t0 = time.perf_counter()
+    be.load(clips)
+    setup_s = time.perf_counter() - t0
+    be.launch()
+    out = be.results()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        be.launch()
+        be.results()  # synchronises, reads the flags once, reruns flagged clips
+        walls.append(time.perf_counter() - t0)
+    out = [[t.cpu().numpy() for t in r] for r in be.results()]
+    flops = sum(EM_FLOPS_PER_N3 * float(len(c.order_set()[0])) ** 3 * len(c.offsets()) for c, _ in clips)
+    wall = float(np.median(walls))
+    res = {"value": n_clips / wall, "unit": "clips/s", "clips": n_clips, "ms_per_pass": 1000 * wall,
+           "p50_pass_ms": 1000 * float(np.percentile(walls, 50)), "p99_pass_ms": 1000 * float(np.percentile(walls, 99)),
+           "runs": len(be.runs), "plans": len(be.plans), "graph_nodes": be.graph_nodes(), "graph_launches_per_pass": 1,
+           "adaptive_reruns": be.reruns, "setup_s": setup_s,
+           "decisions": {f"{p.cfg.ndivx}nm K{p.cfg.orders[1]} x{p.n_clips}": p.decisions() for p in be.plans},
+           "problems": {f"{c.ndivx}nm K{c.orders[1]} {sum(len(x[2]) for x in c.source_points())}pts "
+                        f"{len(c.offsets())}div": sum(1 for cc, _ in clips if cc is c) for c in dict.fromkeys(
+                            cc for cc, _ in clips)},
+           "roofline": {"bound": "mfma", "achieved": flops / wall / 1e12, "peak": FP64_PEAK_TFLOPS if not prec
+                        else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": flops / wall / 1e12 / (FP64_PEAK_TFLOPS if not prec else FP32_PEAK_TFLOPS),
+                        "flops_counted": "164 N^3 per (clip, division) solve over the whole pass (wall)"}}
+    be.close()
+    if with_eager:
+        engs = {}
+        for c, _ in clips:
+            if c not in engs:
+                engs[c] = Engine(c, precision=prec)
+                engs[c].blank_amplitudes()
+        bits = [engs[c].upload_mask(m) for c, m in clips]
+        torch.cuda.synchronize()
+
+        def eager():
+            r = []
+            for (c, _), b in zip(clips, bits):
+                e = engs[c]
+                Ax = e.amp_s(b, "X")
+                a0, ax, ay = e.m3d(b, Ax0=Ax[0])
+                r.append((a0, ax, ay, e.abbe(b, Ax=Ax)))
+            torch.cuda.synchronize()
+            return r
+
+        eager()
+        t0 = time.perf_counter()
+        ref = eager()
+        te = time.perf_counter() - t0
+        em, ei = 0.0, 0.0
+        for g, r in zip(out, ref):
+            r = [t.cpu().numpy() for t in r]
+            s = np.abs(r[0]).max()
+            em = max(em, max(float(np.abs(g[i] - r[i]).max() / s) for i in range(3)))
+            ei = max(ei, float(np.abs(g[3] - r[3]).max() / r[3].max()))
+        res["eager_per_clip"] = {"value": n_clips / te, "unit": "clips/s", "ms_per_pass": 1000 * te}
+        res["err_vs_eager"] = {"m3d_of_max_a0": em, "image_of_peak": ei}
+        for e in engs.values():
+            e.close()
+    return res
+
+
 def ref_default_leg(reps=3):
     """The reference's own m3d.cpp path on its mask (tests/golden/ref_mask.bin,
     2048 nm, NA 0.33, 4x/4x): the two blank solves on the narrow ellipse set
@@ -412,6 +504,8 @@ def main():
     ap.add_argument("--cnn-only", action="store_true", help="only run the CNN predictor leg")
     ap.add_argument("--no-stcc", action="store_true", help="skip the TCCEE image leg")
     ap.add_argument("--no-ref-default", action="store_true", help="skip the reference-default m3d.cpp leg")
+    ap.add_argument("--batched", type=int, default=64, help="clips of the batched form (0 = skip)")
+    ap.add_argument("--batched-only", action="store_true", help="only run the batched form")
     ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -424,6 +518,9 @@ def main():
         return
     if args.mask_producer_only:
         print(json.dumps({"mask_producer": mask_producer(with_cpu=not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.batched_only:
+        print(json.dumps({"batched": run_batched(args.batched)}), flush=True)
         return
     if args.cnn_only:
         print(json.dumps({"cnn_predictor": cnn_predictor(with_cpu=not args.no_cpu_baseline)}), flush=True)
@@ -568,6 +665,8 @@ def main():
                      "mfma_issue_frac": 0.75 * executed / FP64_PEAK_TFLOPS},
         "forms": {"mixed": mixed} if mixed else {},
     }
+    if args.batched > 0:
+        line["forms"]["batched"] = run_batched(args.batched)
     if args.served > 0:
         line["forms"]["served"] = run_served(args.served, args.slots)
     if not args.no_mask_producer:

@@ -51,7 +51,7 @@ class PlanDesc(ctypes.Structure):
         ("n_clips", c_int), ("kx0", c_double), ("ky0", c_double), ("z0", c_double),
         ("ncut", c_int), ("h_pupil3", P_int), ("lpmax_x", c_int), ("lpmax_y", c_int), ("h_vc", P_double),
         ("h_ab", P_double), ("n_src", c_int), ("h_src", P_int), ("z", c_double), ("nx", c_int), ("ny", c_int),
-        ("weight", c_double),
+        ("weight", c_double), ("alpha_margin", c_double),
     ]
 
 

@@ -8,9 +8,9 @@ is cut into runs of a few clips whose EM products are batched over clips x
 source divisions (libeuvl plans: euvl_plan_create / euvl_plan_run), and the
 runs of every group are captured once into one HIP graph that is replayed
 with one launch.  Decisions the eager path takes on the host (doubling count,
-Neumann length, pivoted fallback of the top solve) are calibrated by one
-adaptive run per plan and then checked on the device; a clip whose check
-fails is run again adaptively after the graph.
+Neumann length, pivoted fallback of the top solve) are calibrated by an
+adaptive run of the loaded clips and then checked on the device; a clip whose
+check fails is run again adaptively after the graph.
 """
 import ctypes
 import math
@@ -44,7 +44,7 @@ class _Run:
     """One plan run: its plan, its clips and its device buffers (inputs and
     outputs at fixed addresses, so that a captured graph can be replayed)."""
 
-    def __init__(self, plan, clip_ids, bits, dev):
+    def __init__(self, plan, clip_ids, bits, dev, flags):
         self.plan = plan
         self.clip_ids = clip_ids
         c = len(clip_ids)
@@ -54,7 +54,7 @@ class _Run:
         self.ax = torch.empty_like(self.a0)
         self.ay = torch.empty_like(self.a0)
         self.img = torch.empty((c,) + plan.cfg.image_shape, dtype=torch.float64, device=dev)
-        self.flags = torch.zeros(c * plan.n_off, dtype=torch.int32, device=dev)
+        self.flags = flags  # a slice of the engine's flag array (c x divisions entries)
 
     def launch(self, fixed):
         p = self.plan
@@ -66,7 +66,7 @@ class _Run:
 class Plan:
     """A libeuvl plan of one problem for `n_clips` clips per run."""
 
-    def __init__(self, eng: Engine, n_clips: int, polar="X", z=0.0):
+    def __init__(self, eng: Engine, n_clips: int, polar="X", z=0.0, alpha_margin=1.0):
         cfg = eng.cfg
         self.cfg, self.lib, self.n_clips = cfg, eng.lib, n_clips
         vcxx, abxx = eng.blank_amplitudes(polar)
@@ -89,7 +89,7 @@ class Plan:
         (_, li), (_, mi), (_, of), (_, pu), (_, vc), (_, ab), (_, sr) = self._keep
         d = _lib.PlanDesc(polar.encode(), ctypes.pointer(eng.params), li, mi, len(lidx), L, M, self.n_off, of,
                           n_clips, cfg.kx0, cfg.ky0, cfg.z0, self.ncut, pu, lpx, lpy, vc, ab, len(src), sr, float(z),
-                          nx, ny, 1.0 / max(len(src), 1))
+                          nx, ny, 1.0 / max(len(src), 1), float(alpha_margin))
         self.pl = ctypes.c_void_p()
         _lib.check(self.lib.euvl_plan_create(eng.h, ctypes.byref(d), ctypes.byref(self.pl)), "euvl_plan_create")
 
@@ -109,7 +109,8 @@ class BatchedEngine:
     captured HIP graph on a stream and handle of its own.
 
     load(clips) makes the plans, uploads the masks (they stay resident in
-    HBM), calibrates each plan with one adaptive run and captures the graph;
+    HBM), runs every clip once adaptively (each plan keeps the largest
+    doubling count and Neumann length its clips needed) and captures the graph;
     launch() replays the graph; results() synchronises, reruns flagged clips
     adaptively and returns per-clip (a0, ax, ay, image) as device tensors."""
 
@@ -162,15 +163,20 @@ class BatchedEngine:
                     raw = b"".join(m if isinstance(m, (bytes, bytearray)) else pack_bits(m)
                                    for m in (clips[i][1] for i in part))
                     bits = torch.from_numpy(np.frombuffer(raw, dtype=np.uint8).copy()).to(self.device)
-                    self.runs.append(_Run(plans[key], part, bits, self.device))
+                    self.runs.append((plans[key], part, bits))
+            n_flags = sum(len(part) * p.n_off for p, part, _ in self.runs)
+            self.all_flags = torch.zeros(n_flags, dtype=torch.int32, device=self.device)
+            runs, at = [], 0
+            for p, part, bits in self.runs:
+                k = len(part) * p.n_off
+                runs.append(_Run(p, part, bits, self.device, self.all_flags[at:at + k]))
+                at += k
+            self.runs = runs
             self._bind()
             arr = (ctypes.c_void_p * len(self.plans))(*[p.pl for p in self.plans])
             _lib.check(self.lib.euvl_plan_reserve(self.h, len(self.plans), arr), "euvl_plan_reserve")
-            calibrated = set()
-            for r in self.runs:  # one adaptive run per plan fixes its decisions
-                if id(r.plan) not in calibrated:
-                    r.launch(fixed=False)
-                    calibrated.add(id(r.plan))
+            for r in self.runs:  # every run once adaptive: each plan keeps the largest decisions
+                r.launch(fixed=False)
             if self.use_graph:
                 _lib.check(self.lib.euvl_graph_begin(self.h), "euvl_graph_begin")
                 try:
@@ -203,11 +209,13 @@ class BatchedEngine:
 
     def results(self):
         """Synchronise, rerun flagged runs adaptively, and return per clip
-        (a0, ax, ay, image) views of the device outputs."""
+        (a0, ax, ay, image) views of the device outputs.  One read-back of the
+        flag array when nothing is flagged."""
         self.stream.synchronize()
         out = [None] * self.n_clips
+        any_flag = int(self.all_flags.max().item()) != 0
         for r in self.runs:
-            if int(r.flags.max().item()) != 0:
+            if any_flag and int(r.flags.max().item()) != 0:
                 self.reruns += 1
                 self._bind()
                 with torch.cuda.stream(self.stream):

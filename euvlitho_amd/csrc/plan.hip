@@ -36,6 +36,7 @@ struct euvl_plan_s {
   euvl::ImagingDev im;
   const double2* tw_dft = nullptr;
   int doublings = -1, neumann = -1;
+  double alpha_max = 0.0, alpha_margin = 1.0;  // calibration of the doubling count
   size_t ws_need = 0, aux_need = 0;
 };
 
@@ -123,6 +124,7 @@ extern "C" int euvl_plan_create(euvl_handle h, const euvl_plan_desc* d, euvl_pla
   pl->ny = d->ny;
   pl->lpmx = d->lpmax_x;
   pl->lpmy = d->lpmax_y;
+  pl->alpha_margin = d->alpha_margin > 1.0 ? d->alpha_margin : 1.0;
   if (d->ncut > 0) {
     for (int i = 0; i < d->ncut; ++i)
       if (std::abs(d->h_pupil3[3 * i]) > d->lpmax_x || std::abs(d->h_pupil3[3 * i + 1]) > d->lpmax_y) {
@@ -237,6 +239,7 @@ extern "C" int euvl_plan_set_decisions(euvl_plan pl, int doublings, int neumann)
   }
   pl->doublings = doublings;
   pl->neumann = neumann;
+  pl->alpha_max = 0.0;  // later adaptive runs calibrate from these values up
   return EUVL_OK;
 }
 
@@ -297,13 +300,16 @@ extern "C" int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, 
   EUVL_TRY(amp_s_core(h, pl->polar, &pl->p, pl->sd, pl->N, pl->lmax, pl->mmax, spec, pl->n_off, wem, pl->E,
                       (double2*)d_Ax, ctl));
   if (!fixed) {
-    // calibration: the doublings this batch needed with a factor-2 margin on its
-    // norm bound alpha, and the Neumann length it needed
+    // calibration over every adaptive run so far: the doublings the largest
+    // norm bound alpha needs (times the plan's margin), the longest Neumann product
     const double theta = cos_sinc_theta(h);
+    pl->alpha_max = std::max(pl->alpha_max, ctl.alpha);
     int s = 0;
-    while (std::ldexp(pl->p.dabs, -s) * std::ldexp(pl->p.dabs, -s) * 2.0 * ctl.alpha > theta && s < 60) ++s;
-    pl->doublings = std::max(s, ctl.doublings_used);
-    pl->neumann = ctl.neumann_used;
+    while (std::ldexp(pl->p.dabs, -s) * std::ldexp(pl->p.dabs, -s) * pl->alpha_margin * pl->alpha_max > theta &&
+           s < 60)
+      ++s;
+    pl->doublings = std::max({pl->doublings, s, ctl.doublings_used});
+    pl->neumann = std::max(pl->neumann, ctl.neumann_used);
   }
   h->ws_top = mark;
   const int nsxy = (2 * pl->p.lsmax_x + 1) * (2 * pl->p.lsmax_y + 1);

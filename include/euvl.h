@@ -410,6 +410,7 @@ typedef struct {
   double z;                    /* defocus */
   int nx, ny;
   double weight;               /* 1 / n_src for the mean over source points */
+  double alpha_margin;         /* >= 1: factor on the calibrated norm bound behind the doubling count */
 } euvl_plan_desc;
 
 /* Makes a plan: all tables go to device memory owned by the plan. */
@@ -424,9 +425,9 @@ int euvl_plan_reserve(euvl_handle h, int n_plans, const euvl_plan* plans);
  * d_Ax [clip][division][nsx][nsy][nrange] complex; d_a0, d_ax, d_ay
  * [clip][ncut] complex; d_img [clip][nx][ny] float64 (overwritten).
  * fixed = 0: adaptive (host-read norms pick the doubling count and the
- * Neumann length, the top solve falls back to partial pivoting), then the
- * plan keeps those decisions (the doublings with a factor-2 margin on the
- * norm bound).  fixed = 1: the plan's decisions are used as they are, checked
+ * Neumann length, the top solve falls back to partial pivoting), and the plan
+ * keeps the largest decisions of all its adaptive runs (the doublings for
+ * alpha_margin times the largest norm bound seen).  fixed = 1: the plan's decisions are used as they are, checked
  * on the device; d_flags [clip x division] receives 1 (doubling bound), 2
  * (Neumann remainder), 4 (top-solve residual), 8 (zero pivot) where a check
  * failed -- those clips must be run again adaptively.  A fixed run only
