@@ -58,8 +58,9 @@ def copy_calibration(mib=512, reps=3):
 def run_served(n_req, slots):
     """Served form: requests of six problems (pitch 256-1024 nm, orders +-8..+-20,
     61-200 source points), mixed precision, `slots` streams with admission by
-    device-memory footprint.  A burst of 24 requests measures the capacity; the
-    Poisson stream is then offered at 80 % of it."""
+    device-memory footprint; arrived requests of one problem share a batched
+    plan run.  A burst of 24 requests measures the capacity; the Poisson
+    stream is then offered at 80 % of it."""
     from euvlitho_amd import _lib
     from euvlitho_amd.serve import Server, poisson_stream, served_configs
     cfgs = served_configs()
@@ -80,6 +81,9 @@ def run_served(n_req, slots):
                 "service_p50_ms": 1000.0 * float(np.percentile(svc, 50)),
                 "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
                 "resident_build_s": warm_s, "peak_reserved_gib": srv.peak_reserved / 2**30,
+                "batches": len(srv.batches), "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
+                "adaptive_reruns": srv.reruns, "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+                "load": 0.8,
                 "problems": ["%dnm K%d %s %dpts" % (c.ndivx, c.orders[1], "ann" if c.source.kind == 1 else "dip+ann",
                                                        sum(len(x[2]) for x in c.source_points())) for c in cfgs]}
     finally:
@@ -137,164 +141,21 @@ _BLANK = {}
 
 
 def cpu_baseline(cfg, masks, threads, reps=3):
-    """The oracle on `reps` clips of the workload (median), and on `reps`
-    clips of the 512 nm / +-10-order / 61-point problem beside it."""
+    """The oracle on one whole clip of the workload, and on `reps` clips
+    (median) of the 512 nm / +-10-order / 61-point problem beside it."""
     from euvlitho_amd.config import preset
     from euvlitho_amd.masks import manhattan
-    ts = [cpu_clip(cfg, masks[i % len(masks)]) for i in range(reps)]
-    t = float(np.median([x[0] for x in ts]))
-    em, ab, fit = ts[0][1]
+    t, (em, ab, fit) = cpu_clip(cfg, masks[0])
     c1 = preset("c1")
     t1 = [cpu_clip(c1, manhattan(c1.ndivx, c1.ndivy, 0.5, s))[0] for s in range(reps)]
     n_src = sum(len(x[2]) for x in cfg.source_points())
     return dict(value=1.0 / t, unit="clips/s", cores=threads, kind="port",
-                sample=f"{reps} whole clips (seeds 0-{reps - 1}), median {t:.1f} s per clip: oracle EM solves of "
-                       f"{len(cfg.offsets())} divisions ({em:.1f} s, N={len(cfg.order_set()[0])}), Abbe over "
-                       f"{n_src} source points ({ab:.1f} s), M3D fit ({fit:.1f} s); LAPACK threads = cores",
-                runs_s=[x[0] for x in ts],
+                sample=f"one whole clip (seed 0), {t:.1f} s: oracle EM solves of {len(cfg.offsets())} divisions "
+                       f"({em:.1f} s, N={len(cfg.order_set()[0])}), Abbe over {n_src} source points ({ab:.1f} s), "
+                       f"M3D fit ({fit:.1f} s); LAPACK threads = cores",
                 c1={"value": 1.0 / float(np.median(t1)), "unit": "clips/s", "runs_s": t1,
-                    "sample": "512 nm clip, +-10 orders (N=441), 61 annular points, 4 divisions, Abbe 256x256"})
-
-
-def batched_clips(n_clips=64, seed=0):
-    """The uneven-clip workload: n_clips seeded Manhattan clips, each of a
-    problem drawn from the mix (pitch 256-1024 nm, orders +-8..+-20, 61-200
-    source points) with a vacuum fraction drawn from U(0.3, 0.7)."""
-    from euvlitho_amd.config import workload_mix
-    from euvlitho_amd.masks import manhattan
-    cfgs = workload_mix()
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(n_clips):
-        cfg = cfgs[int(rng.integers(len(cfgs)))]
-        out.append((cfg, manhattan(cfg.ndivx, cfg.ndivy, float(rng.uniform(0.3, 0.7)), int(rng.integers(1 << 30)))))
-    return out
-
-
-def run_batched(n_clips=64, reps=5, precision=None, with_eager=True):
-    """Batched form: the uneven-clip workload in batched plan runs captured
-    into one HIP graph (one launch per pass over all clips), FP64 unless a
-    precision mask is given.  Roofline: 164 N^3 per (clip, division) solve over
-    the pass's wall time.  Beside it: the same clips through the eager
-    per-clip path (Engine, host decisions), and the largest difference
-    between the two (M3D parameters of max|a0|, images of peak)."""
-    import torch
-    from euvlitho_amd import _lib
-    from euvlitho_amd.batch import BatchedEngine
-    from euvlitho_amd.engine import Engine
-    prec = _lib.PREC_FP64 if precision is None else precision
-    clips = batched_clips(n_clips)
-    be = BatchedEngine(precision=prec)
-    t0 = time.perf_counter()
-    be.load(clips)
-    setup_s = time.perf_counter() - t0
-    be.launch()
-    out = be.results()
-    walls = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        be.launch()
-        be.results()  # synchronises, reads the flags once, reruns flagged clips
-        walls.append(time.perf_counter() - t0)
-    out = [[t.cpu().numpy() for t in r] for r in be.results()]
-    flops = sum(EM_FLOPS_PER_N3 * float(len(c.order_set()[0])) ** 3 * len(c.offsets()) for c, _ in clips)
-    wall = float(np.median(walls))
-    res = {"value": n_clips / wall, "unit": "clips/s", "clips": n_clips, "ms_per_pass": 1000 * wall,
-           "p50_pass_ms": 1000 * float(np.percentile(walls, 50)), "p99_pass_ms": 1000 * float(np.percentile(walls, 99)),
-           "runs": len(be.runs), "plans": len(be.plans), "graph_nodes": be.graph_nodes(), "graph_launches_per_pass": 1,
-           "adaptive_reruns": be.reruns, "setup_s": setup_s,
-           "decisions": {f"{p.cfg.ndivx}nm K{p.cfg.orders[1]} x{p.n_clips}": p.decisions() for p in be.plans},
-           "problems": {f"{c.ndivx}nm K{c.orders[1]} {sum(len(x[2]) for x in c.source_points())}pts "
-                        f"{len(c.offsets())}div": sum(1 for cc, _ in clips if cc is c) for c in dict.fromkeys(
-                            cc for cc, _ in clips)},
-           "roofline": {"bound": "mfma", "achieved": flops / wall / 1e12, "peak": FP64_PEAK_TFLOPS if not prec
-                        else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": flops / wall / 1e12 / (FP64_PEAK_TFLOPS if not prec else FP32_PEAK_TFLOPS),
-                        "flops_counted": "164 N^3 per (clip, division) solve over the whole pass (wall)"}}
-    be.close()
-    if with_eager:
-        engs = {}
-        for c, _ in clips:
-            if c not in engs:
-                engs[c] = Engine(c, precision=prec)
-                engs[c].blank_amplitudes()
-        bits = [engs[c].upload_mask(m) for c, m in clips]
-        torch.cuda.synchronize()
-
-        def eager():
-            r = []
-            for (c, _), b in zip(clips, bits):
-                e = engs[c]
-                Ax = e.amp_s(b, "X")
-                a0, ax, ay = e.m3d(b, Ax0=Ax[0])
-                r.append((a0, ax, ay, e.abbe(b, Ax=Ax)))
-            torch.cuda.synchronize()
-            return r
-
-        eager()
-        t0 = time.perf_counter()
-        ref = eager()
-        te = time.perf_counter() - t0
-        em, ei = 0.0, 0.0
-        for g, r in zip(out, ref):
-            r = [t.cpu().numpy() for t in r]
-            s = np.abs(r[0]).max()
-            em = max(em, max(float(np.abs(g[i] - r[i]).max() / s) for i in range(3)))
-            ei = max(ei, float(np.abs(g[3] - r[3]).max() / r[3].max()))
-        res["eager_per_clip"] = {"value": n_clips / te, "unit": "clips/s", "ms_per_pass": 1000 * te}
-        res["err_vs_eager"] = {"m3d_of_max_a0": em, "image_of_peak": ei}
-        for e in engs.values():
-            e.close()
-    return res
-
-
-def ref_default_leg(reps=3):
-    """The reference's own m3d.cpp path on its mask (tests/golden/ref_mask.bin,
-    2048 nm, NA 0.33, 4x/4x): the two blank solves on the narrow ellipse set
-    (N = 1009), the wide solve on the diamond set (N = 8705) and the M3D fit,
-    per mask, FP64 -- the run that the reference's README quotes at about
-    one minute per mask.  Roofline: 164 N^3 per solve over the GEMM time of
-    the run (HIP events), and over its wall time."""
-    import ctypes
-    import torch
-    from euvlitho_amd import _lib
-    from euvlitho_amd.config import reference_default
-    from euvlitho_amd.engine import Engine, Resident
-    cfg = reference_default()
-    eng = Engine(cfg)
-    with open(os.path.join(ROOT, "tests", "golden", "ref_mask.bin"), "rb") as f:
-        bits = eng.upload_mask(f.read())
-
-    def run():
-        eng.resident = Resident()  # the reference solves the blank masks for every mask
-        Ax = eng.amp_s(bits, "X")
-        return eng.m3d(bits, Ax0=Ax[0])
-
-    run()
-    torch.cuda.synchronize()
-    walls = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        run()
-        torch.cuda.synchronize()
-        walls.append(time.perf_counter() - t0)
-    stats = (ctypes.c_double * 8)()
-    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
-    run()
-    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
-    N, Nb = len(eng.lidx), len(cfg.blank_order_set()[0])
-    eng.close()
-    wall = float(np.median(walls))
-    flops = EM_FLOPS_PER_N3 * (float(N) ** 3 + 2.0 * float(Nb) ** 3)
-    ach = flops / (stats[1] * 1e-3) / 1e12
-    return {"value": 1.0 / wall, "unit": "masks/s", "s_per_mask": wall, "runs_s": walls,
-            "config": f"reference default: 2048 nm mask (tests/golden/ref_mask.bin), N={N} wide solve + 2 blank "
-                      f"solves N={Nb} + M3D fit, X polarisation, FP64",
-            "published": "~1 min per mask (reference README.md:32, hardware not stated)",
-            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": ach,
-                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
-                         "flops_counted": "164 N^3 per solve (3 solves)", "gemm_ms_per_mask": stats[1],
-                         "gemm_launches": int(stats[0]), "whole_run_tflops": flops / wall / 1e12}}
+                    "sample": f"median of {reps} clips (seeds 0-{reps - 1}): 512 nm, +-10 orders (N=441), 61 annular "
+                              "points, 4 divisions, Abbe 256x256"})
 
 
 def mask_producer(n_masks=64, ndiv=2048, reps=5, with_cpu=True):

@@ -270,7 +270,10 @@ class Engine:
         ncut = len(self.cfg.pupil[0])
         out = []
         for v in (a0, ax, ay):
-            t = torch.as_tensor(np.asarray(v.cpu() if torch.is_tensor(v) else v), dtype=torch.complex128)
+            if torch.is_tensor(v) and v.device == self.device and v.dtype == torch.complex128:
+                t = v.reshape(-1)  # already on the device: no round trip through the host
+            else:
+                t = torch.as_tensor(np.asarray(v.cpu() if torch.is_tensor(v) else v), dtype=torch.complex128)
             if t.numel() != ncut:
                 raise ValueError(f"M3D vector has {t.numel()} values, the pupil has {ncut} points")
             out.append(t.to(self.device).contiguous())

@@ -2,25 +2,30 @@
 
 Each request names a problem (Config: pitch, order truncation, source) and a
 mask.  The server owns S slots; a slot is a HIP stream with its own libeuvl
-handle (device workspace) and per-configuration engines bound to that
-handle.  Worker threads, one per slot, take requests as they arrive: a
-request is admitted when a slot is idle and its device-memory footprint fits
-the HBM budget (the slot's workspace may have to grow), so requests that
-arrive while others run start at once on another stream and their kernels
-overlap on the device.  Requests that fit may pass an older one that does
-not (admission by footprint), but the oldest waiting request that fits is
-always taken first.
+handle (device workspace), per-configuration engines and batched plans
+(libeuvl plans for 1..B clips of each problem) bound to that handle.  Worker
+threads, one per slot, take requests as they arrive: when a slot is idle it
+takes the oldest arrived request whose batch fits the HBM budget, together
+with the later arrived requests of the same problem (up to the problem's
+batch size), and runs them as one batched plan run -- mask spectra, EM
+solves batched over clips x source divisions, M3D fits and Abbe images --
+then the SOCS image of each clip.  Requests that arrive while others run are
+admitted as slots and memory free up, and their kernels overlap on the
+device with those of the other slots.  Requests that fit may pass an older
+one that does not (admission by footprint), but the oldest waiting request
+that fits is always taken first.
 
 Mask-independent state is resident and shared by every slot (one Resident
-store): the blank-mask amplitudes per configuration and the SOCS kernels.
-Per request the server computes the M3D parameters, the rigorous Abbe image
-and the SOCS image (the per-clip work of the FP64 drop-in, in the precision
-mode chosen for the server).
+store): the blank-mask amplitudes per configuration, the SOCS kernels and
+the decisions of the fixed-decision EM solve (doubling count and Neumann
+length, calibrated once per problem at warm-up on seeded masks with a
+factor-2 margin on the norm bound, and checked on the device for every clip;
+a flagged batch is run again with host decisions).
 
-Results are independent of the mix: every request runs the same kernels on
-its own stream and workspace, and nothing a request reads is written by
-another, so a clip's images are identical whether it runs alone or among
-others.
+Results are independent of the mix: every clip of a batch runs the same
+kernels with the same decisions as when it runs alone, each batch entry is
+computed by its own tiles, and nothing a request reads is written by another,
+so a clip's images agree whether it runs alone or among others.
 """
 import ctypes
 import threading
@@ -68,22 +73,31 @@ def served_configs():
     return workload_mix()
 
 
-def footprint(cfg: Config):
-    """Device bytes one request of this configuration needs beyond the
-    resident state: the EM workspace (13 N x N complex matrices per offset
-    plus vectors and the incidence columns) or the image workspace, whichever
-    is larger, plus the amplitude and image tensors."""
-    lidx, _, _, _ = cfg.order_set()
-    N, B = len(lidx), len(cfg.offsets())
+def batch_size(cfg: Config, max_batch=4):
+    """Clips of one problem batched in one plan run of the served form."""
+    from .batch import clips_per_run
+    return clips_per_run(cfg, max_batch)
+
+
+def footprint(cfg: Config, n_clips=1):
+    """Device bytes one batch of n_clips requests of this configuration needs
+    beyond the resident state: the EM workspace (13 N x N complex matrices per
+    clip and offset plus vectors, the incidence columns and the mask spectra)
+    or the image workspace, whichever is larger, plus the amplitude and image
+    tensors."""
+    lidx, _, L, M = cfg.order_set()
+    N, B = len(lidx), len(cfg.offsets()) * n_clips
     lsx, lsy = cfg.lsmax
     ls, ms = np.meshgrid(np.arange(-lsx, lsx + 1), np.arange(-lsy, lsy + 1), indexing="ij")
     ninc = int(cfg.in_na(ls, ms).sum())
     mat, vec = 16 * N * N * B, 16 * N * B
+    lpx, lpy = cfg.lpmax
+    spectra = 16 * n_clips * ((4 * L + 1) * (4 * M + 1) + (2 * lpx + 1) * (2 * lpy + 1))
     em = 13 * mat + 8 * vec + 2 * 16 * N * ninc * B + 4 * N * B + 2 * MIB
     imaging = 272 * MIB
     nx, ny = cfg.image_shape
-    tensors = 16 * B * (2 * lsx + 1) * (2 * lsy + 1) * N + 3 * 8 * nx * ny
-    total = max(em, imaging) + tensors
+    tensors = 16 * B * (2 * lsx + 1) * (2 * lsy + 1) * N + 3 * 8 * nx * ny * n_clips
+    total = max(em, imaging) + spectra + tensors
     return -(-total // MIB) * MIB  # whole MiB (the library rounds reservations up to 256 bytes)
 
 
@@ -100,7 +114,11 @@ class _Slot:
         self.precision = precision
         self.resident = resident
         self.engines = {}
+        self.runs = {}  # (cfg, n) -> _Run of a plan for n clips
         self.reserved = 0
+
+    def bind(self):
+        _lib.check(self.lib.euvl_set_stream(self.h, ctypes.c_void_p(self.stream.cuda_stream)), "euvl_set_stream")
 
     def engine(self, cfg):
         eng = self.engines.get(cfg)
@@ -109,10 +127,29 @@ class _Slot:
             self.engines[cfg] = eng
         return eng
 
+    def run_for(self, cfg, n, decisions):
+        """The plan run (plan + device buffers) of n clips of cfg on this slot."""
+        from .batch import Plan, _Run
+        r = self.runs.get((cfg, n))
+        if r is None:
+            with torch.cuda.stream(self.stream):
+                self.bind()
+                plan = Plan(self.engine(cfg), n)
+                _lib.check(self.lib.euvl_plan_set_decisions(plan.pl, *decisions), "euvl_plan_set_decisions")
+                nbytes = cfg.ndivx * cfg.ndivy // 8
+                bits = torch.empty(n * nbytes, dtype=torch.uint8, device=self.device)
+                flags = torch.zeros(n * plan.n_off, dtype=torch.int32, device=self.device)
+                r = _Run(plan, list(range(n)), bits, self.device, flags)
+            self.runs[(cfg, n)] = r
+        return r
+
     def workspace(self):
         return int(self.lib.euvl_workspace_bytes(self.h))
 
     def close(self):
+        for r in self.runs.values():
+            r.plan.close()
+        self.runs = {}
         for e in self.engines.values():
             e.close()
         if self.h:
@@ -122,7 +159,7 @@ class _Slot:
 
 class Server:
     def __init__(self, n_slots=4, precision=_lib.PREC_MIXED, hbm_budget=None, device="cuda:0",
-                 socs_kernels=24, socs_xy=20):
+                 socs_kernels=24, socs_xy=20, max_batch=4):
         if not torch.cuda.is_available():
             raise _lib.EuvlError("a ROCm GPU is required")
         self.device = torch.device(device)
@@ -134,13 +171,50 @@ class Server:
         self.slots = [_Slot(i, self.device, precision, self.resident) for i in range(n_slots)]
         self.precision = precision
         self.socs_kernels, self.socs_xy = socs_kernels, socs_xy
+        self.max_batch = max_batch
         self.peak_reserved = 0
+        self.reruns = 0
+        self.batches = []  # sizes of the batches run by the last run()
 
     # ------------------------------------------------------------ residency
+    def decisions(self, cfg):
+        """Doubling count and Neumann length of the fixed-decision EM solve of
+        cfg: calibrated once (resident), on seeded masks of vacuum fraction
+        0.3, 0.5 and 0.7 in adaptive runs, with a factor-2 margin on the norm
+        bound behind the doubling count."""
+        key = ("decisions", cfg, self.precision)
+        got = self.resident.get(key)
+        if got is not None:
+            return got
+        from .batch import Plan
+        from .masks import manhattan, pack_bits
+        s0 = self.slots[0]
+        with torch.cuda.stream(s0.stream):
+            s0.bind()
+            plan = Plan(s0.engine(cfg), 1, alpha_margin=2.0)
+            try:
+                nb = cfg.ndivx * cfg.ndivy // 8
+                from .batch import _Run
+                r = _Run(plan, [0], torch.empty(nb, dtype=torch.uint8, device=self.device), self.device,
+                         torch.zeros(plan.n_off, dtype=torch.int32, device=self.device))
+                for k, dens in enumerate((0.3, 0.5, 0.7)):
+                    raw = pack_bits(manhattan(cfg.ndivx, cfg.ndivy, dens, 7919 + k))
+                    r.bits.copy_(torch.from_numpy(np.frombuffer(raw, dtype=np.uint8).copy()))
+                    r.launch(fixed=False)
+                dec = plan.decisions()
+            finally:
+                plan.close()
+        s0.stream.synchronize()
+        return self.resident.put(key, dec)
+
+    def batch_size(self, cfg):
+        return batch_size(cfg, self.max_batch)
+
     def warm(self, cfgs):
         """Build the resident state of every configuration (blank amplitudes,
-        SOCS kernels) on slot 0, and grow each slot's workspace to the largest
-        footprint that the budget allows, so that serving does not allocate."""
+        SOCS kernels, EM decisions) on slot 0, make every slot's plans, and
+        grow each slot's workspace to the largest batch footprint that the
+        budget allows, so that serving does not allocate."""
         s0 = self.slots[0]
         with torch.cuda.stream(s0.stream):
             for cfg in cfgs:
@@ -148,7 +222,12 @@ class Server:
                 eng.blank_amplitudes()
                 eng.eigenfunction(0.0, self.socs_kernels)
         s0.stream.synchronize()
-        need = max(footprint(c) for c in cfgs)
+        for cfg in cfgs:
+            dec = self.decisions(cfg)
+            for s in self.slots:
+                for n in range(1, self.batch_size(cfg) + 1):
+                    s.run_for(cfg, n, dec)
+        need = max(footprint(c, self.batch_size(c)) for c in cfgs)
         for s in self.slots:
             if self._reserved_total() - s.reserved + need <= self.budget:
                 _lib.check(self.lib_reserve(s, need), "euvl_reserve_workspace")
@@ -162,18 +241,30 @@ class Server:
     def _reserved_total(self):
         return sum(s.reserved for s in self.slots)
 
-    # ------------------------------------------------------------ one clip
-    def process(self, slot, req):
-        """M3D parameters, Abbe image and SOCS image of one request on one slot
-        (runs on the slot's stream; returns host arrays)."""
+    # ------------------------------------------------------------ one batch
+    def process(self, slot, reqs):
+        """M3D parameters, Abbe images and SOCS images of a batch of requests of
+        one problem on one slot (one batched plan run, then the SOCS image of
+        each clip; runs on the slot's stream; returns host arrays per request)."""
+        from .masks import pack_bits
+        cfg = reqs[0].cfg
+        run = slot.run_for(cfg, len(reqs), self.decisions(cfg))
+        raw = b"".join(r.mask if isinstance(r.mask, (bytes, bytearray)) else pack_bits(r.mask) for r in reqs)
+        nb = cfg.ndivx * cfg.ndivy // 8
         with torch.cuda.stream(slot.stream):
-            eng = slot.engine(req.cfg)
-            bits = eng.upload_mask(req.mask)
-            Ax = eng.amp_s(bits, "X")
-            a0, ax, ay = eng.m3d(bits, Ax0=Ax[0])
-            img = eng.abbe(bits, Ax=Ax)
-            socs = eng.socs(bits, a0, ax, ay, nsocs=self.socs_kernels, nsocsxy=self.socs_xy)
-            out = (a0.cpu().numpy(), img.cpu().numpy(), socs.cpu().numpy())
+            slot.bind()
+            run.bits.copy_(torch.from_numpy(np.frombuffer(raw, dtype=np.uint8).copy()))
+            run.launch(fixed=True)
+            if int(run.flags.max().item()) != 0:  # a device-side check failed: host decisions for this batch
+                self.reruns += 1
+                run.launch(fixed=False)
+            eng = slot.engine(cfg)
+            out = []
+            for k in range(len(reqs)):
+                bits = run.bits[k * nb:(k + 1) * nb]
+                socs = eng.socs(bits, run.a0[k], run.ax[k], run.ay[k], nsocs=self.socs_kernels,
+                                nsocsxy=self.socs_xy)
+                out.append((run.a0[k].cpu().numpy(), run.img[k].cpu().numpy(), socs.cpu().numpy()))
         slot.stream.synchronize()
         return out
 
@@ -182,23 +273,32 @@ class Server:
         """Serve `requests` (arrival times in seconds from now); returns the
         results in request order.  Exceptions of a worker are re-raised."""
         reqs = sorted(requests, key=lambda r: r.arrival)
-        need = {r.rid: footprint(r.cfg) for r in reqs}
-        if any(n > self.budget for n in need.values()):
-            raise _lib.EuvlError("a request needs more device memory than the HBM budget")
+        for c in {r.cfg for r in reqs}:
+            if footprint(c) > self.budget:
+                raise _lib.EuvlError("a request needs more device memory than the HBM budget")
+            self.decisions(c)
         pending = list(reqs)
         results, errors = {}, []
+        self.batches = []
         cv = threading.Condition()
         t0 = time.perf_counter()
         state = {"done": 0}
 
         def pick(slot, now):
-            # oldest arrived request whose footprint fits this slot under the budget
+            # the oldest arrived request whose batch fits this slot under the budget, with the
+            # later arrived requests of its problem (up to the batch size), largest batch that fits
             for i, r in enumerate(pending):
                 if r.arrival > now:
                     break
-                grow = max(0, need[r.rid] - slot.reserved)
-                if self._reserved_total() + grow <= self.budget:
-                    return pending.pop(i), grow
+                same = [j for j in range(i, len(pending)) if pending[j].arrival <= now and pending[j].cfg == r.cfg]
+                for n in range(min(len(same), self.batch_size(r.cfg)), 0, -1):
+                    grow = max(0, footprint(r.cfg, n) - slot.reserved)
+                    if self._reserved_total() + grow <= self.budget:
+                        take = same[:n]
+                        batch = [pending[j] for j in take]
+                        for j in reversed(take):
+                            pending.pop(j)
+                        return batch, grow
             return None, 0
 
         def worker(slot):
@@ -209,8 +309,8 @@ class Server:
                             if state["done"] == len(reqs) or errors:
                                 return
                             now = time.perf_counter() - t0
-                            req, grow = pick(slot, now)
-                            if req is not None:
+                            batch, grow = pick(slot, now)
+                            if batch is not None:
                                 if grow:
                                     slot.reserved += grow  # claim the growth before releasing the lock
                                     self.peak_reserved = max(self.peak_reserved, self._reserved_total())
@@ -221,12 +321,15 @@ class Server:
                         _lib.check(slot.lib.euvl_reserve_workspace(slot.h, ctypes.c_int64(slot.reserved)),
                                    "euvl_reserve_workspace")
                     start = time.perf_counter() - t0
-                    a0, img, socs = self.process(slot, req)
+                    outs = self.process(slot, batch)
                     finish = time.perf_counter() - t0
                     with cv:
                         slot.reserved = max(slot.reserved, slot.workspace())
-                        results[req.rid] = ClipResult(req.rid, slot.idx, req.arrival, start, finish, a0, img, socs)
-                        state["done"] += 1
+                        self.batches.append(len(batch))
+                        for req, (a0, img, socs) in zip(batch, outs):
+                            results[req.rid] = ClipResult(req.rid, slot.idx, req.arrival, start, finish, a0, img,
+                                                          socs)
+                        state["done"] += len(batch)
                         cv.notify_all()
             except BaseException as e:  # surface it to run()
                 with cv:

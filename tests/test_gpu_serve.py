@@ -1,6 +1,8 @@
 """GPU: the served form.  A burst of requests of different problems runs on
-several streams at once; every result equals the same request run alone
-(same precision mode), work of different requests overlaps in time, the
+several streams at once, requests of one problem batched into one plan run;
+every result equals the same request run alone (same precision mode, to
+1e-12 of the peak / largest value), work of different requests overlaps in
+time, the
 admission keeps the reserved device memory under the budget, and the
 footprint estimate covers what the library actually reserves."""
 import sys
@@ -60,9 +62,11 @@ def test_results_same_alone_and_in_the_mix():
         say(f"  overlapping pairs {overlaps // 2}, wall {srv.wall:.3f} s")
         assert overlaps > 0
         assert len({m.slot for m in mixed}) == 3
+        say(f"  batch sizes {srv.batches}, adaptive reruns {srv.reruns}")
+        assert max(srv.batches) > 1  # requests of one problem shared a batched run
         # the footprint estimate covers the workspace the library reserved
         for s in srv.slots:
-            assert s.workspace() <= max(footprint(c) for c in cfgs)
+            assert s.workspace() <= max(footprint(c, srv.batch_size(c)) for c in cfgs)
     finally:
         srv.close()
 
@@ -71,7 +75,7 @@ def test_admission_respects_budget():
     from euvlitho_amd import _lib
     from euvlitho_amd.serve import Server, poisson_stream, footprint
     cfgs = small_mix()
-    big = max(footprint(c) for c in cfgs)
+    big = max(footprint(c, 4) for c in cfgs)
     budget = int(1.5 * big)  # room for one large request at a time, not two
     say(f"budget {budget / 2**20:.0f} MiB, largest footprint {big / 2**20:.0f} MiB, 3 slots")
     srv = Server(n_slots=3, precision=_lib.PREC_MIXED, hbm_budget=budget)
