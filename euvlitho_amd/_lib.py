@@ -74,6 +74,8 @@ _SIGS = {
                             c_int64, c_int],
     "euvl_zcos_sinc_batched": [c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_int, P_int],
     "euvl_mask_spectrum": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    "euvl_mask_spectrum_noalloc": [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    "euvl_mask_spectrum_noalloc_max": [],
     "euvl_multilayer": [c_void_p, c_char, ctypes.POINTER(EmParams), c_int, c_void_p, c_void_p, c_void_p, c_void_p],
     "euvl_amp_s": [c_void_p, c_char, ctypes.POINTER(EmParams), P_int, P_int, c_int, c_int, c_int, c_void_p, c_int,
                    P_double, P_double, c_void_p, P_int],
@@ -135,7 +137,8 @@ def load():
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = ctypes.c_int64 if name in ("euvl_workspace_bytes", "euvl_plan_workspace_bytes") else c_int
+        fn.restype = ctypes.c_int64 if name in ("euvl_workspace_bytes", "euvl_plan_workspace_bytes",
+                                                "euvl_mask_spectrum_noalloc_max") else c_int
     lib.euvl_last_error.argtypes = []
     lib.euvl_last_error.restype = ctypes.c_char_p
     _lib = lib

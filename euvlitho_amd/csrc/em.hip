@@ -20,6 +20,7 @@
 #include <complex>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ew.h"
@@ -322,8 +323,14 @@ void dft_twiddles(int nx, int ny, std::vector<double2>& t) {
 
 int mask_spectrum_enqueue(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
                           double2* out, double2* scratch, double2* part, const double2* tw) {
+  return mask_spectrum_splits(h, bits, nx, ny, n_clips, lw, mw, out, scratch, part, tw,
+                              dft_splits(nx, ny, mw, n_clips));
+}
+
+// the same with the number S of y-range splits given (part holds S partial sums when S > 1)
+int mask_spectrum_splits(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
+                         double2* out, double2* scratch, double2* part, const double2* tw, int S) {
   const int nm = 2 * mw + 1;
-  const int S = dft_splits(nx, ny, mw, n_clips);
   const int ks = ((ny + S - 1) / S + 3) / 4 * 4;
   const dim3 g1((nx + DFT_TILE - 1) / DFT_TILE, (nm + DFT_TILE - 1) / DFT_TILE, S * n_clips);
   hipLaunchKernelGGL(mask_dft_y_kernel, g1, dim3(256), 0, h->stream, bits, nx, ny, mw, tw + nx + 1,
@@ -370,6 +377,77 @@ extern "C" int euvl_mask_spectrum(euvl_handle h, const uint8_t* d_bits, int ndiv
   double2* pp = part ? (double2*)euvl::ws_take(h, part) : nullptr;
   double2* t = (double2*)euvl::ws_take(h, tw);
   return euvl::mask_spectrum(h, d_bits, ndivx, ndivy, n_clips, lw, mw, (double2*)d_out, s, pp, t);
+}
+
+// Scratch of the allocation-free mask spectrum: a fixed device array (64 MiB
+// per GPU, held until the process ends).  Host code never uses the name as an
+// address: the device address comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr int64_t NOALLOC_SCRATCH = (int64_t)1 << 22;  // double2 entries
+static __device__ __align__(256) double2 scratch[NOALLOC_SCRATCH];
+std::mutex g_scratch_mu;
+double2* g_scratch_ptr[64] = {};
+
+int scratch_for_device(int dev, double2** out) {
+  if (dev < 0 || dev >= 64) {
+    euvl::set_error("euvl_mask_spectrum_noalloc: device %d out of range", dev);
+    return EUVL_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      euvl::set_error("euvl_mask_spectrum_noalloc: hipGetSymbolAddress: %s", hipGetErrorString(e));
+      return EUVL_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = (double2*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return EUVL_OK;
+}
+}  // namespace
+
+extern "C" int64_t euvl_mask_spectrum_noalloc_max(void) { return NOALLOC_SCRATCH * (int64_t)sizeof(double2); }
+
+extern "C" int euvl_mask_spectrum_noalloc(euvl_handle h, const uint8_t* d_bits, int ndivx, int ndivy, int n_clips,
+                                          int lw, int mw, double* d_out) {
+  if (!h || !d_bits || !d_out || ndivx <= 0 || ndivy <= 0 || n_clips <= 0 || n_clips > 65535 || lw < 0 || mw < 0 ||
+      ((int64_t)ndivx * ndivy) % 8 != 0 || (int64_t)n_clips * (2 * lw + 1) * (2 * mw + 1) >= ((int64_t)1 << 31)) {
+    euvl::set_error("euvl_mask_spectrum_noalloc: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  int dev = -1;
+  EUVL_HIP_CHECK(hipGetDevice(&dev));
+  if (dev != h->device) {
+    euvl::set_error("euvl_mask_spectrum_noalloc: current device %d is not the handle's device %d", dev, h->device);
+    return EUVL_ERR_ARG;
+  }
+  double2* base = nullptr;
+  EUVL_TRY(scratch_for_device(dev, &base));
+  // layout in the scratch array: twiddles, the y sums, then the split partials;
+  // every entry read is written first in the same call, so nothing needs clearing
+  const int64_t tw = ndivx + ndivy + 2, tw_al = (tw + 15) / 16 * 16;
+  const int64_t per = (int64_t)n_clips * ndivx * (2 * mw + 1), per_al = (per + 15) / 16 * 16;
+  if (tw_al + per_al > NOALLOC_SCRATCH) {
+    euvl::set_error("euvl_mask_spectrum_noalloc: %d clips of %d x %d px with %d orders need %lld bytes of scratch, "
+                    "above the maximum of %lld", n_clips, ndivx, ndivy, 2 * mw + 1,
+                    (long long)(16 * (tw_al + per_al)), (long long)(16 * NOALLOC_SCRATCH));
+    return EUVL_ERR_ARG;
+  }
+  // as many y-range splits as the scratch holds (the same count as euvl_mask_spectrum when it fits)
+  int S = euvl::dft_splits(ndivx, ndivy, mw, n_clips);
+  while (S > 1 && tw_al + per_al * (1 + S) > NOALLOC_SCRATCH) --S;
+  double2* t = base;
+  double2* sc = base + tw_al;
+  double2* part = S > 1 ? sc + per_al : nullptr;
+  std::vector<double2> host;
+  euvl::dft_twiddles(ndivx, ndivy, host);
+  EUVL_HIP_CHECK(hipMemcpyAsync(t, host.data(), sizeof(double2) * host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(euvl::mask_spectrum_splits(h, d_bits, ndivx, ndivy, n_clips, lw, mw, (double2*)d_out, sc, part, t, S));
+  // the host table must outlive the copy
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return EUVL_OK;
 }
 
 extern "C" int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n, const double* d_kx,

@@ -92,6 +92,8 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
 // twiddle tables already on the device (tw: nx + 1 then ny + 1 entries); no synchronisation
 int mask_spectrum_enqueue(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
                           double2* out, double2* scratch, double2* part, const double2* tw);
+int mask_spectrum_splits(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clips, int lw, int mw,
+                         double2* out, double2* scratch, double2* part, const double2* tw, int S);
 int dft_splits(int nx, int ny, int mw, int n_clips);
 void dft_twiddles(int nx, int ny, std::vector<double2>& t);
 

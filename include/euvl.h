@@ -182,6 +182,20 @@ typedef struct {
 int euvl_mask_spectrum(euvl_handle h, const uint8_t* d_bits, int ndivx, int ndivy, int n_clips,
                        int lw, int mw, double* d_out);
 
+/* euvl_mask_spectrum without any allocation: no reference counterpart; it is
+ * for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results (to rounding:
+ * the y sums may be split into fewer partial sums).  Its scratch is one fixed
+ * device array of euvl_mask_spectrum_noalloc_max() bytes (64 MiB) per GPU,
+ * held until the process ends; requests that need more (twiddles plus
+ * n_clips x ndivx x (2mw+1) complex) fail with EUVL_ERR_ARG.  Every call on a
+ * GPU shares that array: calls that overlap on one GPU (several threads or
+ * streams) race and must be serialised by the caller.  The current device
+ * must be the handle's. */
+int euvl_mask_spectrum_noalloc(euvl_handle h, const uint8_t* d_bits, int ndivx, int ndivy, int n_clips, int lw,
+                               int mw, double* d_out);
+int64_t euvl_mask_spectrum_noalloc_max(void);
+
 /* Diagonal T-matrix cascade of the multilayer for n modes with transverse
  * wave vectors (kx, ky) (device arrays of n doubles), polarisation 'X' or 'Y'.
  * Writes d_uu, d_ub (n complex each).  Replaces multilayerS
