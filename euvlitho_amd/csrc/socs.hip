@@ -5,7 +5,7 @@
 // TCC_w = sum_c P_c diag(w) P_c^H / S is one MFMA GEMM per weight (the three
 // polarisation components stacked along K); the eigendecomposition is a
 // parallel two-sided Jacobi method (round-robin pairs, n/2 rotations per
-// step applied as one row pass and one column pass).
+// step applied in one out-of-place launch over 2 x 2 blocks).
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -56,71 +56,91 @@ __global__ void tcc_phase_kernel(TccSetup a, const int* pupil, int ncut, const d
   }
 }
 
-// Jacobi rotation parameters for the pairs of one round-robin step:
-// block [[a, b], [conj b, d]], b = |b| e^{i phi}: J = diag(1, e^{-i phi}) G,
-// G the real Jacobi rotation of [[a, |b|], [|b|, d]] (c, s with t = tan theta).
-// rot[b][k] = (c, s, cos phi, sin phi)
-__global__ void jacobi_params_kernel(int n, int npair, const int* pairs, const double2* A, double4_t* rot,
-                                     int batch) {
-  const int64_t total = (int64_t)npair * batch;
+// The Jacobi rotation of pair (p, q) of the Hermitian matrix M: block
+// [[a, b], [conj b, d]], b = |b| e^{i phi}: J = diag(1, e^{-i phi}) G, G the real
+// Jacobi rotation of [[a, |b|], [|b|, d]] (c, s with t = tan theta).
+// Returns (c, s, cos phi, sin phi); the identity for a bye (p or q < 0).
+__device__ inline double4_t pair_rotation(const double2* M, int n, int p, int q) {
+  double4_t r = {1.0, 0.0, 1.0, 0.0};
+  if (p < 0 || q < 0) return r;
+  const double a = M[(int64_t)p * n + p].x, d = M[(int64_t)q * n + q].x;
+  const double2 bq = M[(int64_t)p * n + q];
+  const double ab = hypot(bq.x, bq.y);
+  if (ab > 1e-300 && ab > 1e-18 * (fabs(a) + fabs(d))) {
+    const double tau = (d - a) / (2.0 * ab);
+    const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+    const double c = 1.0 / sqrt(1.0 + t * t);
+    r = (double4_t){c, t * c, bq.x / ab, bq.y / ab};
+  }
+  return r;
+}
+
+// One round-robin step in one launch, out of place (Ain -> Aout, Vin -> Vout):
+// thread (k, l) rotates the 2 x 2 block rows {p_k, q_k} x columns {p_l, q_l}
+// from both sides (rows by J_k^H, then columns by J_l); the pairs partition the
+// indices, so every element is written by exactly one thread, and reading only
+// the input copy makes the parameters of every pair those of the step's start
+// (rows first, then columns: J_k^H A J_l on each block).  Threads past the
+// npair^2 blocks rotate columns {p_l, q_l} of row i of V.
+__global__ void jacobi_step_kernel(int n, int npair, const int* pairs, const double2* __restrict__ Ain,
+                                   double2* __restrict__ Aout, const double2* __restrict__ Vin,
+                                   double2* __restrict__ Vout, int batch) {
+  const int64_t nblk = (int64_t)npair * npair, nv = (int64_t)n * npair, per = nblk + nv;
+  const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int b = (int)(e / npair), k = (int)(e % npair);
-    const int p = pairs[2 * k], q = pairs[2 * k + 1];
-    double4_t r = {1.0, 0.0, 1.0, 0.0};
-    if (p >= 0 && q >= 0) {
-      const double2* M = A + (int64_t)b * n * n;
-      const double a = M[(int64_t)p * n + p].x, d = M[(int64_t)q * n + q].x;
-      const double2 bq = M[(int64_t)p * n + q];
-      const double ab = hypot(bq.x, bq.y);
-      if (ab > 1e-300 && ab > 1e-18 * (fabs(a) + fabs(d))) {
-        const double tau = (d - a) / (2.0 * ab);
-        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
-        const double c = 1.0 / sqrt(1.0 + t * t);
-        r = (double4_t){c, t * c, bq.x / ab, bq.y / ab};
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const double2* M = Ain + (int64_t)b * n * n;
+    if (r < nblk) {
+      const int k = (int)(r / npair), l = (int)(r % npair);
+      const int rows[2] = {pairs[2 * k], pairs[2 * k + 1]}, cols[2] = {pairs[2 * l], pairs[2 * l + 1]};
+      const double4_t Rk = pair_rotation(M, n, rows[0], rows[1]);
+      const double4_t Rl = pair_rotation(M, n, cols[0], cols[1]);
+      double2 X[2][2];
+      for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v)
+          X[u][v] = (rows[u] >= 0 && cols[v] >= 0) ? M[(int64_t)rows[u] * n + cols[v]] : make_double2(0.0, 0.0);
+      if (rows[0] >= 0 && rows[1] >= 0 && Rk[1] != 0.0) {
+        const double2 ep = make_double2(Rk[2], Rk[3]);  // e^{i phi_k}
+        for (int v = 0; v < 2; ++v) {
+          const double2 x = X[0][v], ey = cmul(ep, X[1][v]);
+          X[0][v] = csub(cscale(x, Rk[0]), cscale(ey, Rk[1]));
+          X[1][v] = cadd(cscale(x, Rk[1]), cscale(ey, Rk[0]));
+        }
       }
-    }
-    rot[e] = r;
-  }
-}
-
-// rows p, q <- J^H rows: (x, y) -> (c x - s e^{i phi} y, s x + c e^{i phi} y)
-__global__ void jacobi_rows_kernel(int n, int npair, const int* pairs, double2* A, const double4_t* rot, int batch) {
-  const int64_t total = (int64_t)npair * n * batch;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int b = (int)(e / ((int64_t)npair * n));
-    const int64_t r = e - (int64_t)b * npair * n;
-    const int k = (int)(r / n), j = (int)(r % n);
-    const int p = pairs[2 * k], q = pairs[2 * k + 1];
-    if (p < 0 || q < 0) continue;
-    const double4_t R = rot[(int64_t)b * npair + k];
-    if (R[1] == 0.0) continue;
-    double2* M = A + (int64_t)b * n * n;
-    const double2 x = M[(int64_t)p * n + j], y = M[(int64_t)q * n + j];
-    const double2 ey = cmul(make_double2(R[2], R[3]), y);  // e^{i phi} y
-    M[(int64_t)p * n + j] = csub(cscale(x, R[0]), cscale(ey, R[1]));
-    M[(int64_t)q * n + j] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
-  }
-}
-
-// columns p, q <- columns J: (x, y) -> (c x - s e^{-i phi} y, s x + c e^{-i phi} y), for A and V
-__global__ void jacobi_cols_kernel(int n, int npair, const int* pairs, double2* A, double2* V, const double4_t* rot,
-                                   int batch) {
-  const int64_t total = (int64_t)npair * n * batch;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int b = (int)(e / ((int64_t)npair * n));
-    const int64_t r = e - (int64_t)b * npair * n;
-    const int i = (int)(r / npair), k = (int)(r % npair);
-    const int p = pairs[2 * k], q = pairs[2 * k + 1];
-    if (p < 0 || q < 0) continue;
-    const double4_t R = rot[(int64_t)b * npair + k];
-    if (R[1] == 0.0) continue;
-    const double2 em = make_double2(R[2], -R[3]);  // e^{-i phi}
-    for (int w = 0; w < 2; ++w) {
-      double2* M = (w == 0 ? A : V) + (int64_t)b * n * n + (int64_t)i * n;
-      const double2 x = M[p], y = M[q];
-      const double2 ey = cmul(em, y);
-      M[p] = csub(cscale(x, R[0]), cscale(ey, R[1]));
-      M[q] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
+      if (cols[0] >= 0 && cols[1] >= 0 && Rl[1] != 0.0) {
+        const double2 em = make_double2(Rl[2], -Rl[3]);  // e^{-i phi_l}
+        for (int u = 0; u < 2; ++u) {
+          const double2 x = X[u][0], ey = cmul(em, X[u][1]);
+          X[u][0] = csub(cscale(x, Rl[0]), cscale(ey, Rl[1]));
+          X[u][1] = cadd(cscale(x, Rl[1]), cscale(ey, Rl[0]));
+        }
+      }
+      double2* O = Aout + (int64_t)b * n * n;
+      for (int u = 0; u < 2; ++u)
+        for (int v = 0; v < 2; ++v)
+          if (rows[u] >= 0 && cols[v] >= 0) O[(int64_t)rows[u] * n + cols[v]] = X[u][v];
+    } else {
+      const int64_t rv = r - nblk;
+      const int i = (int)(rv / npair), l = (int)(rv % npair);
+      const int p = pairs[2 * l], q = pairs[2 * l + 1];
+      const double2* Vi = Vin + (int64_t)b * n * n + (int64_t)i * n;
+      double2* Vo = Vout + (int64_t)b * n * n + (int64_t)i * n;
+      if (p < 0 || q < 0) {
+        const int c = p < 0 ? q : p;
+        if (c >= 0) Vo[c] = Vi[c];
+        continue;
+      }
+      const double4_t R = pair_rotation(M, n, p, q);
+      const double2 x = Vi[p], y = Vi[q];
+      if (R[1] == 0.0) {
+        Vo[p] = x;
+        Vo[q] = y;
+        continue;
+      }
+      const double2 ey = cmul(make_double2(R[2], -R[3]), y);
+      Vo[p] = csub(cscale(x, R[0]), cscale(ey, R[1]));
+      Vo[q] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
     }
   }
 }
@@ -366,17 +386,17 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
   const size_t o_src = up.add(h_src, sizeof(double) * 2 * S);
   const size_t mat = sizeof(double2) * (size_t)n * n * B;
   const size_t pb = sizeof(double2) * (size_t)n * 3 * S;
-  EUVL_TRY(ws_reserve(h, up.host.size() + 2 * mat + 4 * pb + sizeof(double4_t) * npair * B + sizeof(double) * n * B +
-                             sizeof(int) * nk * B + 8192));
+  EUVL_TRY(ws_reserve(h, up.host.size() + 4 * mat + 4 * pb + sizeof(double) * n * B + sizeof(int) * nk * B + 8192));
   char* blk = (char*)ws_take(h, up.host.size());
   double2* A = (double2*)ws_take(h, mat);
   double2* V = (double2*)ws_take(h, mat);
+  double2* A2 = (double2*)ws_take(h, mat);
+  double2* V2 = (double2*)ws_take(h, mat);
   double2* P = (double2*)ws_take(h, pb);
   double2* Q = (double2*)ws_take(h, 3 * pb);
-  double4_t* rot = (double4_t*)ws_take(h, sizeof(double4_t) * npair * B);
   double* w = (double*)ws_take(h, sizeof(double) * n * B);
   int* order_d = (int*)ws_take(h, sizeof(int) * nk * B);
-  if (!A || !V || !P || !Q || !rot || !w || !order_d) {
+  if (!A || !V || !A2 || !V2 || !P || !Q || !w || !order_d) {
     set_error("euvl_tcc_kernels: workspace carve failed");
     return EUVL_ERR_NOMEM;
   }
@@ -391,9 +411,15 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
   hipLaunchKernelGGL(eye_kernel, grid1((int64_t)n * n * B), dim3(256), 0, h->stream, n, V, B);
   EUVL_LAUNCH_CHECK();
   const int* d_pairs = (const int*)(blk + o_pairs);
+  // Stop once the squared off-diagonal / total Frobenius ratio is at the
+  // rounding floor of an n x n matrix, (2 n eps)^2, or stops falling below
+  // 1e3 times it; Jacobi converges quadratically, so this takes few sweeps.
+  const double eps = 2.220446049250313e-16;
+  const double floor2 = (2.0 * n * eps) * (2.0 * n * eps);
+  const int64_t step_threads = ((int64_t)npair * npair + (int64_t)n * npair) * B;
   int sweep = 0;
-  double worst = INFINITY;
-  for (sweep = 0; sweep < 30; ++sweep) {
+  double worst = INFINITY, prev = INFINITY;
+  for (sweep = 0; sweep <= 40; ++sweep) {
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, h->stream, h->d_red, 2 * B);
     EUVL_LAUNCH_CHECK();
     hipLaunchKernelGGL(offdiag_kernel, dim3(256, B), dim3(256), 0, h->stream, n, A, h->d_red, B);
@@ -406,21 +432,20 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
       const double r = (tot > 0.0) ? off / tot : 0.0;
       worst = std::max(worst, std::isnan(r) ? INFINITY : r);
     }
-    if (worst < 1e-30) break;
+    if (worst <= floor2 || (worst <= 1e3 * floor2 && worst > 0.25 * prev) || sweep == 40) break;
+    prev = worst;
     for (int st = 0; st < nsteps; ++st) {
       const int* pr = d_pairs + (size_t)st * npair * 2;
-      hipLaunchKernelGGL(jacobi_params_kernel, grid1((int64_t)npair * B), dim3(256), 0, h->stream, n, npair, pr, A,
-                         rot, B);
-      hipLaunchKernelGGL(jacobi_rows_kernel, grid1((int64_t)npair * n * B), dim3(256), 0, h->stream, n, npair, pr, A,
-                         rot, B);
-      hipLaunchKernelGGL(jacobi_cols_kernel, grid1((int64_t)npair * n * B), dim3(256), 0, h->stream, n, npair, pr, A,
-                         V, rot, B);
+      hipLaunchKernelGGL(jacobi_step_kernel, grid1(step_threads), dim3(256), 0, h->stream, n, npair, pr, A, A2, V,
+                         V2, B);
+      std::swap(A, A2);
+      std::swap(V, V2);
     }
     EUVL_LAUNCH_CHECK();
   }
   if (h_sweeps) *h_sweeps = sweep;
-  if (!(worst < 1e-26)) {
-    set_error("euvl_tcc_kernels: Jacobi off-diagonal ratio %.3e after %d sweeps", worst, sweep);
+  if (!(worst <= 1e3 * floor2)) {
+    set_error("euvl_tcc_kernels: Jacobi off-diagonal ratio %.3e after %d sweeps (floor %.3e)", worst, sweep, floor2);
     return EUVL_ERR_NOT_CONVERGED;
   }
   // eigenvalues, stable sort by |alpha| descending (the reference's bubble sort order), top nk
