@@ -21,8 +21,13 @@ class EuvlError(RuntimeError):
 # euvl_set_precision bits (include/euvl.h)
 PREC_FP32_MATFUN, PREC_FP32_COUPLING, PREC_FP32_LU, PREC_FP32_GEMM = 1, 2, 4, 8
 PREC_GEMM_4M = 16  # complex products with 4 real MFMAs, FP64 and FP32 (default: 3, the zgemm3m form)
+PREC_FP32_IMAGING = 32  # Abbe and SOCS transforms in FP32 (x pass on FP32 MFMAs), compensated sums
 PREC_FP64 = 0
-PREC_MIXED = PREC_FP32_MATFUN | PREC_FP32_COUPLING | PREC_FP32_LU
+# the precision form: mode-coupling and top-solve products and the whole imaging in FP32; the matrix
+# functions of the absorber operator (the eigendecomposition's counterpart) and the multilayer cascade in FP64
+PREC_MIXED = PREC_FP32_COUPLING | PREC_FP32_LU | PREC_FP32_IMAGING
+# every product of the EM solve in FP32 too (the matrix functions as well): faster, larger error
+PREC_FAST = PREC_FP32_MATFUN | PREC_FP32_COUPLING | PREC_FP32_LU | PREC_FP32_IMAGING
 
 
 class EmParams(ctypes.Structure):

@@ -102,7 +102,7 @@ extern "C" int64_t euvl_workspace_bytes(euvl_handle h) { return h ? (int64_t)h->
 
 extern "C" int euvl_set_precision(euvl_handle h, int mask) {
   const int all = EUVL_PREC_FP32_MATFUN | EUVL_PREC_FP32_COUPLING | EUVL_PREC_FP32_LU | EUVL_PREC_FP32_GEMM |
-                  EUVL_PREC_GEMM_4M;
+                  EUVL_PREC_GEMM_4M | EUVL_PREC_FP32_IMAGING;
   if (!h || (mask & ~all)) {
     euvl::set_error("euvl_set_precision: unknown bits in mask 0x%x", mask);
     return EUVL_ERR_ARG;

@@ -132,6 +132,96 @@ __global__ void abbe_xreduce_kernel(int S, int64_t per, const double* __restrict
   }
 }
 
+// ---- FP32 imaging (EUVL_PREC_FP32_IMAGING) ----
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ void to_float2_kernel(int64_t n, const double2* in, float2* out) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+    out[e] = make_float2((float)in[e].x, (float)in[e].y);
+}
+
+// G32[s][c][l][y] = sum_m F[s][c][l][m] exp(+2 pi i (m mod ny) y / ny), in FP32
+// (fields rounded to FP32 as they are read; the pupil fields themselves are
+// formed in FP64, whose propagation phases need it)
+__global__ void abbe_ypass32_kernel(int n_sc, int nl, int nm, int lpy, int ny, const double2* F, const float2* twy,
+                                    float2* G) {
+  const int64_t total = (int64_t)n_sc * nl * ny;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(e % ny);
+    const int64_t sl = e / ny;
+    const double2* f = F + sl * nm;
+    float re = 0.0f, im = 0.0f;
+    for (int mm = 0; mm < nm; ++mm) {
+      const int m = mm - lpy;
+      const float2 t = twy[(int)((((int64_t)m * y) % ny + ny) % ny)];
+      const float fr = (float)f[mm].x, fi = (float)f[mm].y;
+      re = fmaf(fr, t.x, fmaf(-fi, t.y, re));
+      im = fmaf(fr, t.y, fmaf(fi, t.x, im));
+    }
+    G[e] = make_float2(re, im);
+  }
+}
+
+// The x pass on FP32 MFMAs: per source/component sc the field of a 16 x 16
+// pixel tile is the complex product tw_x (16 x nl) G[sc] (nl x 16) on
+// v_mfma_f32_16x16x4_f32 (4 real products per k-step, FP32 accumulation over
+// l); |E|^2 is added over the sc of the split with Kahan compensation in FP32,
+// and the split's sum goes out in FP64 (P[s][x][y], then summed in split order
+// by abbe_xreduce_kernel).  A workgroup of 4 waves takes 32 x by 32 y; lane l
+// feeds A = tw(x0 + l%16, k0 + l/16), B = G(k0 + l/16, y0 + l%16), and holds
+// D(x0 + 4 (l/16) + r, y0 + l%16).
+constexpr int XM_T = 32;
+__global__ __launch_bounds__(256) void abbe_xpass_mfma_kernel(int n_sc, int nl, int lpx, int nx, int ny,
+                                                              const float2* __restrict__ G,
+                                                              const float2* __restrict__ twx, int scs,
+                                                              double* __restrict__ P) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int xw = blockIdx.y * XM_T + (w >> 1) * 16, yw = blockIdx.x * XM_T + (w & 1) * 16;
+  const int s = blockIdx.z;
+  const int nlp = (nl + 3) & ~3;
+  extern __shared__ float2 sh_twf[];  // [XM_T][nlp] twiddles of this block's x values (zero past nl)
+  for (int e = threadIdx.x; e < XM_T * nlp; e += blockDim.x) {
+    const int xb = e / nlp, kk = e % nlp;
+    const int64_t x = blockIdx.y * XM_T + xb;
+    float2 t = make_float2(0.0f, 0.0f);
+    if (kk < nl) t = twx[(int)((((int64_t)(kk - lpx) * x) % nx + nx) % nx)];
+    sh_twf[e] = t;
+  }
+  __syncthreads();
+  const int y = yw + li;
+  const bool yin = y < ny;
+  float4_t acc = {0.0f, 0.0f, 0.0f, 0.0f}, cmp = {0.0f, 0.0f, 0.0f, 0.0f};
+  const int sc1 = min(n_sc, (s + 1) * scs);
+  const float2* tw = sh_twf + (int64_t)((w >> 1) * 16 + li) * nlp;
+  for (int sc = s * scs; sc < sc1; ++sc) {
+    const float2* g = G + (int64_t)sc * nl * ny;
+    float4_t dre = {0.0f, 0.0f, 0.0f, 0.0f}, dim = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < nlp; k0 += 4) {
+      const int k = k0 + lk;
+      const float2 a = tw[k];
+      const float2 b = (k < nl && yin) ? g[(int64_t)k * ny + y] : make_float2(0.0f, 0.0f);
+      dre = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, dre, 0, 0, 0);
+      dre = __builtin_amdgcn_mfma_f32_16x16x4f32(-a.y, b.y, dre, 0, 0, 0);
+      dim = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, dim, 0, 0, 0);
+      dim = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, dim, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const float v = dre[r] * dre[r] + dim[r] * dim[r];
+      const float yk = v - cmp[r];
+      const float t = acc[r] + yk;
+      cmp[r] = (t - acc[r]) - yk;
+      acc[r] = t;
+    }
+  }
+  if (!yin) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int x = xw + 4 * lk + r;
+    if (x < nx) P[((int64_t)s * nx + x) * ny + y] = (double)acc[r] - (double)cmp[r];
+  }
+}
+
 // splits of the source/component range: about 4096 waves in all (four per
 // SIMD, so that the G loads of one wave overlap another's arithmetic)
 int xpass_splits(int n_sc, int nx, int ny) {
@@ -158,6 +248,21 @@ int xpass(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const double
 __device__ inline bool in_na_d(const M3dSetup& a, int l, int m) {
   const double u = l * a.mx / a.dx, v = m * a.my / a.dy, c = a.na / a.lambda;
   return u * u + v * v <= c * c;
+}
+
+int xpass32(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const float2* G, const float2* twx, double w,
+            double* img, double* P) {
+  const int64_t blocks = (int64_t)((ny + XM_T - 1) / XM_T) * ((nx + XM_T - 1) / XM_T);
+  const int S0 = (int)std::max<int64_t>(1, std::min<int64_t>((1024 + blocks - 1) / blocks, std::min(n_sc, 64)));
+  const int scs = (n_sc + S0 - 1) / S0, S = (n_sc + scs - 1) / scs;
+  const int nlp = (nl + 3) & ~3;
+  hipLaunchKernelGGL(abbe_xpass_mfma_kernel, dim3((ny + XM_T - 1) / XM_T, (nx + XM_T - 1) / XM_T, S), dim3(256),
+                     sizeof(float2) * XM_T * nlp, h->stream, n_sc, nl, lpx, nx, ny, G, twx, scs, P);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(abbe_xreduce_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, S, (int64_t)nx * ny, P,
+                     w, img);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
 }
 
 // blockIdx.y = clip: fmask, Ax and the outputs advance by their clip strides
@@ -211,8 +316,24 @@ __global__ void m3d_fit_kernel(M3dSetup a, const double2* fmask, const double2* 
       sy2 += ms * ms;
       sxy += ls * ms;
     }
-  if (xin) {
-    const double cd = c1 * sx2 * sy2 + 2.0 * sx * sxy * sy - c1 * sxy * sxy - sx2 * sy * sy - sy2 * sx * sx;
+  const double cd = c1 * sx2 * sy2 + 2.0 * sx * sxy * sy - c1 * sxy * sxy - sx2 * sy * sy - sy2 * sx * sx;
+  if (xin && cd == 0.0) {
+    // Deviation: the reference divides by cd = 0 (NaN/Inf) when its snum >= 8
+    // sources lie on one row or one column (the sums of half-integer positions
+    // are exact, so cd is exactly 0 then).  Here the slope along the direction
+    // that varies is fitted with a0 and the other slope is 0.
+    const double vx = c1 * sx2 - sx * sx, vy = c1 * sy2 - sy * sy;
+    ax[n] = ay[n] = make_double2(0.0, 0.0);
+    if (vx > 0.0) {
+      ax[n] = cscale(csub(cscale(fx, c1), cscale(f1, sx)), 1.0 / vx);
+      a0[n] = cscale(csub(f1, cscale(ax[n], sx)), 1.0 / c1);
+    } else if (vy > 0.0) {
+      ay[n] = cscale(csub(cscale(fy, c1), cscale(f1, sy)), 1.0 / vy);
+      a0[n] = cscale(csub(f1, cscale(ay[n], sy)), 1.0 / c1);
+    } else {
+      a0[n] = cscale(f1, 1.0 / c1);
+    }
+  } else if (xin) {
     const double i = 1.0 / cd;
     a0[n] = cscale(cadd(cadd(cscale(f1, sx2 * sy2 - sxy * sxy), cscale(fx, -sx * sy2 + sy * sxy)),
                         cscale(fy, sx * sxy - sy * sx2)), i);
@@ -374,32 +495,48 @@ int abbe_chunk(int n_src, int nl, int nm, int ny) {
   return std::max(1, std::min(chunk, n_src));
 }
 
+// bytes of the x pass's split partials: at most 64 splits in either precision
+size_t xpass_partial_bytes(int n_sc, int nx, int ny) {
+  return sizeof(double) * (size_t)std::min(n_sc, 64) * nx * ny;
+}
+
 size_t abbe_workspace_bytes(int n_src, int lpx, int lpy, int nx, int ny) {
   const int nl = 2 * lpx + 1, nm = 2 * lpy + 1;
   const int chunk = abbe_chunk(n_src, nl, nm, ny);
   const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
-  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
-  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
-  return fbytes + gbytes + pbytes + 3 * 256;
+  const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;  // FP64 or FP32 (and its twiddles)
+  return fbytes + gbytes + xpass_partial_bytes(chunk * 3, nx, ny) + sizeof(float2) * (nx + ny) + 4 * 256;
 }
 
 // Abbe images of n_clips clips of one problem: clip c reads Ax + c ax_stride and
 // accumulates weight x |E|^2 into img + c img_stride (zeroed first when `zero`).
 // Takes its spectrum buffers from the workspace.
+// Abbe images of n_clips clips of one problem: clip c reads Ax + c ax_stride and
+// accumulates weight x |E|^2 into img + c img_stride (zeroed first when `zero`).
+// With EUVL_PREC_FP32_IMAGING the transforms run in FP32 (the x pass on FP32
+// MFMAs, compensated sums over source points).  Takes its spectrum buffers
+// from the workspace.
 int abbe_enqueue(euvl_handle h, const AbbeSetup& a, const double2* Ax, int64_t ax_stride, int n_clips,
                  const ImagingDev& t, int n_src, int ncut, int nx, int ny, double weight, double* img,
                  int64_t img_stride, bool zero) {
+  const bool f32 = (h->prec_mask & EUVL_PREC_FP32_IMAGING) != 0;
   const int nl = 2 * a.lpx + 1, nm = 2 * a.lpy + 1;
   const int chunk = abbe_chunk(n_src, nl, nm, ny);
   const size_t fbytes = sizeof(double2) * (size_t)chunk * 3 * nl * nm;
   const size_t gbytes = sizeof(double2) * (size_t)chunk * 3 * nl * ny;
-  const size_t pbytes = sizeof(double) * (size_t)xpass_splits(chunk * 3, nx, ny) * nx * ny;
   double2* F = (double2*)ws_take(h, fbytes);
   double2* G = (double2*)ws_take(h, gbytes);
-  double* Pxp = (double*)ws_take(h, pbytes);
-  if (!F || !G || !Pxp) {
+  double* Pxp = (double*)ws_take(h, xpass_partial_bytes(chunk * 3, nx, ny));
+  float2* tw32 = f32 ? (float2*)ws_take(h, sizeof(float2) * (nx + ny)) : nullptr;
+  if (!F || !G || !Pxp || (f32 && !tw32)) {
     set_error("abbe image: workspace too small");
     return EUVL_ERR_NOMEM;
+  }
+  if (f32) {
+    hipLaunchKernelGGL(to_float2_kernel, grid1(nx), dim3(256), 0, h->stream, (int64_t)nx, t.twx, tw32);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(to_float2_kernel, grid1(ny), dim3(256), 0, h->stream, (int64_t)ny, t.twy, tw32 + nx);
+    EUVL_LAUNCH_CHECK();
   }
   for (int c = 0; c < n_clips; ++c) {
     double* im = img + c * img_stride;
@@ -413,10 +550,18 @@ int abbe_enqueue(euvl_handle h, const AbbeSetup& a, const double2* Ax, int64_t a
       hipLaunchKernelGGL(abbe_field_kernel, grid1((int64_t)ns * ncut), dim3(256), 0, h->stream, a,
                          Ax + c * ax_stride, t.map, t.off, t.src + 3 * s0, ns, t.pup2, ncut, F);
       EUVL_LAUNCH_CHECK();
-      hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3, nl,
-                         nm, a.lpy, ny, F, t.twy, G);
-      EUVL_LAUNCH_CHECK();
-      EUVL_TRY(xpass(h, ns * 3, nl, a.lpx, nx, ny, G, t.twx, weight, im, Pxp));
+      if (f32) {
+        float2* G32 = (float2*)G;
+        hipLaunchKernelGGL(abbe_ypass32_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3,
+                           nl, nm, a.lpy, ny, F, tw32 + nx, G32);
+        EUVL_LAUNCH_CHECK();
+        EUVL_TRY(xpass32(h, ns * 3, nl, a.lpx, nx, ny, G32, tw32, weight, im, Pxp));
+      } else {
+        hipLaunchKernelGGL(abbe_ypass_kernel, grid1((int64_t)ns * 3 * nl * ny), dim3(256), 0, h->stream, ns * 3,
+                           nl, nm, a.lpy, ny, F, t.twy, G);
+        EUVL_LAUNCH_CHECK();
+        EUVL_TRY(xpass(h, ns * 3, nl, a.lpx, nx, ny, G, t.twx, weight, im, Pxp));
+      }
     }
   }
   return EUVL_OK;

@@ -345,6 +345,76 @@ __global__ void socs_up_x_kernel(int nout, int nx, int ny, const double2* T, con
   img[e] = acc;
 }
 
+// ---- FP32 variants (EUVL_PREC_FP32_IMAGING): field transforms in FP32, the
+// kernel sum of the small image with Kahan compensation in FP32 ----
+__global__ void socs_tw32_kernel(int n, const double2* tw, float2* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) out[e] = make_float2((float)tw[e].x, (float)tw[e].y);
+}
+
+__global__ void socs_ypass32_kernel(int nf, int nl, int nm, int lpy, int nout, const double2* F, const float2* tw,
+                                    float2* G) {
+  const int64_t total = (int64_t)nf * nl * nout;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int y = (int)(e % nout);
+    const double2* f = F + (e / nout) * nm;
+    float re = 0.0f, im = 0.0f;
+    for (int mm = 0; mm < nm; ++mm) {
+      const float2 t = tw[((((int64_t)(mm - lpy)) * y) % nout + nout) % nout];
+      const float fr = (float)f[mm].x, fi = (float)f[mm].y;
+      re = fmaf(fr, t.x, fmaf(-fi, t.y, re));
+      im = fmaf(fr, t.y, fmaf(fi, t.x, im));
+    }
+    G[e] = make_float2(re, im);
+  }
+}
+
+__global__ void socs_xpass32_kernel(int nf, int nl, int lpx, int nout, const float2* G, const float2* tw,
+                                    float2* fld) {
+  const int64_t total = (int64_t)nf * nout * nout;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int f = (int)(e / ((int64_t)nout * nout));
+    const int64_t r = e - (int64_t)f * nout * nout;
+    const int x = (int)(r / nout), y = (int)(r % nout);
+    float re = 0.0f, im = 0.0f;
+    for (int l = 0; l < nl; ++l) {
+      const float2 t = tw[(((int64_t)(l - lpx) * x) % nout + nout) % nout];
+      const float2 g = G[((int64_t)f * nl + l) * nout + y];
+      re = fmaf(t.x, g.x, fmaf(-t.y, g.y, re));
+      im = fmaf(t.x, g.y, fmaf(t.y, g.x, im));
+    }
+    fld[e] = make_float2(re, im);
+  }
+}
+
+__device__ inline void kahan_add(float& s, float& c, float v) {
+  const float y = v - c, t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
+__device__ inline float re_dot32(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
+
+__global__ void socs_combine32_kernel(int nout, int n0, int nxy, const float2* fld, const double* alpha, int nk,
+                                      double* small) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= nout * nout) return;
+  const int64_t np = (int64_t)nout * nout;
+  float acc = 0.0f, cmp = 0.0f;
+  for (int m = 0; m < n0; ++m) {
+    const float2 f0 = fld[m * np + pix];
+    kahan_add(acc, cmp, (float)alpha[m] * re_dot32(f0, f0));
+    if (m < nxy) {
+      const float2* g = fld + (n0 + 5 * (int64_t)m) * np + pix;
+      const float2 f0x = g[0], f0y = g[np], fx = g[2 * np], fy = g[3 * np], fxy = g[4 * np];
+      kahan_add(acc, cmp, 2.0f * (float)alpha[nk + m] * re_dot32(f0x, fx));
+      kahan_add(acc, cmp, 2.0f * (float)alpha[2 * nk + m] * re_dot32(f0y, fy));
+      kahan_add(acc, cmp, 2.0f * (float)alpha[m] * re_dot32(f0, fxy));
+    }
+  }
+  small[pix] = (double)acc - (double)cmp;
+}
+
 std::vector<std::complex<double>> twiddles(int n, double sign) {
   std::vector<std::complex<double>> t(n);
   const std::complex<double> zi(0.0, 1.0);
@@ -560,20 +630,39 @@ extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const dou
   hipLaunchKernelGGL(socs_amps_kernel, dim3((ncut + 255) / 256), dim3(256), 0, h->stream, a, (const double2*)d_fmask,
                      (const double2*)d_a0, (const double2*)d_ax, (const double2*)d_ay, pup, ncut, amps);
   EUVL_LAUNCH_CHECK();
+  const bool f32 = (h->prec_mask & EUVL_PREC_FP32_IMAGING) != 0;
   if (nf > 0) {
     EUVL_TRY(fill(h, (int64_t)nf * nl * nm, F, make_double2(0.0, 0.0)));
     hipLaunchKernelGGL(socs_spectra_kernel, grid1((int64_t)nf * ncut), dim3(256), 0, h->stream, a, nf,
                        (const int*)(blk + o_fd), amps, (const double2*)d_phi, nk, pup, ncut, F);
     EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(socs_ypass_kernel, grid1((int64_t)nf * nl * nout), dim3(256), 0, h->stream, nf, nl, nm, lpy,
-                       nout, F, (const double2*)(blk + o_two), G);
-    EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(socs_xpass_kernel, grid1((int64_t)nf * nout * nout), dim3(256), 0, h->stream, nf, nl, lpx,
-                       nout, G, (const double2*)(blk + o_two), fld);
-    EUVL_LAUNCH_CHECK();
+    if (f32) {
+      // FP32 copies of the twiddles go after the FP32 fields (half of the FP64 field buffer)
+      float2* tw32 = (float2*)T;
+      hipLaunchKernelGGL(socs_tw32_kernel, dim3((nout + 255) / 256), dim3(256), 0, h->stream, nout,
+                         (const double2*)(blk + o_two), tw32);
+      EUVL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(socs_ypass32_kernel, grid1((int64_t)nf * nl * nout), dim3(256), 0, h->stream, nf, nl, nm,
+                         lpy, nout, F, tw32, (float2*)G);
+      EUVL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(socs_xpass32_kernel, grid1((int64_t)nf * nout * nout), dim3(256), 0, h->stream, nf, nl,
+                         lpx, nout, (const float2*)G, tw32, (float2*)fld);
+      EUVL_LAUNCH_CHECK();
+    } else {
+      hipLaunchKernelGGL(socs_ypass_kernel, grid1((int64_t)nf * nl * nout), dim3(256), 0, h->stream, nf, nl, nm,
+                         lpy, nout, F, (const double2*)(blk + o_two), G);
+      EUVL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(socs_xpass_kernel, grid1((int64_t)nf * nout * nout), dim3(256), 0, h->stream, nf, nl, lpx,
+                         nout, G, (const double2*)(blk + o_two), fld);
+      EUVL_LAUNCH_CHECK();
+    }
   }
-  hipLaunchKernelGGL(socs_combine_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, nsocs,
-                     nsocsxy, fld, d_alpha, nk, small);
+  if (f32 && nf > 0)
+    hipLaunchKernelGGL(socs_combine32_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, nsocs,
+                       nsocsxy, (const float2*)fld, d_alpha, nk, small);
+  else
+    hipLaunchKernelGGL(socs_combine_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, nsocs,
+                       nsocsxy, fld, d_alpha, nk, small);
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(socs_spec_y_kernel, dim3((nout * nout + 255) / 256), dim3(256), 0, h->stream, nout, small,
                      (const double2*)(blk + o_twm), Rp);

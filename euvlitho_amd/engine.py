@@ -48,7 +48,9 @@ class Resident:
 class Engine:
     def __init__(self, cfg: Config, device="cuda:0", precision=_lib.PREC_FP64, handle=None, resident=None):
         """precision: euvl_set_precision mask (0 = the FP64 drop-in; _lib.PREC_MIXED =
-        every product of the EM solve on FP32 MFMA, storage and the rest in FP64).
+        the coupling and top-solve products on FP32 MFMA and the imaging in FP32, the
+        matrix functions and the multilayer cascade in FP64; _lib.PREC_FAST = every
+        product of the EM solve on FP32 MFMA too).
         handle: an existing euvl handle to run on (not owned: close() leaves it);
         resident: a Resident store shared between engines for the mask-independent
         state (blank amplitudes, SOCS kernels)."""

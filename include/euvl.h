@@ -83,6 +83,12 @@ int euvl_gemm_stats(euvl_handle h, int enable, double* out);
  * theirs in norm ((|Ar| + |Ai|)(|Br| + |Bi|) in place of |A||B|).
  * EUVL_PREC_GEMM_4M selects the 4-multiplication form in both precisions. */
 #define EUVL_PREC_GEMM_4M 16
+/* Image formation in FP32: the Abbe transforms (the y pass in FP32, the x pass
+ * on FP32 MFMAs with Kahan-compensated sums of |E|^2 over the source points of
+ * each split, the split sums in FP64) and the SOCS field transforms and kernel
+ * sums (FP32, compensated).  Pupil fields and their propagation phases are
+ * formed in FP64 and rounded. */
+#define EUVL_PREC_FP32_IMAGING 32
 int euvl_set_precision(euvl_handle h, int mask);
 
 /* GEMM autotuner: times every kernel variant (tile shape, waves per

@@ -163,8 +163,20 @@ def m3d_params(p, d, mask2d, lidx, midx, Ax, vcxx, abxx):
                 cx2 += ls * ls
                 cy2 += ms * ms
                 cxy += ls * ms
-        if sn >= 8:
-            cd = c1 * cx2 * cy2 + 2.0 * cx * cxy * cy - c1 * cxy * cxy - cx2 * cy * cy - cy2 * cx * cx
+        cd = c1 * cx2 * cy2 + 2.0 * cx * cxy * cy - c1 * cxy * cxy - cx2 * cy * cy - cy2 * cx * cx
+        if sn >= 8 and cd == 0.0:
+            # deviation from m3d.cpp:451-457, which divides by cd = 0 when the sources lie on one
+            # row or column: fit the slope along the direction that varies, the other slope is 0
+            vx, vy = c1 * cx2 - cx * cx, c1 * cy2 - cy * cy
+            if vx > 0:
+                ax[n] = (c1 * fx - cx * f1) / vx
+                a0[n] = (f1 - ax[n] * cx) / c1
+            elif vy > 0:
+                ay[n] = (c1 * fy - cy * f1) / vy
+                a0[n] = (f1 - ay[n] * cy) / c1
+            else:
+                a0[n] = f1 / c1
+        elif sn >= 8:
             a0[n] = ((cx2 * cy2 - cxy * cxy) * f1 + (-cx * cy2 + cy * cxy) * fx + (cx * cxy - cy * cx2) * fy) / cd
             ax[n] = ((-cx * cy2 + cy * cxy) * f1 + (c1 * cy2 - cy * cy) * fx + (-c1 * cxy + cx * cy) * fy) / cd
             ay[n] = ((cx * cxy - cy * cx2) * f1 + (-c1 * cxy + cx * cy) * fx + (c1 * cx2 - cx * cx) * fy) / cd

@@ -21,6 +21,20 @@ def valid_sources(lp, mp, lsx, lsy, nsx, nsy, in_na):
     return out
 
 
+def identifiable(lin, min_, snum, lsx, lsy, nsx, nsy, in_na):
+    """Per pupil point, which of (ax, ay) the fit determines: both where
+    snum >= 8 and the sources span two dimensions; one where they lie on a
+    single row (ms constant: ax) or column (ls constant: ay); none where
+    snum < 8 (the mean)."""
+    out = np.zeros((len(lin), 2), dtype=bool)
+    for q, (lp, mp, sn) in enumerate(zip(lin, min_, snum)):
+        if sn < 8:
+            continue
+        v = np.array([(ls, ms) for _, _, ls, ms in valid_sources(int(lp), int(mp), lsx, lsy, nsx, nsy, in_na)])
+        out[q] = (np.ptp(v[:, 0]) > 0, np.ptp(v[:, 1]) > 0)
+    return out
+
+
 def linear_amplitudes(lin, min_, lidx, midx, lsx, lsy, nsx, nsy, in_na, phase, a0, ax, ay):
     """Ax[is, js, n] = phase[is, js] (a0 + ax ls + ay ms) of the pupil point
     whose order l = lp + is - lsx, m = mp + js - lsy is mode n.  Each

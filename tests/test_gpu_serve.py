@@ -33,12 +33,23 @@ def small_mix():
             high_na(512, 12, ANNULAR, ndivs=1, image=(128, 128))]
 
 
-def test_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("mix", ["small", "served"])
+def test_results_same_alone_and_in_the_mix(mix):
     from euvlitho_amd import _lib
-    from euvlitho_amd.serve import Server, poisson_stream, footprint
-    cfgs = small_mix()
-    reqs = poisson_stream(9, 0.0, seed=3, cfgs=cfgs)  # rate 0: all arrive at once
-    say(f"9 requests over {len(cfgs)} problems, 3 slots, mixed precision")
+    from euvlitho_amd.serve import Server, poisson_stream, footprint, served_configs
+    cfgs = small_mix() if mix == "small" else served_configs()
+    n = 9 if mix == "small" else 12
+    reqs = poisson_stream(n, 0.0, seed=3, cfgs=cfgs)  # rate 0: all arrive at once
+    if mix == "served":  # every problem of the served stream at least once
+        reqs = reqs[:6]
+        for i, c in enumerate(cfgs):
+            reqs[i].cfg = c
+            from euvlitho_amd.masks import manhattan
+            reqs[i].mask = manhattan(c.ndivx, c.ndivy, 0.5, 100 + i)
+        reqs += poisson_stream(6, 0.0, seed=5, cfgs=cfgs)
+        for i, r in enumerate(reqs):
+            r.rid = i
+    say(f"{len(reqs)} requests over {len(cfgs)} problems, 3 slots, mixed precision")
     srv = Server(n_slots=3, precision=_lib.PREC_MIXED)
     try:
         srv.warm(cfgs)

@@ -271,6 +271,39 @@ def test_m3d_fit_recovers_linear_data_exactly():
     assert np.abs(got[1][~big]).max() == 0 and np.abs(got[2][~big]).max() == 0
 
 
+def test_m3d_fit_single_row_sources():
+    """768 nm, +-16 orders: pupil points where snum >= 8 sources lie on one
+    row of the lattice make the reference's 3 x 3 system singular
+    (m3d.cpp:451-457 divides by zero there).  The fit takes the slope along
+    the row with a0 and sets the other to 0: exact recovery of data without
+    the unidentifiable slope, and every output finite."""
+    from tests import m3d_synth as S
+    p = P.hna(768, 16, dict(type=1, sigma1=0.8, sigma2=0.3, openangle=90.0))
+    d = G.derived(p)
+    li, mi, L, M = G.order_set(p, "rect", K=16)
+    lin, min_, snum = G.pupil_points(p, d)
+    nsx, nsy = d["nsourceX"], d["nsourceY"]
+    in_na = lambda l_, m_: G.in_na(l_, m_, p, d)
+    ok = S.identifiable(lin, min_, snum, d["lsmaxX"], d["lsmaxY"], nsx, nsy, in_na)
+    big = snum >= 8
+    single = big & (ok.sum(axis=1) == 1)
+    assert single.sum() >= 8
+    rng = np.random.default_rng(8)
+    n = len(lin)
+    a = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x = np.where(ok[:, 0], rng.normal(size=n) + 1j * rng.normal(size=n), 0)
+    y = np.where(ok[:, 1], rng.normal(size=n) + 1j * rng.normal(size=n), 0)
+    vc = rng.normal(size=(nsx, nsy)) + 1j * rng.normal(size=(nsx, nsy))
+    ab = 0.3 * vc
+    m = rand_mask(768, 768, 5)
+    Ax = S.linear_amplitudes(lin, min_, li, mi, d["lsmaxX"], d["lsmaxY"], nsx, nsy, in_na, vc / np.abs(vc), a, x, y)
+    got = OA.m3d_params(p, d, m, li, mi, Ax, vc, ab)
+    zero = OA.m3d_params(p, d, m, li, mi, np.zeros_like(Ax), vc, ab)
+    for g, z, t in zip(got, zero, (a, x, y)):
+        assert np.isfinite(g).all()
+        assert np.abs((g - z) - t).max() < 1e-12 * np.abs(t).max()
+
+
 def test_m3d_fit_equals_lstsq():
     """On random amplitudes the fit equals numpy.linalg.lstsq on [1, ls, ms]
     (snum >= 8) and the mean (snum < 8)."""
