@@ -158,6 +158,147 @@ def cpu_baseline(cfg, masks, threads, reps=3):
                               "points, 4 divisions, Abbe 256x256"})
 
 
+def batched_clips(n_clips=64, seed=0):
+    """The uneven-clip workload: n_clips seeded Manhattan clips, each of a
+    problem drawn from the mix (pitch 256-1024 nm, orders +-8..+-20, 61-200
+    source points) with a vacuum fraction drawn from U(0.3, 0.7)."""
+    from euvlitho_amd.config import workload_mix
+    from euvlitho_amd.masks import manhattan
+    cfgs = workload_mix()
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_clips):
+        cfg = cfgs[int(rng.integers(len(cfgs)))]
+        out.append((cfg, manhattan(cfg.ndivx, cfg.ndivy, float(rng.uniform(0.3, 0.7)), int(rng.integers(1 << 30)))))
+    return out
+
+
+def run_batched(n_clips=64, reps=5, precision=None, with_eager=True):
+    """Batched form: the uneven-clip workload in batched plan runs captured
+    into one HIP graph (one launch per pass over all clips), FP64 unless a
+    precision mask is given.  Roofline: 164 N^3 per (clip, division) solve over
+    the pass's wall time.  Beside it: the same clips through the eager
+    per-clip path (Engine, host decisions), and the largest difference
+    between the two (M3D parameters of max|a0|, images of peak)."""
+    import torch
+    from euvlitho_amd import _lib
+    from euvlitho_amd.batch import BatchedEngine
+    from euvlitho_amd.engine import Engine
+    prec = _lib.PREC_FP64 if precision is None else precision
+    clips = batched_clips(n_clips)
+    be = BatchedEngine(precision=prec)
+    t0 = time.perf_counter()
+    be.load(clips)
+    setup_s = time.perf_counter() - t0
+    be.launch()
+    out = be.results()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        be.launch()
+        be.results()  # synchronises, reads the flags once, reruns flagged clips
+        walls.append(time.perf_counter() - t0)
+    out = [[t.cpu().numpy() for t in r] for r in be.results()]
+    flops = sum(EM_FLOPS_PER_N3 * float(len(c.order_set()[0])) ** 3 * len(c.offsets()) for c, _ in clips)
+    wall = float(np.median(walls))
+    res = {"value": n_clips / wall, "unit": "clips/s", "clips": n_clips, "ms_per_pass": 1000 * wall,
+           "p50_pass_ms": 1000 * float(np.percentile(walls, 50)), "p99_pass_ms": 1000 * float(np.percentile(walls, 99)),
+           "runs": len(be.runs), "plans": len(be.plans), "graph_nodes": be.graph_nodes(), "graph_launches_per_pass": 1,
+           "adaptive_reruns": be.reruns, "setup_s": setup_s,
+           "decisions": {f"{p.cfg.ndivx}nm K{p.cfg.orders[1]} x{p.n_clips}": p.decisions() for p in be.plans},
+           "problems": {f"{c.ndivx}nm K{c.orders[1]} {sum(len(x[2]) for x in c.source_points())}pts "
+                        f"{len(c.offsets())}div": sum(1 for cc, _ in clips if cc is c) for c in dict.fromkeys(
+                            cc for cc, _ in clips)},
+           "roofline": {"bound": "mfma", "achieved": flops / wall / 1e12, "peak": FP64_PEAK_TFLOPS if not prec
+                        else FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": flops / wall / 1e12 / (FP64_PEAK_TFLOPS if not prec else FP32_PEAK_TFLOPS),
+                        "flops_counted": "164 N^3 per (clip, division) solve over the whole pass (wall)"}}
+    be.close()
+    if with_eager:
+        engs = {}
+        for c, _ in clips:
+            if c not in engs:
+                engs[c] = Engine(c, precision=prec)
+                engs[c].blank_amplitudes()
+        bits = [engs[c].upload_mask(m) for c, m in clips]
+        torch.cuda.synchronize()
+
+        def eager():
+            r = []
+            for (c, _), b in zip(clips, bits):
+                e = engs[c]
+                Ax = e.amp_s(b, "X")
+                a0, ax, ay = e.m3d(b, Ax0=Ax[0])
+                r.append((a0, ax, ay, e.abbe(b, Ax=Ax)))
+            torch.cuda.synchronize()
+            return r
+
+        eager()
+        t0 = time.perf_counter()
+        ref = eager()
+        te = time.perf_counter() - t0
+        em, ei = 0.0, 0.0
+        for g, r in zip(out, ref):
+            r = [t.cpu().numpy() for t in r]
+            s = np.abs(r[0]).max()
+            em = max(em, max(float(np.abs(g[i] - r[i]).max() / s) for i in range(3)))
+            ei = max(ei, float(np.abs(g[3] - r[3]).max() / r[3].max()))
+        res["eager_per_clip"] = {"value": n_clips / te, "unit": "clips/s", "ms_per_pass": 1000 * te}
+        res["err_vs_eager"] = {"m3d_of_max_a0": em, "image_of_peak": ei}
+        for e in engs.values():
+            e.close()
+    return res
+
+
+def ref_default_leg(reps=3):
+    """The reference's own m3d.cpp path on its mask (tests/golden/ref_mask.bin,
+    2048 nm, NA 0.33, 4x/4x): the two blank solves on the narrow ellipse set
+    (N = 1009), the wide solve on the diamond set (N = 8705) and the M3D fit,
+    per mask, FP64 -- the run that the reference's README quotes at about
+    one minute per mask.  Roofline: 164 N^3 per solve over the GEMM time of
+    the run (HIP events), and over its wall time."""
+    import ctypes
+    import torch
+    from euvlitho_amd import _lib
+    from euvlitho_amd.config import reference_default
+    from euvlitho_amd.engine import Engine, Resident
+    cfg = reference_default()
+    eng = Engine(cfg)
+    with open(os.path.join(ROOT, "tests", "golden", "ref_mask.bin"), "rb") as f:
+        bits = eng.upload_mask(f.read())
+
+    def run():
+        eng.resident = Resident()  # the reference solves the blank masks for every mask
+        Ax = eng.amp_s(bits, "X")
+        return eng.m3d(bits, Ax0=Ax[0])
+
+    run()
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    stats = (ctypes.c_double * 8)()
+    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
+    run()
+    _lib.check(eng.lib.euvl_gemm_stats(eng.h, 0, stats), "euvl_gemm_stats")
+    N, Nb = len(eng.lidx), len(cfg.blank_order_set()[0])
+    eng.close()
+    wall = float(np.median(walls))
+    flops = EM_FLOPS_PER_N3 * (float(N) ** 3 + 2.0 * float(Nb) ** 3)
+    ach = flops / (stats[1] * 1e-3) / 1e12
+    return {"value": 1.0 / wall, "unit": "masks/s", "s_per_mask": wall, "runs_s": walls,
+            "config": f"reference default: 2048 nm mask (tests/golden/ref_mask.bin), N={N} wide solve + 2 blank "
+                      f"solves N={Nb} + M3D fit, X polarisation, FP64",
+            "published": "~1 min per mask (reference README.md:32, hardware not stated)",
+            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel", "achieved": ach,
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS,
+                         "flops_counted": "164 N^3 per solve (3 solves)", "gemm_ms_per_mask": stats[1],
+                         "gemm_launches": int(stats[0]), "whole_run_tflops": flops / wall / 1e12}}
+
+
 def mask_producer(n_masks=64, ndiv=2048, reps=5, with_cpu=True):
     """The line/space mask program (mask.cpp) on the GPU: n_masks 2048 x 2048
     clips per call (host draws + one raster launch).  Raster roofline: the
@@ -351,8 +492,8 @@ def stcc_leg(reps=5, with_cpu=True):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="time the FP64 form only (profiling runs)")
     ap.add_argument("--served", type=int, default=2000,
@@ -444,6 +585,7 @@ def main():
 
     f64 = run_form(_lib.PREC_FP64)
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
+    fast = None if args.no_mixed else run_form(_lib.PREC_FAST)
     ms = 1000.0 * f64["wall"] / args.steps
     launches, gemm_ms, gemm_flops, gemm_bytes = f64["launches"], f64["gemm_ms"], f64["gemm_flops"], f64["gemm_bytes"]
     executed = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
@@ -453,37 +595,50 @@ def main():
     achieved = algo_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
     traffic = measured_traffic()
     lat = f64["lat"]
-    mixed = None
-    if mix is not None:
-        # mixed form: its error against the FP64 form on the same clip (amplitudes, a0, Abbe, SOCS)
-        (A64, a064, I64, S64), (A32, a032, I32, S32) = f64["outs"], mix["outs"]
-        # the FP32-product launches only (the form keeps some products in FP64)
-        mix_ach = mix["f32_flops"] / (mix["f32_ms"] * 1e-3) / 1e12 if mix["f32_ms"] > 0 else 0.0
-        mix_traffic = measured_traffic("mixed")
-        mixed = {
-            "value": args.steps / mix["wall"], "unit": "clips/s", "ms_per_step": 1000.0 * mix["wall"] / args.steps,
-            "p50_ms": 1000.0 * float(np.percentile(mix["lat"], 50)),
-            "p99_ms": 1000.0 * float(np.percentile(mix["lat"], 99)),
-            "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
-            "em_solve": mix["em"],
+
+    def form_line(r, dtype, traffic_key):
+        """A precision form against the FP64 form on the same clip: rate,
+        latency, errors per quantity, and the GEMM launches split by the
+        precision their products ran in."""
+        (A64, a064, I64, S64), (A32, a032, I32, S32) = f64["outs"], r["outs"]
+        f32_ms, f64_ms = r["f32_ms"], r["gemm_ms"] - r["f32_ms"]
+        f32_l, f64_l = r["f32_launches"], r["launches"] - r["f32_launches"]
+        f64_flops = r["gemm_flops"] - r["f32_flops"]
+        t = measured_traffic(traffic_key)
+        return {
+            "value": args.steps / r["wall"], "unit": "clips/s", "ms_per_step": 1000.0 * r["wall"] / args.steps,
+            "p50_ms": 1000.0 * float(np.percentile(r["lat"], 50)),
+            "p99_ms": 1000.0 * float(np.percentile(r["lat"], 99)),
+            "dtype": dtype, "em_solve": r["em"],
             "err_vs_fp64": {"abbe_image_of_peak": float(np.abs(I32 - I64).max() / I64.max()),
                             "socs_image_of_peak": float(np.abs(S32 - S64).max() / np.abs(S64).max()),
                             "amplitudes_of_max": float(np.abs(A32 - A64).max() / np.abs(A64).max()),
                             "a0_of_max": float(np.abs(a032 - a064).max() / np.abs(a064).max())},
-            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel<F32M3>", "achieved": mix_ach,
-                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": mix_ach / FP32_PEAK_TFLOPS,
-                         "avg_launch_us": 1000.0 * mix["f32_ms"] / max(mix["f32_launches"], 1),
-                         "launches_per_clip": int(mix["f32_launches"]),
-                         "traffic": mix_traffic.get("traffic_bytes_per_launch"),
-                         "traffic_source": mix_traffic.get("source"),
-                         "algorithmic_bytes_per_launch": mix["f32_bytes"] / max(mix["f32_launches"], 1),
-                         "fp64_launches_per_clip": int(mix["launches"] - mix["f32_launches"]),
-                         "gemm_ms_per_clip": mix["gemm_ms"], "f32_gemm_ms_per_clip": mix["f32_ms"],
-                         "autotuned_variant": mix["tuned"]["fp32"][0],
-                         "flops_counted": "8mnk per complex GEMM",
-                         "complex_form": "3M (3 real MFMAs per complex product)",
-                         "mfma_issue_frac": 0.75 * mix_ach / FP32_PEAK_TFLOPS},
+            "roofline": {"bound": "mfma", "kernel": "zgemm_mfma_kernel",
+                         "achieved": algo_flops / (r["gemm_ms"] * 1e-3) / 1e12, "unit": "TFLOP/s",
+                         "flops_counted": "164 N^3 per solve over the clip's GEMM time (FP64 and FP32 launches)",
+                         "peak": FP64_PEAK_TFLOPS if f64_ms >= f32_ms else FP32_PEAK_TFLOPS,
+                         "frac": algo_flops / (r["gemm_ms"] * 1e-3) / 1e12 /
+                                 (FP64_PEAK_TFLOPS if f64_ms >= f32_ms else FP32_PEAK_TFLOPS),
+                         "fp32_launches": {"launches_per_clip": int(f32_l), "ms_per_clip": f32_ms,
+                                           "executed_8mnk_tflops": r["f32_flops"] / max(f32_ms, 1e-9) / 1e9,
+                                           "avg_launch_us": 1000.0 * f32_ms / max(f32_l, 1),
+                                           "algorithmic_bytes_per_launch": r["f32_bytes"] / max(f32_l, 1),
+                                           "traffic": t.get("traffic_bytes_per_launch"),
+                                           "traffic_source": t.get("source")},
+                         "fp64_launches": {"launches_per_clip": int(f64_l), "ms_per_clip": f64_ms,
+                                           "executed_8mnk_tflops": f64_flops / max(f64_ms, 1e-9) / 1e9},
+                         "autotuned_variant": r["tuned"].get("fp32", (None,))[0],
+                         "complex_form": "3M (3 real MFMAs per complex product)"},
         }
+
+    mixed = None
+    if mix is not None:
+        mixed = form_line(mix, "f32 MFMA for the coupling / top-solve products (FP32 accumulation) and the imaging "
+                               "(x pass on FP32 MFMA, Kahan-compensated source sums); f64 matrix functions, "
+                               "multilayer cascade and storage", "mixed")
+        mixed["fast_variant"] = form_line(fast, "f32 MFMA for every EM product (matrix functions too) and the "
+                                                "imaging; f64 multilayer and storage", "mixed")
     tcc_ms = f64["tcc_ms"]
 
     line = {

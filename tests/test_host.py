@@ -130,3 +130,17 @@ def test_presets_have_the_workload_point_counts(name, count, n_off):
     for (a, b, l0, m0), (_, _, fl, fm) in zip(divs, full):
         assert set(zip(l0.tolist(), m0.tolist())) <= set(zip(fl.tolist(), fm.tolist()))
     assert [len(d[2]) for d in preset(name).source_points()] == [len(d[2]) for d in divs]
+
+
+def test_bench_defines_every_leg():
+    """bench.py imports without a GPU and defines every leg main() calls."""
+    import ast
+    import bench
+    tree = ast.parse(open(os.path.join(ROOT, "bench.py")).read())
+    main = next(n for n in tree.body if isinstance(n, ast.FunctionDef) and n.name == "main")
+    called = {n.func.id for n in ast.walk(main) if isinstance(n, ast.Call) and isinstance(n.func, ast.Name)}
+    local = {n.name for n in ast.walk(main) if isinstance(n, ast.FunctionDef)}
+    local |= {a.asname or a.name for n in ast.walk(main) if isinstance(n, ast.ImportFrom) for a in n.names}
+    import builtins
+    missing = [c for c in called if not hasattr(bench, c) and not hasattr(builtins, c) and c not in local]
+    assert not missing, missing
