@@ -55,7 +55,7 @@ def copy_calibration(mib=512, reps=3):
     print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps}), flush=True)
 
 
-def run_served(n_req, slots):
+def run_served(n_req, slots, precision=None):
     """Served form: requests of six problems (pitch 256-1024 nm, orders +-8..+-20,
     61-200 source points), mixed precision, `slots` streams with admission by
     device-memory footprint; arrived requests of one problem share a batched
@@ -64,7 +64,8 @@ def run_served(n_req, slots):
     from euvlitho_amd import _lib
     from euvlitho_amd.serve import Server, poisson_stream, served_configs
     cfgs = served_configs()
-    srv = Server(n_slots=slots, precision=_lib.PREC_MIXED)
+    prec = _lib.PREC_MIXED if precision is None else precision
+    srv = Server(n_slots=slots, precision=prec)
     try:
         t = time.perf_counter()
         srv.warm(cfgs)
@@ -79,7 +80,12 @@ def run_served(n_req, slots):
                 "offered_rate": rate, "burst_capacity": cap, "slots": slots,
                 "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
                 "service_p50_ms": 1000.0 * float(np.percentile(svc, 50)),
-                "dtype": "f32 MFMA products (FP32 accumulation), f64 storage and everything else",
+                "dtype": ("f32 MFMA for the coupling / top-solve products and the imaging; f64 matrix functions, "
+                          "multilayer and storage" if prec == _lib.PREC_MIXED else
+                          "f32 MFMA for every EM product and the imaging; f64 multilayer and storage"),
+                "per_problem_ms": {name: [round(1000.0 * float(np.percentile(l, 50)), 1),
+                                          round(1000.0 * float(np.percentile(l, 99)), 1)]
+                                   for name, l in per_problem(res, cfgs).items()},
                 "resident_build_s": warm_s, "peak_reserved_gib": srv.peak_reserved / 2**30,
                 "batches": len(srv.batches), "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
                 "adaptive_reruns": srv.reruns, "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
@@ -88,6 +94,17 @@ def run_served(n_req, slots):
                                                        sum(len(x[2]) for x in c.source_points())) for c in cfgs]}
     finally:
         srv.close()
+
+
+def per_problem(res, cfgs):
+    """Latencies (s) of the served results grouped by problem."""
+    from euvlitho_amd.serve import poisson_stream  # noqa: F401
+    out = {}
+    for r in res:
+        c = r.cfg
+        name = "%dnm K%d %dpts" % (c.ndivx, c.orders[1], sum(len(x[2]) for x in c.source_points()))
+        out.setdefault(name, []).append(r.latency)
+    return out
 
 
 def measured_traffic(form="fp64"):
@@ -508,6 +525,8 @@ def main():
     ap.add_argument("--no-ref-default", action="store_true", help="skip the reference-default m3d.cpp leg")
     ap.add_argument("--batched", type=int, default=64, help="clips of the batched form (0 = skip)")
     ap.add_argument("--batched-only", action="store_true", help="only run the batched form")
+    ap.add_argument("--only-form", choices=["fp64", "mixed", "fast"],
+                    help="profiling runs: time one precision form on the workload and print its GEMM launch figures")
     ap.add_argument("--mask-producer-only", action="store_true", help="only run the mask producer leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -583,6 +602,16 @@ def main():
                     gemm_flops=stats[2], gemm_bytes=stats[3], f32_launches=stats[4], f32_ms=stats[5],
                     f32_flops=stats[6], f32_bytes=stats[7], tuned=tuned, em=em)
 
+    if args.only_form:
+        r = run_form({"fp64": _lib.PREC_FP64, "mixed": _lib.PREC_MIXED, "fast": _lib.PREC_FAST}[args.only_form])
+        print(json.dumps({"form": args.only_form, "clips_per_s": args.steps / r["wall"],
+                          "launches_per_clip": int(r["launches"]), "f32_launches_per_clip": int(r["f32_launches"]),
+                          "gemm_ms_per_clip": r["gemm_ms"], "f32_gemm_ms_per_clip": r["f32_ms"],
+                          "gemm_flops_8mnk": r["gemm_flops"], "f32_flops_8mnk": r["f32_flops"],
+                          "algorithmic_bytes_per_launch": r["gemm_bytes"] / max(r["launches"], 1),
+                          "f32_algorithmic_bytes_per_launch": r["f32_bytes"] / max(r["f32_launches"], 1)}),
+              flush=True)
+        return
     f64 = run_form(_lib.PREC_FP64)
     mix = None if args.no_mixed else run_form(_lib.PREC_MIXED)
     fast = None if args.no_mixed else run_form(_lib.PREC_FAST)
@@ -685,6 +714,7 @@ def main():
         line["forms"]["batched"] = run_batched(args.batched)
     if args.served > 0:
         line["forms"]["served"] = run_served(args.served, args.slots)
+        line["forms"]["served_fast"] = run_served(args.served, args.slots, _lib.PREC_FAST)
     if not args.no_mask_producer:
         line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)
     if not args.no_cnn:

@@ -53,6 +53,7 @@ class ClipRequest:
 @dataclass
 class ClipResult:
     rid: int
+    cfg: Config
     slot: int
     arrival: float
     start: float
@@ -327,8 +328,8 @@ class Server:
                         slot.reserved = max(slot.reserved, slot.workspace())
                         self.batches.append(len(batch))
                         for req, (a0, img, socs) in zip(batch, outs):
-                            results[req.rid] = ClipResult(req.rid, slot.idx, req.arrival, start, finish, a0, img,
-                                                          socs)
+                            results[req.rid] = ClipResult(req.rid, req.cfg, slot.idx, req.arrival, start, finish,
+                                                          a0, img, socs)
                         state["done"] += len(batch)
                         cv.notify_all()
             except BaseException as e:  # surface it to run()
