@@ -75,32 +75,58 @@ __device__ inline double4_t pair_rotation(const double2* M, int n, int p, int q)
   return r;
 }
 
-// One round-robin step in one launch, out of place (Ain -> Aout, Vin -> Vout):
-// thread (k, l) rotates the 2 x 2 block rows {p_k, q_k} x columns {p_l, q_l}
-// from both sides (rows by J_k^H, then columns by J_l); the pairs partition the
-// indices, so every element is written by exactly one thread, and reading only
-// the input copy makes the parameters of every pair those of the step's start
-// (rows first, then columns: J_k^H A J_l on each block).  Threads past the
-// npair^2 blocks rotate columns {p_l, q_l} of row i of V.
-__global__ void jacobi_step_kernel(int n, int npair, const int* pairs, const double2* __restrict__ Ain,
-                                   double2* __restrict__ Aout, const double2* __restrict__ Vin,
-                                   double2* __restrict__ Vout, int batch) {
-  const int64_t nblk = (int64_t)npair * npair, nv = (int64_t)n * npair, per = nblk + nv;
+// Pair order.  The matrix (np x np, n padded to even np with a zero row and
+// column that only ever pairs with itself as a no-op) is stored with the
+// indices permuted so that the pairs of the current round-robin step are the
+// positions (2k, 2k+1): the step then reads contiguous 2 x 2 blocks.  The
+// tournament's ring (position 0 fixed, the rest rotating by one each step)
+// moves the index at pair position p to pair_next(p) in the next step's order;
+// every step writes its output in that order, and after np - 1 steps (a
+// sweep) the order is the initial one again.
+__device__ __host__ inline int pair_next(int p, int np) {
+  const int r = (p & 1) ? np - 1 - (p - 1) / 2 : p / 2;  // ring position
+  const int r2 = (r == 0) ? 0 : (r == np - 1 ? 1 : r + 1);
+  return (r2 < np / 2) ? 2 * r2 : 2 * (np - 1 - r2) + 1;
+}
+__device__ __host__ inline int pair_index0(int p, int np) { return (p & 1) ? np - 1 - p / 2 : p / 2; }
+
+// A (n x n, natural order) -> Ap (np x np, initial pair order, zero padded);
+// Vp = the identity with its columns in pair order (rows in natural order)
+__global__ void pair_order_init_kernel(int n, int np, const double2* A, double2* Ap, double2* Vp, int batch) {
+  const int64_t per = (int64_t)np * np, total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int i = (int)(r / np), j = (int)(r % np);
+    const int oi = pair_index0(i, np), oj = pair_index0(j, np);
+    Ap[e] = (oi < n && oj < n) ? A[(int64_t)b * n * n + (int64_t)oi * n + oj] : make_double2(0.0, 0.0);
+    Vp[e] = (i == oj) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
+  }
+}
+
+// One round-robin step in one launch, out of place, in pair order: thread
+// (k, l) rotates the 2 x 2 block rows (2k, 2k+1) x columns (2l, 2l+1) from both
+// sides (rows by J_k^H, then columns by J_l) and writes it at the positions of
+// the next step's order; the pairs partition the positions, so every element
+// is written exactly once, and reading only the input copy makes the
+// parameters of every pair those of the step's start.  Threads past the
+// npair^2 blocks rotate columns (2l, 2l+1) of row i of V.
+__global__ void jacobi_pairstep_kernel(int np, const double2* __restrict__ Ain, double2* __restrict__ Aout,
+                                       const double2* __restrict__ Vin, double2* __restrict__ Vout, int batch) {
+  const int npair = np / 2;
+  const int64_t nblk = (int64_t)npair * npair, nv = (int64_t)np * npair, per = nblk + nv;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / per);
     const int64_t r = e - (int64_t)b * per;
-    const double2* M = Ain + (int64_t)b * n * n;
+    const double2* M = Ain + (int64_t)b * np * np;
     if (r < nblk) {
       const int k = (int)(r / npair), l = (int)(r % npair);
-      const int rows[2] = {pairs[2 * k], pairs[2 * k + 1]}, cols[2] = {pairs[2 * l], pairs[2 * l + 1]};
-      const double4_t Rk = pair_rotation(M, n, rows[0], rows[1]);
-      const double4_t Rl = pair_rotation(M, n, cols[0], cols[1]);
-      double2 X[2][2];
-      for (int u = 0; u < 2; ++u)
-        for (int v = 0; v < 2; ++v)
-          X[u][v] = (rows[u] >= 0 && cols[v] >= 0) ? M[(int64_t)rows[u] * n + cols[v]] : make_double2(0.0, 0.0);
-      if (rows[0] >= 0 && rows[1] >= 0 && Rk[1] != 0.0) {
+      const double4_t Rk = pair_rotation(M, np, 2 * k, 2 * k + 1);
+      const double4_t Rl = pair_rotation(M, np, 2 * l, 2 * l + 1);
+      const double2* row0 = M + (int64_t)(2 * k) * np + 2 * l;
+      double2 X[2][2] = {{row0[0], row0[1]}, {row0[np], row0[np + 1]}};
+      if (Rk[1] != 0.0) {
         const double2 ep = make_double2(Rk[2], Rk[3]);  // e^{i phi_k}
         for (int v = 0; v < 2; ++v) {
           const double2 x = X[0][v], ey = cmul(ep, X[1][v]);
@@ -108,7 +134,7 @@ __global__ void jacobi_step_kernel(int n, int npair, const int* pairs, const dou
           X[1][v] = cadd(cscale(x, Rk[1]), cscale(ey, Rk[0]));
         }
       }
-      if (cols[0] >= 0 && cols[1] >= 0 && Rl[1] != 0.0) {
+      if (Rl[1] != 0.0) {
         const double2 em = make_double2(Rl[2], -Rl[3]);  // e^{-i phi_l}
         for (int u = 0; u < 2; ++u) {
           const double2 x = X[u][0], ey = cmul(em, X[u][1]);
@@ -116,31 +142,28 @@ __global__ void jacobi_step_kernel(int n, int npair, const int* pairs, const dou
           X[u][1] = cadd(cscale(x, Rl[1]), cscale(ey, Rl[0]));
         }
       }
-      double2* O = Aout + (int64_t)b * n * n;
-      for (int u = 0; u < 2; ++u)
-        for (int v = 0; v < 2; ++v)
-          if (rows[u] >= 0 && cols[v] >= 0) O[(int64_t)rows[u] * n + cols[v]] = X[u][v];
+      double2* O = Aout + (int64_t)b * np * np;
+      const int r0 = pair_next(2 * k, np), r1 = pair_next(2 * k + 1, np);
+      const int c0 = pair_next(2 * l, np), c1 = pair_next(2 * l + 1, np);
+      O[(int64_t)r0 * np + c0] = X[0][0];
+      O[(int64_t)r0 * np + c1] = X[0][1];
+      O[(int64_t)r1 * np + c0] = X[1][0];
+      O[(int64_t)r1 * np + c1] = X[1][1];
     } else {
       const int64_t rv = r - nblk;
       const int i = (int)(rv / npair), l = (int)(rv % npair);
-      const int p = pairs[2 * l], q = pairs[2 * l + 1];
-      const double2* Vi = Vin + (int64_t)b * n * n + (int64_t)i * n;
-      double2* Vo = Vout + (int64_t)b * n * n + (int64_t)i * n;
-      if (p < 0 || q < 0) {
-        const int c = p < 0 ? q : p;
-        if (c >= 0) Vo[c] = Vi[c];
-        continue;
+      const double2* Vi = Vin + (int64_t)b * np * np + (int64_t)i * np;
+      double2* Vo = Vout + (int64_t)b * np * np + (int64_t)i * np;
+      const double4_t R = pair_rotation(M, np, 2 * l, 2 * l + 1);
+      double2 x = Vi[2 * l], y = Vi[2 * l + 1];
+      if (R[1] != 0.0) {
+        const double2 ey = cmul(make_double2(R[2], -R[3]), y);
+        const double2 nx = csub(cscale(x, R[0]), cscale(ey, R[1]));
+        y = cadd(cscale(x, R[1]), cscale(ey, R[0]));
+        x = nx;
       }
-      const double4_t R = pair_rotation(M, n, p, q);
-      const double2 x = Vi[p], y = Vi[q];
-      if (R[1] == 0.0) {
-        Vo[p] = x;
-        Vo[q] = y;
-        continue;
-      }
-      const double2 ey = cmul(make_double2(R[2], -R[3]), y);
-      Vo[p] = csub(cscale(x, R[0]), cscale(ey, R[1]));
-      Vo[q] = cadd(cscale(x, R[1]), cscale(ey, R[0]));
+      Vo[pair_next(2 * l, np)] = x;
+      Vo[pair_next(2 * l + 1, np)] = y;
     }
   }
 }
@@ -167,13 +190,6 @@ __global__ void offdiag_kernel(int n, const double2* A, double* out, int batch) 
   }
 }
 
-__global__ void eye_kernel(int n, double2* V, int batch) {
-  const int64_t total = (int64_t)n * n * batch;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e % ((int64_t)n * n);
-    V[e] = (r / n == r % n) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
-  }
-}
 
 __global__ void zero_kernel(double* p, int n) {
   for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
@@ -186,14 +202,14 @@ __global__ void diag_kernel(int n, const double2* A, double* w, int batch) {
   w[e] = A[(int64_t)b * n * n + (int64_t)i * n + i].x;
 }
 
-// phi[b][m][j] = V[b][j][order[b][m]], alpha[b][m] = w[b][order[b][m]]
-__global__ void gather_kernel(int n, int nk, const double2* V, const int* order, double2* phi, int batch) {
+// phi[b][m][j] = V[b][j][order[b][m]] (V: np x np, rows the n components)
+__global__ void gather_kernel(int n, int np, int nk, const double2* V, const int* order, double2* phi, int batch) {
   const int64_t total = (int64_t)nk * n * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / ((int64_t)nk * n));
     const int64_t r = e - (int64_t)b * nk * n;
     const int m = (int)(r / n), j = (int)(r % n);
-    phi[e] = V[(int64_t)b * n * n + (int64_t)j * n + order[b * nk + m]];
+    phi[e] = V[(int64_t)b * np * np + (int64_t)j * np + order[b * nk + m]];
   }
 }
 
@@ -433,40 +449,28 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
     return EUVL_ERR_ARG;
   }
   const int n = ncut, B = 3, S = n_src;
-  // round-robin schedule over n' = n rounded up to even players (index -1 = bye)
+  // parallel Jacobi over a round-robin tournament of np = n rounded up to even
+  // players, stored in pair order (see pair_next)
   const int np = n + (n & 1), npair = np / 2, nsteps = np - 1;
-  std::vector<int> pairs((size_t)nsteps * npair * 2);
-  {
-    std::vector<int> ring(np);
-    std::iota(ring.begin(), ring.end(), 0);
-    for (int st = 0; st < nsteps; ++st) {
-      for (int k = 0; k < npair; ++k) {
-        int a = ring[k], b = ring[np - 1 - k];
-        if (a >= n) a = -1;
-        if (b >= n) b = -1;
-        pairs[((size_t)st * npair + k) * 2] = a;
-        pairs[((size_t)st * npair + k) * 2 + 1] = b;
-      }
-      std::rotate(ring.begin() + 1, ring.end() - 1, ring.end());
-    }
-  }
   Upload up;
-  const size_t o_pairs = up.add(pairs.data(), sizeof(int) * pairs.size());
   const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * n);
   const size_t o_src = up.add(h_src, sizeof(double) * 2 * S);
   const size_t mat = sizeof(double2) * (size_t)n * n * B;
+  const size_t matp = sizeof(double2) * (size_t)np * np * B;
   const size_t pb = sizeof(double2) * (size_t)n * 3 * S;
-  EUVL_TRY(ws_reserve(h, up.host.size() + 4 * mat + 4 * pb + sizeof(double) * n * B + sizeof(int) * nk * B + 8192));
+  EUVL_TRY(ws_reserve(h, up.host.size() + mat + 4 * matp + 4 * pb + sizeof(double) * np * B +
+                             sizeof(int) * nk * B + 8192));
   char* blk = (char*)ws_take(h, up.host.size());
-  double2* A = (double2*)ws_take(h, mat);
-  double2* V = (double2*)ws_take(h, mat);
-  double2* A2 = (double2*)ws_take(h, mat);
-  double2* V2 = (double2*)ws_take(h, mat);
+  double2* T0 = (double2*)ws_take(h, mat);
+  double2* A = (double2*)ws_take(h, matp);
+  double2* V = (double2*)ws_take(h, matp);
+  double2* A2 = (double2*)ws_take(h, matp);
+  double2* V2 = (double2*)ws_take(h, matp);
   double2* P = (double2*)ws_take(h, pb);
   double2* Q = (double2*)ws_take(h, 3 * pb);
-  double* w = (double*)ws_take(h, sizeof(double) * n * B);
+  double* w = (double*)ws_take(h, sizeof(double) * np * B);
   int* order_d = (int*)ws_take(h, sizeof(int) * nk * B);
-  if (!A || !V || !A2 || !V2 || !P || !Q || !w || !order_d) {
+  if (!T0 || !A || !V || !A2 || !V2 || !P || !Q || !w || !order_d) {
     set_error("euvl_tcc_kernels: workspace carve failed");
     return EUVL_ERR_NOMEM;
   }
@@ -477,22 +481,22 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
   EUVL_LAUNCH_CHECK();
   // TCC_w = P Q_w  (n x 3S x n), the three weights as a batch sharing P
   const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  EUVL_TRY(zgemm(h, n, n, 3 * S, one, P, 3 * S, 0, Q, n, (int64_t)3 * S * n, zero, A, n, (int64_t)n * n, zero, B));
-  hipLaunchKernelGGL(eye_kernel, grid1((int64_t)n * n * B), dim3(256), 0, h->stream, n, V, B);
+  EUVL_TRY(zgemm(h, n, n, 3 * S, one, P, 3 * S, 0, Q, n, (int64_t)3 * S * n, zero, T0, n, (int64_t)n * n, zero, B));
+  hipLaunchKernelGGL(pair_order_init_kernel, grid1((int64_t)np * np * B), dim3(256), 0, h->stream, n, np, T0, A, V,
+                     B);
   EUVL_LAUNCH_CHECK();
-  const int* d_pairs = (const int*)(blk + o_pairs);
   // Stop once the squared off-diagonal / total Frobenius ratio is at the
   // rounding floor of an n x n matrix, (2 n eps)^2, or stops falling below
   // 1e3 times it; Jacobi converges quadratically, so this takes few sweeps.
   const double eps = 2.220446049250313e-16;
   const double floor2 = (2.0 * n * eps) * (2.0 * n * eps);
-  const int64_t step_threads = ((int64_t)npair * npair + (int64_t)n * npair) * B;
+  const int64_t step_threads = ((int64_t)npair * npair + (int64_t)np * npair) * B;
   int sweep = 0;
   double worst = INFINITY, prev = INFINITY;
   for (sweep = 0; sweep <= 40; ++sweep) {
     hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, h->stream, h->d_red, 2 * B);
     EUVL_LAUNCH_CHECK();
-    hipLaunchKernelGGL(offdiag_kernel, dim3(256, B), dim3(256), 0, h->stream, n, A, h->d_red, B);
+    hipLaunchKernelGGL(offdiag_kernel, dim3(256, B), dim3(256), 0, h->stream, np, A, h->d_red, B);
     EUVL_LAUNCH_CHECK();
     EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 2 * B, hipMemcpyDeviceToHost, h->stream));
     EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
@@ -505,9 +509,7 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
     if (worst <= floor2 || (worst <= 1e3 * floor2 && worst > 0.25 * prev) || sweep == 40) break;
     prev = worst;
     for (int st = 0; st < nsteps; ++st) {
-      const int* pr = d_pairs + (size_t)st * npair * 2;
-      hipLaunchKernelGGL(jacobi_step_kernel, grid1(step_threads), dim3(256), 0, h->stream, n, npair, pr, A, A2, V,
-                         V2, B);
+      hipLaunchKernelGGL(jacobi_pairstep_kernel, grid1(step_threads), dim3(256), 0, h->stream, np, A, A2, V, V2, B);
       std::swap(A, A2);
       std::swap(V, V2);
     }
@@ -518,29 +520,40 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
     set_error("euvl_tcc_kernels: Jacobi off-diagonal ratio %.3e after %d sweeps (floor %.3e)", worst, sweep, floor2);
     return EUVL_ERR_NOT_CONVERGED;
   }
-  // eigenvalues, stable sort by |alpha| descending (the reference's bubble sort order), top nk
-  hipLaunchKernelGGL(diag_kernel, grid1((int64_t)n * B), dim3(256), 0, h->stream, n, A, w, B);
+  // eigenvalues (the diagonal, one per position of the pair order; a sweep ends
+  // in the initial order), stable sort by |alpha| descending (the reference's
+  // bubble sort order), top nk; the padding position (odd n) is left out
+  hipLaunchKernelGGL(diag_kernel, grid1((int64_t)np * B), dim3(256), 0, h->stream, np, A, w, B);
   EUVL_LAUNCH_CHECK();
-  std::vector<double> hw((size_t)n * B);
-  EUVL_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * n * B, hipMemcpyDeviceToHost, h->stream));
+  std::vector<double> hw((size_t)np * B);
+  EUVL_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * np * B, hipMemcpyDeviceToHost, h->stream));
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   std::vector<int> order((size_t)nk * B);
   std::vector<double> halpha((size_t)nk * B);
   for (int b = 0; b < B; ++b) {
-    std::vector<int> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
+    // positions sorted by the index they hold, so that ties keep ascending index order
+    std::vector<int> idx;
+    for (int q = 0; q < n; ++q) {
+      for (int pos = 0; pos < np; ++pos)
+        if (pair_index0(pos, np) == q) {
+          idx.push_back(pos);
+          break;
+        }
+    }
     // ties keep ascending eigenvalue order, as after the reference's ascending eigh
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return hw[(size_t)b * n + x] < hw[(size_t)b * n + y]; });
     std::stable_sort(idx.begin(), idx.end(),
-                     [&](int x, int y) { return std::fabs(hw[(size_t)b * n + x]) > std::fabs(hw[(size_t)b * n + y]); });
+                     [&](int x, int y) { return hw[(size_t)b * np + x] < hw[(size_t)b * np + y]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+      return std::fabs(hw[(size_t)b * np + x]) > std::fabs(hw[(size_t)b * np + y]);
+    });
     for (int m = 0; m < nk; ++m) {
       order[(size_t)b * nk + m] = idx[m];
-      halpha[(size_t)b * nk + m] = hw[(size_t)b * n + idx[m]];
+      halpha[(size_t)b * nk + m] = hw[(size_t)b * np + idx[m]];
     }
   }
   EUVL_HIP_CHECK(hipMemcpyAsync(order_d, order.data(), sizeof(int) * nk * B, hipMemcpyHostToDevice, h->stream));
   EUVL_HIP_CHECK(hipMemcpyAsync(d_alpha, halpha.data(), sizeof(double) * nk * B, hipMemcpyHostToDevice, h->stream));
-  hipLaunchKernelGGL(gather_kernel, grid1((int64_t)nk * n * B), dim3(256), 0, h->stream, n, nk, V, order_d,
+  hipLaunchKernelGGL(gather_kernel, grid1((int64_t)nk * n * B), dim3(256), 0, h->stream, n, np, nk, V, order_d,
                      (double2*)d_phi, B);
   EUVL_LAUNCH_CHECK();
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
