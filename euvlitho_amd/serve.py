@@ -28,6 +28,7 @@ computed by its own tiles, and nothing a request reads is written by another,
 so a clip's images agree whether it runs alone or among others.
 """
 import ctypes
+import gc
 import threading
 import time
 from dataclasses import dataclass, field
@@ -278,6 +279,10 @@ class Server:
             if footprint(c) > self.budget:
                 raise _lib.EuvlError("a request needs more device memory than the HBM budget")
             self.decisions(c)
+        # Objects made so far (the set-up, torch's own) go to the permanent generation, so that a
+        # full collection during the stream does not stall a worker for tens of milliseconds.
+        gc.collect()
+        gc.freeze()
         pending = list(reqs)
         results, errors = {}, []
         self.batches = []
