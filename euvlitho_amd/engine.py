@@ -167,19 +167,23 @@ class Engine:
         self.last_info = info.reshape(n_off, 4)
         return Ax
 
-    def blank_amplitudes(self, polar="X"):
+    def blank_amplitudes(self, polar="X", masks=None):
         """vcxx, abxx: blank vacuum / absorber amplitudes on the narrow order set at
         the central offset (m3d.cpp:211-258); kept in the resident store per
-        configuration and polarisation."""
+        configuration and polarisation.  masks: optional (vacuum, absorber)
+        0/1 arrays of ndivx*ndivy to solve in place of the uniform blanks (not
+        cached)."""
         key = ("blank", self.cfg, polar, self.precision)
-        got = self.resident.get(key)
-        if got is not None:
-            return got
+        if masks is None:
+            got = self.resident.get(key)
+            if got is not None:
+                return got
         cfg = self.cfg
         orders = cfg.blank_order_set()
         n = cfg.ndivx * cfg.ndivy
-        vc = self.amp_s(self.upload_mask(np.ones(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
-        ab = self.amp_s(self.upload_mask(np.zeros(n, np.int8)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
+        m_vc, m_ab = (np.ones(n, np.int8), np.zeros(n, np.int8)) if masks is None else masks
+        vc = self.amp_s(self.upload_mask(np.asarray(m_vc).reshape(-1)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
+        ab = self.amp_s(self.upload_mask(np.asarray(m_ab).reshape(-1)), polar, [(cfg.kx0, cfg.ky0)], orders)[0]
         lidx, midx, L, M = orders
         mm = Config.mode_map(lidx, midx, L, M)
         nsx, nsy = cfg.nsource
@@ -195,14 +199,17 @@ class Engine:
                     if n_ >= 0:
                         vcxx[is_, js] = vch[is_, js, n_]
                         abxx[is_, js] = abh[is_, js, n_]
+        if masks is not None:
+            return vcxx, abxx
         return self.resident.put(key, (vcxx, abxx))
 
     # --------------------------------------------------------------- M3D params
-    def m3d(self, bits, Ax0=None):
+    def m3d(self, bits, Ax0=None, blanks=None):
         """(a0, ax, ay) of one clip (m3d.cpp main); ax, ay only where xinput = 1.
-        Ax0: wide amplitudes at the central offset (computed when None)."""
+        Ax0: wide amplitudes at the central offset (computed when None);
+        blanks: (vcxx, abxx) in place of the resident blank amplitudes."""
         cfg = self.cfg
-        vcxx, abxx = self.blank_amplitudes()
+        vcxx, abxx = self.blank_amplitudes() if blanks is None else blanks
         if Ax0 is None:
             Ax0 = self.amp_s(bits, "X", [(cfg.kx0, cfg.ky0)])[0]
         lpx, lpy = cfg.lpmax

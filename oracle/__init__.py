@@ -24,12 +24,14 @@ How each part is pinned (checked against something other than itself):
 * EM core (ampS): blank-mask closed forms, invariance of the amplitudes to
   eigenvector scaling/permutation, mirror symmetry and the integer-shift
   theorem.  End to end, the reference fixtures ``inputxx.csv``/``emint.csv``
-  at the reference-default optics are NOT reproduced: the oracle misses them
-  by 0.26-3.6 % of the row maxima and 2.5 % of the image peak
-  (tests/golden/ref_default_e2e.py); the tests carry the fixtures' 2e-5 bar
-  as a strict expected failure.
+  at the reference-default optics (N = 8705, blank solves N = 1009) are
+  reproduced to their 6 significant digits on the masks as the revision that
+  wrote them read them (every mask array one row off in y, the out-of-range
+  row from the adjacent memory: tests/golden/fixture_reads.py), at the
+  fixtures' 2e-5 bar (tests/golden/ref_default_e2e.py, the slow
+  test_reference_default_end_to_end).
 * Abbe image: blank-mask uniformity, grid refinement sub-sampling identity
-  (and ``emint.csv``, missed as above).
+  and ``emint.csv`` as above.
 * M3D fit: exact recovery of synthetic linear data in both branches (Cramer
   solve where snum >= 8, mean elsewhere) and equality with
   ``numpy.linalg.lstsq`` on the design matrix [1, ls, ms].

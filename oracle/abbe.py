@@ -85,11 +85,12 @@ def abbe_image(p, d, mask2d, lidx, midx, LMAX, MMAX, divisions, nx, ny, z=0.0, a
     return img / total
 
 
-def blank_amplitudes(p, d, narrow=("ellipse", 1.5)):
+def blank_amplitudes(p, d, narrow=("ellipse", 1.5), masks=None):
     """vcxx, abxx on the narrow order set (m3d.cpp:101-129, 211-258).  In-NA
     lattice incidences that the truncated ellipse misses (small pitches) are
     appended to the set: the reference reads an unset entry there, and a
-    uniform mask couples no modes, so the other amplitudes are unchanged."""
+    uniform mask couples no modes, so the other amplitudes are unchanged.
+    masks: optional (vacuum, absorber) 0/1 masks in place of the uniform ones."""
     li, mi, L, M = order_set(p, narrow[0], cutoff=narrow[1])
     have = set(zip(li.tolist(), mi.tolist()))
     extra = [(ls, ms) for ls in range(-d["lsmaxX"], d["lsmaxX"] + 1) for ms in range(-d["lsmaxY"], d["lsmaxY"] + 1)
@@ -99,8 +100,9 @@ def blank_amplitudes(p, d, narrow=("ellipse", 1.5)):
         li, mi = np.concatenate([li, el]), np.concatenate([mi, em_])
         L, M = max(L, int(np.abs(el).max())), max(M, int(np.abs(em_).max()))
     n = p["NDIVX"] * p["NDIVY"]
-    Avc = em.amp_s("X", np.ones(n, np.int64), p, d, li, mi, L, M, d["kx0"], d["ky0"])
-    Aab = em.amp_s("X", np.zeros(n, np.int64), p, d, li, mi, L, M, d["kx0"], d["ky0"])
+    m_vc, m_ab = (np.ones(n, np.int64), np.zeros(n, np.int64)) if masks is None else masks
+    Avc = em.amp_s("X", np.asarray(m_vc).reshape(-1), p, d, li, mi, L, M, d["kx0"], d["ky0"])
+    Aab = em.amp_s("X", np.asarray(m_ab).reshape(-1), p, d, li, mi, L, M, d["kx0"], d["ky0"])
     vc = np.zeros((d["nsourceX"], d["nsourceY"]), dtype=np.complex128)
     ab = np.zeros_like(vc)
     pos = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(li, mi))}

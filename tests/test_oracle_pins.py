@@ -347,14 +347,13 @@ def test_abbe_blank_mask_uniform():
 
 # ------------------------------------------------------------ end to end (slow)
 @pytest.mark.slow
-@pytest.mark.xfail(strict=True, reason="the oracle misses inputxx.csv by 0.26-3.6 % of the row maxima and "
-                                       "emint.csv by 2.5 % of peak (tests/golden/ref_default_e2e.json); open")
 def test_reference_default_end_to_end(golden, tmp_path):
     """Oracle at the reference-default optics vs m3d/inputxx.csv and
     abbe/emint.csv (one N = 8705 eigenproblem; ~15 min on 6 cores), at the
-    2e-5 bar that the fixtures' 6 significant digits support."""
+    2e-5 bar that the fixtures' 6 significant digits support, on the masks as
+    the fixtures' program revision read them (tests/golden/fixture_reads.py)."""
     from tests.golden.ref_default_e2e import run
-    rep = run(golden, out=str(tmp_path / "ref_default_e2e.json"), arrays=str(tmp_path / "ref_default_oracle.npz"))
+    rep = run(golden, out=str(tmp_path / "ref_default_e2e.json"))
     assert all(e < 2e-5 for e in rep["inputxx_rel_per_row"])
     assert rep["emint_max_rel"] < 2e-5
 
