@@ -601,9 +601,9 @@ def main():
         _lib.check(eng.lib.euvl_gemm_stats(eng.h, 1, None), "euvl_gemm_stats")
         outs = [t.cpu().numpy() for t in step(bits[args.warmup])]
         _lib.check(eng.lib.euvl_gemm_stats(eng.h, 2, stats), "euvl_gemm_stats")
-        info = eng.last_info  # per offset: LU path, doublings, Neumann factors, incidences
+        info = eng.last_info  # per offset: LU path, doublings, (reserved), incidences
         em = {"lu_path": [int(x) for x in info[:, 0]], "doublings": int(info[:, 1].max()),
-              "neumann_factors": int(info[:, 2].max()), "incidences": int(info[:, 3].max())}
+              "incidences": int(info[:, 3].max())}
         eng.close()
         return dict(lat=lat, wall=wall, tcc_ms=tcc_ms, outs=outs, launches=stats[0], gemm_ms=stats[1],
                     gemm_flops=stats[2], gemm_bytes=stats[3], f32_launches=stats[4], f32_ms=stats[5],

@@ -115,8 +115,8 @@ _SIGS = {
     "euvl_plan_workspace_bytes": [c_void_p],
     "euvl_plan_reserve": [c_void_p, c_int, ctypes.POINTER(c_void_p)],
     "euvl_plan_run": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int],
-    "euvl_plan_decisions": [c_void_p, P_int, P_int],
-    "euvl_plan_set_decisions": [c_void_p, c_int, c_int],
+    "euvl_plan_decisions": [c_void_p, P_int],
+    "euvl_plan_set_decisions": [c_void_p, c_int],
     "euvl_graph_begin": [c_void_p],
     "euvl_graph_end": [c_void_p, ctypes.POINTER(c_void_p)],
     "euvl_graph_launch": [c_void_p, c_void_p],

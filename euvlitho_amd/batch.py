@@ -8,7 +8,7 @@ is cut into runs of a few clips whose EM products are batched over clips x
 source divisions (libeuvl plans: euvl_plan_create / euvl_plan_run), and the
 runs of every group are captured once into one HIP graph that is replayed
 with one launch.  Decisions the eager path takes on the host (doubling count,
-Neumann length, pivoted fallback of the top solve) are calibrated by an
+pivoted fallback of the top solve) are calibrated by an
 adaptive run of the loaded clips and then checked on the device; a clip whose
 check fails is run again adaptively after the graph.
 """
@@ -27,7 +27,7 @@ from .masks import pack_bits
 TARGET_TILES = 1536
 MAX_CLIPS_PER_RUN = 16
 # flag bits of a fixed-decision run (euvl_plan_run)
-FLAG_NAMES = {1: "doubling bound", 2: "Neumann remainder", 4: "top-solve residual", 8: "zero pivot"}
+FLAG_NAMES = {1: "doubling bound", 4: "top-solve residual", 8: "zero pivot"}
 
 
 def clips_per_run(cfg: Config, n_clips: int):
@@ -94,9 +94,10 @@ class Plan:
         _lib.check(self.lib.euvl_plan_create(eng.h, ctypes.byref(d), ctypes.byref(self.pl)), "euvl_plan_create")
 
     def decisions(self):
-        d, n = ctypes.c_int(), ctypes.c_int()
-        _lib.check(self.lib.euvl_plan_decisions(self.pl, ctypes.byref(d), ctypes.byref(n)), "euvl_plan_decisions")
-        return d.value, n.value
+        """The plan's doubling count of the matrix functions (fixed-mode runs)."""
+        d = ctypes.c_int()
+        _lib.check(self.lib.euvl_plan_decisions(self.pl, ctypes.byref(d)), "euvl_plan_decisions")
+        return d.value
 
     def close(self):
         if self.pl:
@@ -110,7 +111,7 @@ class BatchedEngine:
 
     load(clips) makes the plans, uploads the masks (they stay resident in
     HBM), runs every clip once adaptively (each plan keeps the largest
-    doubling count and Neumann length its clips needed) and captures the graph;
+    doubling count its clips needed) and captures the graph;
     launch() replays the graph; results() synchronises, reruns flagged clips
     adaptively and returns per-clip (a0, ax, ay, image) as device tensors."""
 

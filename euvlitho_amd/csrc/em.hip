@@ -28,7 +28,7 @@
 
 namespace {
 
-const double2 ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0};
+const double2 ONE = {1.0, 0.0}, ZERO = {0.0, 0.0}, MONE = {-1.0, 0.0}, MI = {0.0, -1.0};
 
 __device__ inline int mask_bit(const uint8_t* bits, int64_t idx) { return (bits[idx >> 3] >> (7 - (idx & 7))) & 1; }
 
@@ -214,38 +214,39 @@ __global__ void toeplitz_kernel(int n, int batch, char polar, double k, double d
   }
 }
 
-// W1 = Minv diag(b2 (uu+ub) / (i k)), in place over Minv (= (I - G)^-1)
-__global__ void w1_kernel(int n, int batch, double k, double2* Minv, ModeVecs v) {
-  const int64_t per = (int64_t)n * n;
-  const int64_t total = per * batch;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int b = (int)(e / per);
-    const int j = (int)((e - (int64_t)b * per) % n);
-    const int64_t vj = (int64_t)b * n + j;
-    const double2 s = cmul(cdiv(v.b2[vj], make_double2(0.0, k)), cadd(v.uu[vj], v.ub[vj]));
-    Minv[e] = cmul(Minv[e], s);
-  }
+// lam_j = alru (uu - ub) i k / (b2 (uu + ub)): L2 S^-1 of the top solve, with
+// L2 = diag(alru (uu - ub)) and S = diag(b2 (uu + ub) / (i k)).  The cascade's
+// scale |uu| cancels: only the multilayer's reflection ratio ub/uu enters.
+__device__ __forceinline__ double2 top_lambda(const ModeVecs& v, int64_t j, double k) {
+  const double2 uu = v.uu[j], ub = v.ub[j];
+  return cdiv(cmul(cmul(v.alru[j], csub(uu, ub)), make_double2(0.0, k)), cmul(v.b2[j], cadd(uu, ub)));
 }
 
-// With L2 = diag(alru (uu-ub)):  T2 <- R = C L2 - i T2,
-// T1 <- U0 = diag(klm) (T1 - i F L2) + R   (T1 = C W1, T2 = G2 W1 on entry)
-__global__ void u0r_kernel(int n, int batch, const double2* C, const double2* F, double2* T1, double2* T2,
-                           ModeVecs v) {
+// H = (C - i diag(klm) F) diag(lam),  V = diag(klm) C - i G2 + H  (V gets - H G next)
+__global__ void top_hv_kernel(int n, int batch, double k, const double2* C, const double2* F, const double2* G2,
+                              double2* H, double2* V, ModeVecs v) {
   const int64_t per = (int64_t)n * n;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
     const int b = (int)(e / per);
     const int64_t r = e - (int64_t)b * per;
     const int i = (int)(r / n), j = (int)(r % n);
-    const int64_t vj = (int64_t)b * n + j;
-    const double2 l2 = cmul(v.alru[vj], csub(v.uu[vj], v.ub[vj]));
-    const double2 t2 = T2[e];
-    const double2 R = csub(cmul(C[e], l2), make_double2(-t2.y, t2.x));  // C l2 - i t2
-    const double2 fl = cmul(F[e], l2);
-    const double2 t1 = T1[e];
-    const double2 p = make_double2(t1.x + fl.y, t1.y - fl.x);  // t1 - i fl
-    T2[e] = R;
-    T1[e] = cadd(cmul(v.klm[(int64_t)b * n + i], p), R);
+    const double2 klm = v.klm[(int64_t)b * n + i];
+    const double2 c = C[e], kf = cmul(klm, F[e]), g2 = G2[e], kc = cmul(klm, c);
+    const double2 hv = cmul(make_double2(c.x + kf.y, c.y - kf.x), top_lambda(v, (int64_t)b * n + j, k));
+    H[e] = hv;
+    V[e] = make_double2(kc.x + g2.y + hv.x, kc.y - g2.x + hv.y);  // klm c - i g2 + h
+  }
+}
+
+// Y = diag(lam) (Z - G Z), in place over GZ (n x ninc)
+__global__ void top_y_kernel(int n, int ninc, int batch, double k, const double2* Z, double2* GZ, ModeVecs v) {
+  const int64_t per = (int64_t)n * ninc;
+  const int64_t total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int i = (int)((e - (int64_t)b * per) / ninc);
+    GZ[e] = cmul(top_lambda(v, (int64_t)b * n + i, k), csub(Z[e], GZ[e]));
   }
 }
 
@@ -579,8 +580,8 @@ size_t solve_workspace_bytes(int N, int E, int ninc) {
 }
 
 // The EM solve of E = n_clips x n_off batch entries (ampS, header.h:1208-1466)
-// on tables already on the device.  ctl.fixed: the doubling count, the Neumann
-// length and the unpivoted top solve are fixed ahead of time and checked on the
+// on tables already on the device.  ctl.fixed: the doubling count and the
+// unpivoted top solve are fixed ahead of time and checked on the
 // device (ctl.d_flags); nothing is read back, so the call can be captured in a
 // graph.  Otherwise the decisions are taken from norms read back to the host,
 // with the pivoted fallback of the top solve.  Takes its matrices from the
@@ -641,64 +642,24 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl, ctl.fixed ? ctl.doublings : -1, red, ctl.d_flags,
                     &ctl.alpha));
 
-  // products of the coupling and the top solve (the LU keeps its own scope)
+  // The top solve.  With M = I - G (G = diag(kt) sigma^ diag(kt) / k^2),
+  // S = diag(b2 (uu + ub) / (i k)) and L2 = diag(alru (uu - ub)), the layer
+  // coupling gives W1 = M^-1 S, U0 = P W1 + Q and R = C L2 - i G2 W1, where
+  // P = diag(klm) C - i G2 and Q = (C - i diag(klm) F) L2; the amplitudes need
+  // only R U0^-1 E (E: the ninc incidence columns).  With Z = W1 X the system
+  // U0 X = E becomes V Z = E with V = P + H M, H = Q S^-1 = (C - i diag(klm) F)
+  // diag(lam), lam = L2 S^-1, and R X = C diag(lam) M Z - i G2 Z.  So neither
+  // M^-1 nor W1 is formed: one N x N product (H G) and three N x ninc products
+  // replace a series for M^-1 and the two N x N products of the reference's
+  // T1 / T2 blocks (header.h:1053-1144, 1355-1391).
   PrecScope scope(h, EUVL_PREC_FP32_COUPLING);
-  // Minv = (I+G)(I+G^2)(I+G^4)... ~ (I - G)^-1 with M = i k (I - G); ||G|| is ~0.1 for these stacks.
-  double2* Gp = Mx[0];  // D no longer needed
-  double2* Fm = Mx[5];
-  double2* Fn = Mx[6];
-  int nfac = 1;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, G, N, sm, ZERO, Gp, N, sm, ZERO, B));           // G^2
-  EUVL_TRY(ew_lincomb(h, N, B, Fm, ONE, G, ONE, Gp, ZERO, nullptr, ONE));                   // I + G + G^2
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G, N, sm, Gp, N, sm, ONE, Fm, N, sm, ZERO, B));           // + G^3
-  // stop once the remainder bound ||G^(2^(j+1))|| <= ||G^(2^j)||^2 is below 1e-16
-  // (1e-10 when these products run in FP32, well below their rounding)
-  const double neumann_tol = (h->prec_mask & EUVL_PREC_FP32_COUPLING) ? 1e-10 : 1e-16;
-  auto next_factor = [&]() -> int {
-    EUVL_TRY(zgemm(h, N, N, N, ONE, Gp, N, sm, Gp, N, sm, ZERO, G, N, sm, ZERO, B));        // next power
-    std::swap(G, Gp);
-    EUVL_TRY(ew_copy(h, sm * B, Fn, Fm));
-    EUVL_TRY(zgemm(h, N, N, N, ONE, Fm, N, sm, Gp, N, sm, ONE, Fn, N, sm, ZERO, B));        // F (I + G^(2^j))
-    std::swap(Fm, Fn);
-    ++nfac;
-    return EUVL_OK;
-  };
-  if (ctl.fixed) {
-    while (nfac < ctl.neumann) EUVL_TRY(next_factor());
-    EUVL_TRY(reduce(h, 1, N, B, Gp, red + 2 * B));
-    EUVL_TRY(flag_check(h, 1, B, red + 2 * B, nullptr, nullptr, 0.0, neumann_tol, FLAG_NEUMANN, ctl.d_flags));
-  } else {
-    if (B > 256) {
-      set_error("euvl_amp_s: batch %d above the 256 of the host read-back", B);
-      return EUVL_ERR_ARG;
-    }
-    for (;;) {
-      std::vector<double> nr(B);
-      EUVL_TRY(reduce_to_host(h, 1, N, B, Gp, nr.data()));
-      double worst = 0.0;
-      for (double x : nr) worst = std::max(worst, std::isnan(x) ? INFINITY : x);
-      if (worst * worst < neumann_tol) break;
-      if (!(worst < 0.5) || nfac >= 8) {
-        set_error("euvl_amp_s: Neumann series of M^-1 not converged (||G^(2^%d)|| = %.3e)", nfac, worst);
-        return EUVL_ERR_NOT_CONVERGED;
-      }
-      EUVL_TRY(next_factor());
-    }
-  }
-  // W1 = M^-1 diag(b2 (uu+ub)) in place over Fm
-  hipLaunchKernelGGL(w1_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, Fm, v);
+  double2* H = Mx[6];
+  double2* V = Mx[5];
+  hipLaunchKernelGGL(top_hv_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, C, Fs, G2c, H, V, v);
   EUVL_LAUNCH_CHECK();
-  // T1 = C W1 (into G), T2 = G2 W1 (into Gp)
-  double2* T1 = G;
-  double2* T2 = Gp;
-  EUVL_TRY(zgemm(h, N, N, N, ONE, C, N, sm, Fm, N, sm, ZERO, T1, N, sm, ZERO, B));
-  EUVL_TRY(zgemm(h, N, N, N, ONE, G2c, N, sm, Fm, N, sm, ZERO, T2, N, sm, ZERO, B));
-  hipLaunchKernelGGL(u0r_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, C, Fs, T1, T2, v);
-  EUVL_LAUNCH_CHECK();
-  double2* R = T2;
-  double2* U0 = T1;
-  double2* U0c = Mx[2];  // C is no longer needed: a copy of U0 for the residual check
-  double2* Bres = Mx[3]; // nor F: the right-hand sides kept for the check
+  EUVL_TRY(zgemm(h, N, N, N, MONE, H, N, sm, G, N, sm, ONE, V, N, sm, ZERO, B));            // V = P + H - H G
+  double2* Vc = Mx[0];    // D is no longer needed: a copy of V for the residual check
+  double2* Bres = Mx[6];  // nor H: the right-hand sides kept for the check
   const int64_t nsxy = (int64_t)nsx * nsy;
   EUVL_TRY(fill(h, nsxy * N * B, Ax, ZERO));
   int lu_path = 0;
@@ -711,18 +672,18 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
       EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
       for (int b = 0; b < B; ++b)
         if (hi[b] != 0) {
-          if (report) set_error("euvl_amp_s: U0 singular (offset %d, column %d)", b, hi[b] - 1);
+          if (report) set_error("euvl_amp_s: top-solve matrix singular (offset %d, column %d)", b, hi[b] - 1);
           return EUVL_ERR_SINGULAR;
         }
       return EUVL_OK;
     };
-    // U0 is well conditioned (cond ~10-20 for these stacks) and its unpivoted
-    // elimination is stable (growth ~1.1), so the LU without row interchanges
-    // (GEMM panels, no pivot searches) comes first; its solve is verified by
-    // the residual ||U0 X - E|| / (||U0|| ||X||) and redone with partial
-    // pivoting if that is not at rounding level (adaptive), or flagged (fixed).
-    EUVL_TRY(ew_copy(h, sm * B, U0c, U0));
-    EUVL_TRY(zgetrf_nopiv(h, N, U0, N, sm, ipiv, B, info));
+    // V is well conditioned for these stacks and its unpivoted elimination is
+    // stable, so the LU without row interchanges (GEMM panels, no pivot
+    // searches) comes first; its solve is verified by the residual
+    // ||V Z - E|| / (||V|| ||Z||) and redone with partial pivoting if that is
+    // not at rounding level (adaptive), or flagged (fixed).
+    EUVL_TRY(ew_copy(h, sm * B, Vc, V));
+    EUVL_TRY(zgetrf_nopiv(h, N, V, N, sm, ipiv, B, info));
     bool ok = true;
     if (ctl.fixed)
       EUVL_TRY(flag_info(h, B, info, FLAG_ZERO_PIVOT, ctl.d_flags));
@@ -732,15 +693,15 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
       hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
       EUVL_LAUNCH_CHECK();
       EUVL_TRY(ew_copy(h, sr * B, Bres, Brhs));
-      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+      EUVL_TRY(zgetrs(h, N, ninc, V, N, sm, ipiv, Brhs, ninc, sr, B));
       {
         PrecScope fp64(h, 0);  // the check itself always in FP64
-        EUVL_TRY(zgemm(h, N, ninc, N, ONE, U0c, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
+        EUVL_TRY(zgemm(h, N, ninc, N, ONE, Vc, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
       }
       double* dr = red + 3 * B;
       EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Bres, dr));
       EUVL_TRY(maxabs(h, N, ninc, ninc, sr, B, Brhs, dr + B));
-      EUVL_TRY(reduce(h, 1, N, B, U0c, dr + 2 * B));
+      EUVL_TRY(reduce(h, 1, N, B, Vc, dr + 2 * B));
       if (ctl.fixed) {
         EUVL_TRY(flag_check(h, 2, B, dr, dr + B, dr + 2 * B, 0.0, tol, FLAG_LU_RESIDUAL, ctl.d_flags));
       } else {
@@ -754,20 +715,24 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
     }
     if (!ok) {
       lu_path = 1;
-      EUVL_TRY(ew_copy(h, sm * B, U0, U0c));
-      EUVL_TRY(zgetrf(h, N, U0, N, sm, ipiv, B, info));
+      EUVL_TRY(ew_copy(h, sm * B, V, Vc));
+      EUVL_TRY(zgetrf(h, N, V, N, sm, ipiv, B, info));
       EUVL_TRY(singular(true));
       hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
       EUVL_LAUNCH_CHECK();
-      EUVL_TRY(zgetrs(h, N, ninc, U0, N, sm, ipiv, Brhs, ninc, sr, B));
+      EUVL_TRY(zgetrs(h, N, ninc, V, N, sm, ipiv, Brhs, ninc, sr, B));
     }
-    EUVL_TRY(zgemm(h, N, ninc, N, ONE, R, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));
+    // R X = C Y - i G2 Z with Y = diag(lam) (Z - G Z)  (Z in Brhs)
+    EUVL_TRY(zgemm(h, N, ninc, N, ONE, G, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));
+    hipLaunchKernelGGL(top_y_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, k, Brhs, FGc, v);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, N, ninc, N, ONE, C, N, sm, FGc, ninc, sr, ZERO, Bres, ninc, sr, ZERO, B));
+    EUVL_TRY(zgemm(h, N, ninc, N, MI, G2c, N, sm, Brhs, ninc, sr, ONE, Bres, ninc, sr, ZERO, B));
     hipLaunchKernelGGL(ax_scatter_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx,
-                       t.inc_slot, t.kz, FGc, v.klm, nsxy * N, Ax);
+                       t.inc_slot, t.kz, Bres, v.klm, nsxy * N, Ax);
     EUVL_LAUNCH_CHECK();
   }
   ctl.doublings_used = ndbl;
-  ctl.neumann_used = nfac;
   ctl.lu_path = lu_path;
   return EUVL_OK;
 }
@@ -804,7 +769,7 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
     for (int b = 0; b < B; ++b) {
       h_info[4 * b + 0] = ctl.lu_path;
       h_info[4 * b + 1] = ctl.doublings_used;
-      h_info[4 * b + 2] = ctl.neumann_used + 1;
+      h_info[4 * b + 2] = 0;
       h_info[4 * b + 3] = td.ninc;
     }
   }

@@ -41,7 +41,7 @@ int zero_ints(euvl_handle h, int count, int* p);
 
 // flag bits of the fixed-decision EM solve
 constexpr int FLAG_DOUBLINGS = 1;  // the fixed doubling count is below what ||D^2||, ||D^3|| ask for
-constexpr int FLAG_NEUMANN = 2;    // the fixed Neumann product has not reached its remainder bound
+// bit 2 is not used (the solve has no series whose length is fixed)
 constexpr int FLAG_LU_RESIDUAL = 4;  // the unpivoted top solve failed its residual check
 constexpr int FLAG_ZERO_PIVOT = 8;   // the unpivoted LU met a zero pivot
 
@@ -75,9 +75,9 @@ struct SolveDev {  // the same tables on the device
 };
 struct SolveCtl {
   bool fixed = false;  // decisions fixed ahead of time, checked on the device
-  int doublings = 0, neumann = 1;
+  int doublings = 0;
   int* d_flags = nullptr;  // per batch entry, FLAG_* bits (fixed mode)
-  int doublings_used = 0, neumann_used = 0, lu_path = 0;  // what ran
+  int doublings_used = 0, lu_path = 0;  // what ran
   double alpha = 0.0;  // max(||D^2||^(1/2), ||D^3||^(1/3)) over the batch (adaptive runs)
 };
 void build_solve_tables(const euvl_em_params* p, const int* lindex, const int* mindex, int N, int lmax, int mmax,

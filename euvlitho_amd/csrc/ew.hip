@@ -187,7 +187,6 @@ int check_ew(int n, int batch) {
 
 // flags[b] |= bit where a device-side check of a fixed decision fails:
 //   kind 0 (doubling count): s * max(sqrt(a[b]), cbrt(c[b])) > theta, or non-finite
-//   kind 1 (Neumann length): !(a[b] < 0.5) or a[b]^2 >= theta
 //   kind 2 (LU residual):    !(a[b] <= theta * c[b] * d[b])
 __global__ void flag_kernel(int kind, int batch, const double* a, const double* c, const double* d, double s,
                             double theta, int bit, int* flags) {
@@ -197,8 +196,6 @@ __global__ void flag_kernel(int kind, int batch, const double* a, const double* 
   if (kind == 0) {
     const double al = fmax(sqrt(a[b]), cbrt(c[b]));
     bad = !(s * al <= theta);
-  } else if (kind == 1) {
-    bad = !(a[b] < 0.5) || !(a[b] * a[b] < theta);
   } else {
     bad = !(a[b] <= theta * c[b] * d[b]);
   }

@@ -6,9 +6,8 @@
 // clips x source divisions, so small problems fill the GPU.
 //
 // A run in fixed mode takes no decision on the host: the doubling count of
-// the matrix functions and the length of the Neumann product are fixed in
-// the plan (calibrated by an adaptive run with a margin) and checked on the
-// device, the top solve always takes the unpivoted LU and its residual is
+// the matrix functions is fixed in the plan (calibrated by adaptive runs with
+// a margin) and checked on the device, the top solve always takes the unpivoted LU and its residual is
 // checked on the device; failures set per-entry flags that the caller reads
 // once per batch and redoes in adaptive mode.  A fixed run enqueues work only
 // (no synchronisation, no allocation, no host copy), so it can be captured
@@ -35,7 +34,7 @@ struct euvl_plan_s {
   euvl::SolveDev sd;
   euvl::ImagingDev im;
   const double2* tw_dft = nullptr;
-  int doublings = -1, neumann = -1;
+  int doublings = -1;
   double alpha_max = 0.0, alpha_margin = 1.0;  // calibration of the doubling count
   size_t ws_need = 0, aux_need = 0;
 };
@@ -225,20 +224,18 @@ extern "C" int euvl_plan_destroy(euvl_plan pl) {
 
 extern "C" int64_t euvl_plan_workspace_bytes(euvl_plan pl) { return pl ? (int64_t)pl->ws_need : 0; }
 
-extern "C" int euvl_plan_decisions(euvl_plan pl, int* doublings, int* neumann) {
+extern "C" int euvl_plan_decisions(euvl_plan pl, int* doublings) {
   if (!pl) return EUVL_ERR_ARG;
   if (doublings) *doublings = pl->doublings;
-  if (neumann) *neumann = pl->neumann;
   return EUVL_OK;
 }
 
-extern "C" int euvl_plan_set_decisions(euvl_plan pl, int doublings, int neumann) {
-  if (!pl || doublings < 0 || doublings > 60 || neumann < 1 || neumann > 8) {
+extern "C" int euvl_plan_set_decisions(euvl_plan pl, int doublings) {
+  if (!pl || doublings < 0 || doublings > 60) {
     euvl::set_error("euvl_plan_set_decisions: bad argument");
     return EUVL_ERR_ARG;
   }
   pl->doublings = doublings;
-  pl->neumann = neumann;
   pl->alpha_max = 0.0;  // later adaptive runs calibrate from these values up
   return EUVL_OK;
 }
@@ -251,7 +248,7 @@ extern "C" int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, 
     set_error("euvl_plan_run: bad argument");
     return EUVL_ERR_ARG;
   }
-  if (fixed && (pl->doublings < 0 || pl->neumann < 1)) {
+  if (fixed && pl->doublings < 0) {
     set_error("euvl_plan_run: fixed mode before the decisions are set (run once adaptive or set them)");
     return EUVL_ERR_ARG;
   }
@@ -295,13 +292,12 @@ extern "C" int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, 
   SolveCtl ctl;
   ctl.fixed = fixed != 0;
   ctl.doublings = pl->doublings;
-  ctl.neumann = pl->neumann;
   ctl.d_flags = d_flags;
   EUVL_TRY(amp_s_core(h, pl->polar, &pl->p, pl->sd, pl->N, pl->lmax, pl->mmax, spec, pl->n_off, wem, pl->E,
                       (double2*)d_Ax, ctl));
   if (!fixed) {
     // calibration over every adaptive run so far: the doublings the largest
-    // norm bound alpha needs (times the plan's margin), the longest Neumann product
+    // norm bound alpha needs (times the plan's margin)
     const double theta = cos_sinc_theta(h);
     pl->alpha_max = std::max(pl->alpha_max, ctl.alpha);
     int s = 0;
@@ -309,7 +305,6 @@ extern "C" int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, 
            s < 60)
       ++s;
     pl->doublings = std::max({pl->doublings, s, ctl.doublings_used});
-    pl->neumann = std::max(pl->neumann, ctl.neumann_used);
   }
   h->ws_top = mark;
   const int nsxy = (2 * pl->p.lsmax_x + 1) * (2 * pl->p.lsmax_y + 1);

@@ -194,33 +194,34 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 #pragma unroll
     for (int u = 0; u < SN; ++u) acc_s[t][u] = (V4){0, 0, 0, 0};
 
-  // one K step of 4: the fragments of As/Bs rows ks..ks+3, then the MFMAs
-  auto kstep = [&](int ks) {
-    T2 fa[TM], fb[TN];
-    T nai[TM];
+  // one K step of 4 in two parts: frag() reads the fragments of As/Bs rows
+  // ks..ks+3 into one of two register sets, mma() issues the MFMAs of a set.
+  // The main loop reads step s + 1 before issuing the MFMAs of step s, so the
+  // LDS latency of a step is covered by the MFMAs of the one before.
+  T2 fa[2][TM], fb[2][TN];
+  auto frag = [&](int ks, int set) {
 #pragma unroll
-    for (int t = 0; t < TM; ++t) {
-      fa[t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
-      nai[t] = -fa[t].y;
-    }
+    for (int t = 0; t < TM; ++t) fa[set][t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
 #pragma unroll
-    for (int u = 0; u < TN; ++u) fb[u] = Bs[ks + lk][wn * 16 * TN + u * 16 + li];
+    for (int u = 0; u < TN; ++u) fb[set][u] = Bs[ks + lk][wn * 16 * TN + u * 16 + li];
+  };
+  auto mma = [&](int set) {
     if constexpr (M3) {
       T sa[TM], sb[TN];
 #pragma unroll
-      for (int t = 0; t < TM; ++t) sa[t] = fa[t].x + fa[t].y;
+      for (int t = 0; t < TM; ++t) sa[t] = fa[set][t].x + fa[set][t].y;
 #pragma unroll
-      for (int u = 0; u < TN; ++u) sb[u] = fb[u].x + fb[u].y;
-#pragma unroll
-      for (int t = 0; t < TM; ++t)
-#pragma unroll
-        for (int u = 0; u < TN; ++u)
-          acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
+      for (int u = 0; u < TN; ++u) sb[u] = fb[set][u].x + fb[set][u].y;
 #pragma unroll
       for (int t = 0; t < TM; ++t)
 #pragma unroll
         for (int u = 0; u < TN; ++u)
-          acc_im[t][u] = P::mfma(fa[t].y, fb[u].y, acc_im[t][u]);
+          acc_re[t][u] = P::mfma(fa[set][t].x, fb[set][u].x, acc_re[t][u]);
+#pragma unroll
+      for (int t = 0; t < TM; ++t)
+#pragma unroll
+        for (int u = 0; u < TN; ++u)
+          acc_im[t][u] = P::mfma(fa[set][t].y, fb[set][u].y, acc_im[t][u]);
 #pragma unroll
       for (int t = 0; t < SM; ++t)
 #pragma unroll
@@ -228,28 +229,31 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
           acc_s[t][u] = P::mfma(sa[t], sb[u], acc_s[t][u]);
       return;
     }
+    T nai[TM];
+#pragma unroll
+    for (int t = 0; t < TM; ++t) nai[t] = -fa[set][t].y;
     // four passes over the tiles so that successive updates of one
     // accumulator are separated by independent MFMAs
 #pragma unroll
     for (int t = 0; t < TM; ++t)
 #pragma unroll
       for (int u = 0; u < TN; ++u)
-        acc_re[t][u] = P::mfma(fa[t].x, fb[u].x, acc_re[t][u]);
+        acc_re[t][u] = P::mfma(fa[set][t].x, fb[set][u].x, acc_re[t][u]);
 #pragma unroll
     for (int t = 0; t < TM; ++t)
 #pragma unroll
       for (int u = 0; u < TN; ++u)
-        acc_im[t][u] = P::mfma(fa[t].x, fb[u].y, acc_im[t][u]);
+        acc_im[t][u] = P::mfma(fa[set][t].x, fb[set][u].y, acc_im[t][u]);
 #pragma unroll
     for (int t = 0; t < TM; ++t)
 #pragma unroll
       for (int u = 0; u < TN; ++u)
-        acc_re[t][u] = P::mfma(nai[t], fb[u].y, acc_re[t][u]);
+        acc_re[t][u] = P::mfma(nai[t], fb[set][u].y, acc_re[t][u]);
 #pragma unroll
     for (int t = 0; t < TM; ++t)
 #pragma unroll
       for (int u = 0; u < TN; ++u)
-        acc_im[t][u] = P::mfma(fa[t].y, fb[u].x, acc_im[t][u]);
+        acc_im[t][u] = P::mfma(fa[set][t].y, fb[set][u].x, acc_im[t][u]);
   };
 
   // waves whose whole sub-tile lies outside C (edge tiles) add nothing: they
@@ -266,16 +270,20 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     __syncthreads();
     const bool more = k0 + BK < K;
     if constexpr (!FAST) {
+      frag(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
+        if (ks + 4 < BK) frag(ks + 4, ((ks >> 2) + 1) & 1);
         if (ks == 0 && more) load_slab(k0 + BK);
-        kstep(ks);
+        mma((ks >> 2) & 1);
       }
     } else if (active) {
+      frag(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
+        if (ks + 4 < BK) frag(ks + 4, ((ks >> 2) + 1) & 1);
         if (ks == 4 && more) load_slab(k0 + BK);
-        kstep(ks);
+        mma((ks >> 2) & 1);
       }
     } else if (more) {
       load_slab(k0 + BK);

@@ -17,8 +17,8 @@ that fits is always taken first.
 
 Mask-independent state is resident and shared by every slot (one Resident
 store): the blank-mask amplitudes per configuration, the SOCS kernels and
-the decisions of the fixed-decision EM solve (doubling count and Neumann
-length, calibrated once per problem at warm-up on seeded masks with a
+the decision of the fixed-decision EM solve (its doubling count,
+calibrated once per problem at warm-up on seeded masks with a
 factor-2 margin on the norm bound, and checked on the device for every clip;
 a flagged batch is run again with host decisions).
 
@@ -137,7 +137,7 @@ class _Slot:
             with torch.cuda.stream(self.stream):
                 self.bind()
                 plan = Plan(self.engine(cfg), n)
-                _lib.check(self.lib.euvl_plan_set_decisions(plan.pl, *decisions), "euvl_plan_set_decisions")
+                _lib.check(self.lib.euvl_plan_set_decisions(plan.pl, int(decisions)), "euvl_plan_set_decisions")
                 nbytes = cfg.ndivx * cfg.ndivy // 8
                 bits = torch.empty(n * nbytes, dtype=torch.uint8, device=self.device)
                 flags = torch.zeros(n * plan.n_off, dtype=torch.int32, device=self.device)
@@ -180,8 +180,7 @@ class Server:
 
     # ------------------------------------------------------------ residency
     def decisions(self, cfg):
-        """Doubling count and Neumann length of the fixed-decision EM solve of
-        cfg: calibrated once (resident), on seeded masks of vacuum fraction
+        """Doubling count of the fixed-decision EM solve of cfg: calibrated once (resident), on seeded masks of vacuum fraction
         0.3, 0.5 and 0.7 in adaptive runs, with a factor-2 margin on the norm
         bound behind the doubling count."""
         key = ("decisions", cfg, self.precision)

@@ -31,7 +31,7 @@ extern "C" {
 #define EUVL_ERR_ARG -1          /* bad argument (size, pointer, polarisation) */
 #define EUVL_ERR_HIP -2          /* HIP runtime failure (launch, allocation) */
 #define EUVL_ERR_SINGULAR -3     /* zero pivot in an LU factorisation */
-#define EUVL_ERR_NOT_CONVERGED -4 /* matrix square root / Neumann / expm bound failed */
+#define EUVL_ERR_NOT_CONVERGED -4 /* a series or iteration bound failed (cos/sinc norms, Jacobi sweeps) */
 #define EUVL_ERR_NOMEM -5        /* workspace allocation failed */
 
 typedef struct euvl_handle_s* euvl_handle;
@@ -218,8 +218,7 @@ int euvl_multilayer(euvl_handle h, char polar, const euvl_em_params* p, int n,
  * the lattice incidence (ls, ms) inside the NA and 0 elsewhere.
  * h_info (optional, 4 ints per offset): LU path of the top solve (0: without
  * row interchanges, verified by its residual; 1: the partial-pivoting
- * fallback), cos/sinc doublings, Neumann factors applied, incidence columns
- * solved.
+ * fallback), cos/sinc doublings, 0 (reserved), incidence columns solved.
  * Replaces ampS (cnn/include/header.h:1208-1466) with absorberS0
  * (header.h:754-1145) and the top solve (header.h:1355-1391). */
 int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p,
@@ -444,19 +443,19 @@ int euvl_plan_reserve(euvl_handle h, int n_plans, const euvl_plan* plans);
 /* One run over n_clips packed masks (d_bits, clip after clip).  Outputs:
  * d_Ax [clip][division][nsx][nsy][nrange] complex; d_a0, d_ax, d_ay
  * [clip][ncut] complex; d_img [clip][nx][ny] float64 (overwritten).
- * fixed = 0: adaptive (host-read norms pick the doubling count and the
- * Neumann length, the top solve falls back to partial pivoting), and the plan
+ * fixed = 0: adaptive (host-read norms pick the doubling count, the top
+ * solve falls back to partial pivoting), and the plan
  * keeps the largest decisions of all its adaptive runs (the doublings for
  * alpha_margin times the largest norm bound seen).  fixed = 1: the plan's decisions are used as they are, checked
- * on the device; d_flags [clip x division] receives 1 (doubling bound), 2
- * (Neumann remainder), 4 (top-solve residual), 8 (zero pivot) where a check
+ * on the device; d_flags [clip x division] receives 1 (doubling bound),
+ * 4 (top-solve residual), 8 (zero pivot) where a check
  * failed -- those clips must be run again adaptively.  A fixed run only
  * enqueues work (no synchronisation, allocation or host copy) and may be
  * captured; the caller synchronises before reading anything. */
 int euvl_plan_run(euvl_plan pl, const uint8_t* d_bits, double* d_Ax, double* d_a0, double* d_ax, double* d_ay,
                   double* d_img, int* d_flags, int fixed);
-int euvl_plan_decisions(euvl_plan pl, int* doublings, int* neumann);
-int euvl_plan_set_decisions(euvl_plan pl, int doublings, int neumann);
+int euvl_plan_decisions(euvl_plan pl, int* doublings);
+int euvl_plan_set_decisions(euvl_plan pl, int doublings);
 
 /* HIP graph of the work enqueued on the handle's stream between begin and
  * end (fixed-mode plan runs; the stream must not be the null stream).  A
