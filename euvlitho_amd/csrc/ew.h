@@ -40,14 +40,14 @@ int flag_info(euvl_handle h, int batch, const int* info, int bit, int* flags);
 int zero_ints(euvl_handle h, int count, int* p);
 
 // flag bits of the fixed-decision EM solve
-constexpr int FLAG_DOUBLINGS = 1;  // the fixed doubling count is below what ||D^2||, ||D^3|| ask for
+constexpr int FLAG_DOUBLINGS = 1;  // the fixed doubling count is below what ||D^3||, ||D^4|| ask for
 // bit 2 is not used (the solve has no series whose length is fixed)
 constexpr int FLAG_LU_RESIDUAL = 4;  // the unpivoted top solve failed its residual check
 constexpr int FLAG_ZERO_PIVOT = 8;   // the unpivoted LU met a zero pivot
 
 // matrix functions (matfun.hip); buffers are batch x n x n contiguous.
 // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (G2 may be null); takes 6 workspace matrices.
-// fixed_s < 0: the doubling count is chosen from ||D^2||, ||D^3|| read back to the host (*doublings
+// fixed_s < 0: the doubling count is chosen from ||D^3||, ||D^4|| read back to the host (*doublings
 // receives it).  fixed_s >= 0: that count is used without a host read-back; the norms go to
 // d_red[0, 2 batch) and entries whose bound needs more doublings get FLAG_DOUBLINGS in d_flags.
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
@@ -78,7 +78,7 @@ struct SolveCtl {
   int doublings = 0;
   int* d_flags = nullptr;  // per batch entry, FLAG_* bits (fixed mode)
   int doublings_used = 0, lu_path = 0;  // what ran
-  double alpha = 0.0;  // max(||D^2||^(1/2), ||D^3||^(1/3)) over the batch (adaptive runs)
+  double alpha = 0.0;  // max(||D^3||^(1/3), ||D^4||^(1/4)) over the batch (adaptive runs)
 };
 void build_solve_tables(const euvl_em_params* p, const int* lindex, const int* mindex, int N, int lmax, int mmax,
                         int n_off, const double* sx0, const double* sy0, int n_clips, SolveHost& t);

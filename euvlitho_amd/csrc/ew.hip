@@ -186,7 +186,7 @@ int check_ew(int n, int batch) {
 }
 
 // flags[b] |= bit where a device-side check of a fixed decision fails:
-//   kind 0 (doubling count): s * max(sqrt(a[b]), cbrt(c[b])) > theta, or non-finite
+//   kind 0 (doubling count): s * max(cbrt(a[b]), c[b]^(1/4)) > theta, or non-finite
 //   kind 2 (LU residual):    !(a[b] <= theta * c[b] * d[b])
 __global__ void flag_kernel(int kind, int batch, const double* a, const double* c, const double* d, double s,
                             double theta, int bit, int* flags) {
@@ -194,7 +194,7 @@ __global__ void flag_kernel(int kind, int batch, const double* a, const double* 
   if (b >= batch) return;
   bool bad;
   if (kind == 0) {
-    const double al = fmax(sqrt(a[b]), cbrt(c[b]));
+    const double al = fmax(cbrt(a[b]), sqrt(sqrt(c[b])));
     bad = !(s * al <= theta);
   } else {
     bad = !(a[b] <= theta * c[b] * d[b]);

@@ -1,10 +1,10 @@
 // Matrix functions of the absorber-layer operator D built from the MFMA GEMM:
 //   C  = cos(d sqrt(D)),  F = sin(d sqrt(D)) / sqrt(D),  G2 = D F = sqrt(D) sin(d sqrt(D)).
 // All three are even in sqrt(D), i.e. power series in D: no square root, no
-// branch choice and no eigendecomposition.  They are evaluated by a degree-9
+// branch choice and no eigendecomposition.  They are evaluated by a degree-12
 // Taylor series in x = h^2 D (h = d / 2^s, Paterson-Stockmeyer in blocks of
-// x^3) followed by s doublings C <- 2 C^2 - I, F <- 2 F C; s is set by the
-// norms of D^2 and D^3 rather than of D.
+// x^4) followed by s doublings C <- 2 C^2 - I, F <- 2 F C; s is set by the
+// norms of D^3 and D^4 rather than of D.
 #include <cmath>
 
 #include "ew.h"
@@ -18,10 +18,12 @@ namespace euvl {
 
 // D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.
 double cos_sinc_theta(euvl_handle h) {
-  // With FP32 products (the precision form) the remainder only has to stay
-  // below FP32 rounding: h^2 alpha <= 8 bounds it by 8^10 / 20! = 4.4e-10.
+  // The degree-12 remainder is bounded by (h^2 alpha)^13 / 26!: h^2 alpha <= 5.9
+  // keeps it below 2.6e-17 (FP64 products); with FP32 products (the precision
+  // form) it only has to stay below FP32 rounding, and h^2 alpha <= 24 bounds
+  // it by 2.2e-9.
   PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
-  return (h->prec_mask & h->prec_scope) ? 8.0 : 1.5;
+  return (h->prec_mask & h->prec_scope) ? 24.0 : 5.9;
 }
 
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
@@ -38,22 +40,23 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     }
   }
   const int64_t sm = (int64_t)n * n;
-  // Unscaled powers P2 = D^2, P3 = D^3 (the Paterson-Stockmeyer blocks).  For
-  // every j >= 2, ||D^j||^(1/j) <= alpha = max(||D^2||^(1/2), ||D^3||^(1/3))
-  // (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009), Lemma 4.1), which
-  // is often well below ||D||: s is the smallest with h^2 alpha <= 1.5, so that
-  // the degree-9 remainder, bounded by (h^2 alpha)^10 / 20!, is below 3e-17
-  // (FP64 products).
+  // Unscaled powers P2 = D^2, P3 = D^3, P4 = D^4 (the Paterson-Stockmeyer
+  // blocks).  For every j >= 6, ||D^j||^(1/j) <= alpha = max(||D^3||^(1/3),
+  // ||D^4||^(1/4)) (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009),
+  // Lemma 4.1), which is often well below ||D||: s is the smallest with
+  // h^2 alpha <= theta, so that the remainder of both series (terms j >= 13) is
+  // below rounding.
   const double2* X1 = D;
-  double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *T = s[5];
+  double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *X4 = s[5];
   EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
   EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, X2, n, sm, ZERO, X4, n, sm, ZERO, batch));
   int sq = 0;
   if (fixed_s >= 0) {
     // the bound is checked on the device; no read-back
     sq = fixed_s;
-    EUVL_TRY(reduce(h, 1, n, batch, X2, d_red));
-    EUVL_TRY(reduce(h, 1, n, batch, X3, d_red + batch));
+    EUVL_TRY(reduce(h, 1, n, batch, X3, d_red));
+    EUVL_TRY(reduce(h, 1, n, batch, X4, d_red + batch));
     const double hh = std::ldexp(d, -sq);
     EUVL_TRY(flag_check(h, 0, batch, d_red, d_red + batch, nullptr, hh * hh, theta, FLAG_DOUBLINGS, d_flags));
   } else {
@@ -61,17 +64,17 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
       set_error("cos_sinc: batch %d above the 256 of the host read-back", batch);
       return EUVL_ERR_ARG;
     }
-    EUVL_TRY(reduce(h, 1, n, batch, X2, h->d_red));
-    EUVL_TRY(reduce(h, 1, n, batch, X3, h->d_red + 256));
+    EUVL_TRY(reduce(h, 1, n, batch, X3, h->d_red));
+    EUVL_TRY(reduce(h, 1, n, batch, X4, h->d_red + 256));
     EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * 512, hipMemcpyDeviceToHost, h->stream));
     EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
     double alpha = 0.0;
     for (int b = 0; b < batch; ++b) {
-      const double a2 = std::sqrt(h->h_pinned[b]), a3 = std::cbrt(h->h_pinned[256 + b]);
-      alpha = std::max(alpha, (std::isnan(a2) || std::isnan(a3)) ? INFINITY : std::max(a2, a3));
+      const double a3 = std::cbrt(h->h_pinned[b]), a4 = std::sqrt(std::sqrt(h->h_pinned[256 + b]));
+      alpha = std::max(alpha, (std::isnan(a3) || std::isnan(a4)) ? INFINITY : std::max(a3, a4));
     }
     if (!std::isfinite(alpha)) {
-      set_error("cos_sinc: non-finite ||D^2|| or ||D^3||");
+      set_error("cos_sinc: non-finite ||D^3|| or ||D^4||");
       return EUVL_ERR_NOT_CONVERGED;
     }
     while (std::ldexp(d, -sq) * std::ldexp(d, -sq) * alpha > theta && sq < 60) ++sq;
@@ -80,9 +83,9 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   if (doublings) *doublings = sq;
   const double hh = std::ldexp(d, -sq);
   // q(x) = sum c_j x^j with x = h^2 D is evaluated as sum (c_j h^(2j)) D^j on the unscaled powers
-  double cc[10], cf[10];
+  double cc[13], cf[13];
   double fact = 1.0;  // m!
-  for (int m = 0; m < 20; ++m) {
+  for (int m = 0; m < 26; ++m) {
     if (m > 0) fact *= m;
     const double sgn = ((m / 2) % 2) ? -1.0 : 1.0;
     const double scale = std::pow(hh * hh, m / 2);
@@ -91,13 +94,14 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     else
       cf[m / 2] = sgn * hh * scale / fact;
   }
-  // q = Q0 + P3 (Q1 + P3 (Q2 + c9 P3)), Qi = c_3i I + c_3i+1 D + c_3i+2 P2
+  // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2)
   auto poly = [&](const double* c, double2* out) -> int {
-    EUVL_TRY(ew_lincomb(h, n, batch, H, re(c[9]), X3, re(c[8]), X2, re(c[7]), X1, re(c[6])));
-    EUVL_TRY(ew_lincomb(h, n, batch, Q, re(c[5]), X2, re(c[4]), X1, ZERO, nullptr, re(c[3])));
-    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X3, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
-    EUVL_TRY(ew_lincomb(h, n, batch, out, re(c[2]), X2, re(c[1]), X1, ZERO, nullptr, re(c[0])));
-    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X3, n, sm, Q, n, sm, re(1.0), out, n, sm, ZERO, batch));
+    EUVL_TRY(ew_lincomb(h, n, batch, H, re(c[11]), X3, re(c[10]), X2, re(c[9]), X1, re(c[8])));
+    EUVL_TRY(ew_lincomb(h, n, batch, H, re(1.0), H, re(c[12]), X4, ZERO, nullptr, ZERO));
+    EUVL_TRY(ew_lincomb(h, n, batch, Q, re(c[7]), X3, re(c[6]), X2, re(c[5]), X1, re(c[4])));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
+    EUVL_TRY(ew_lincomb(h, n, batch, out, re(c[3]), X3, re(c[2]), X2, re(c[1]), X1, re(c[0])));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Q, n, sm, re(1.0), out, n, sm, ZERO, batch));
     return EUVL_OK;
   };
   EUVL_TRY(poly(cc, C));
@@ -112,7 +116,6 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   }
   if (c != C) EUVL_TRY(ew_copy(h, sm * batch, C, c));
   if (f != F) EUVL_TRY(ew_copy(h, sm * batch, F, f));
-  (void)T;
   if (G2) EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, F, n, sm, ZERO, G2, n, sm, ZERO, batch));
   return EUVL_OK;
 }
