@@ -75,6 +75,7 @@ _SIGS = {
                            c_int64, P_double, c_void_p, c_int64, c_int64, P_double, c_int],
     "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
     "euvl_zgetrf_nopiv_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],
+    "euvl_zgesv_nopiv_batched": [c_void_p, c_int, c_int, c_void_p, c_int64, c_int64, c_int],
     "euvl_zgetrs_batched": [c_void_p, c_int, c_int, c_void_p, c_int64, c_int64, c_void_p, c_void_p, c_int64,
                             c_int64, c_int],
     "euvl_zcos_sinc_batched": [c_void_p, c_int, c_void_p, c_double, c_void_p, c_void_p, c_void_p, c_int, P_int],

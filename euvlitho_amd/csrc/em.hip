@@ -224,7 +224,7 @@ __device__ __forceinline__ double2 top_lambda(const ModeVecs& v, int64_t j, doub
 
 // H = (C - i diag(klm) F) diag(lam),  V = diag(klm) C - i G2 + H  (V gets - H G next)
 __global__ void top_hv_kernel(int n, int batch, double k, const double2* C, const double2* F, const double2* G2,
-                              double2* H, double2* V, ModeVecs v) {
+                              double2* H, double2* V, int64_t ldv, int64_t sv, ModeVecs v) {
   const int64_t per = (int64_t)n * n;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
@@ -235,7 +235,7 @@ __global__ void top_hv_kernel(int n, int batch, double k, const double2* C, cons
     const double2 c = C[e], kf = cmul(klm, F[e]), g2 = G2[e], kc = cmul(klm, c);
     const double2 hv = cmul(make_double2(c.x + kf.y, c.y - kf.x), top_lambda(v, (int64_t)b * n + j, k));
     H[e] = hv;
-    V[e] = make_double2(kc.x + g2.y + hv.x, kc.y - g2.x + hv.y);  // klm c - i g2 + h
+    V[(int64_t)b * sv + (int64_t)i * ldv + j] = make_double2(kc.x + g2.y + hv.x, kc.y - g2.x + hv.y);  // klm c - i g2 + h
   }
 }
 
@@ -255,14 +255,30 @@ __global__ void fill_kernel(int64_t count, double2* out, double2 v) {
     out[e] = v;
 }
 
-// right-hand sides e_idx(c) for every batch entry: Bm (n x ninc)
-__global__ void rhs_kernel(int n, int ninc, int batch, const int* idx, double2* Bm) {
+// right-hand sides e_idx(c) for every batch entry: Bm (n x ninc, leading
+// dimension ld, batch stride sb)
+__global__ void rhs_kernel(int n, int ninc, int batch, const int* idx, double2* Bm, int64_t ld, int64_t sb) {
   const int64_t per = (int64_t)n * ninc;
   const int64_t total = per * batch;
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t r = e % per;
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
     const int i = (int)(r / ninc), c = (int)(r % ninc);
-    Bm[e] = (idx[c] == i) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
+    Bm[(int64_t)b * sb + (int64_t)i * ld + c] = (idx[c] == i) ? make_double2(1.0, 0.0) : make_double2(0.0, 0.0);
+  }
+}
+
+// dst = src for a rows x cols block of every batch entry (leading dimensions
+// and batch strides of their own)
+__global__ void copy2d_kernel(int rows, int cols, int batch, const double2* src, int64_t lds, int64_t ss, double2* dst,
+                              int64_t ldd, int64_t sd) {
+  const int64_t per = (int64_t)rows * cols;
+  const int64_t total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int i = (int)(r / cols), c = (int)(r % cols);
+    dst[(int64_t)b * sd + (int64_t)i * ldd + c] = src[(int64_t)b * ss + (int64_t)i * lds + c];
   }
 }
 
@@ -575,7 +591,7 @@ size_t solve_workspace_bytes(int N, int E, int ninc) {
   const size_t mat = sizeof(double2) * (size_t)N * N * E;
   const size_t vec = sizeof(double2) * (size_t)N * E;
   const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * E;
-  return 13 * mat + 8 * vec + 2 * rhs + sizeof(int) * (size_t)N * E + sizeof(double) * 8 * (size_t)E +
+  return 13 * mat + 8 * vec + 3 * rhs + sizeof(int) * (size_t)N * E + sizeof(double) * 8 * (size_t)E +
          sizeof(int) * (size_t)E + 64 * 1024;
 }
 
@@ -604,7 +620,10 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   v.alru = (double2*)ws_take(h, vec);
   v.b2 = (double2*)ws_take(h, vec);
   double2* Mx[7];
-  for (int i = 0; i < 7; ++i) Mx[i] = (double2*)ws_take(h, mat);
+  for (int i = 0; i < 7; ++i) Mx[i] = (i == 5) ? nullptr : (double2*)ws_take(h, mat);
+  // the top-solve matrix with its right-hand sides appended: [V | E], N x (N + ninc)
+  const int64_t ldv = (int64_t)N + ninc, sv = (int64_t)N * ldv;
+  Mx[5] = (double2*)ws_take(h, sizeof(double2) * (size_t)sv * B);
   double2* Brhs = (double2*)ws_take(h, rhs);
   double2* FGc = (double2*)ws_take(h, rhs);
   int* ipiv = (int*)ws_take(h, sizeof(int) * (size_t)N * B);
@@ -654,10 +673,10 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   // T1 / T2 blocks (header.h:1053-1144, 1355-1391).
   PrecScope scope(h, EUVL_PREC_FP32_COUPLING);
   double2* H = Mx[6];
-  double2* V = Mx[5];
-  hipLaunchKernelGGL(top_hv_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, C, Fs, G2c, H, V, v);
+  double2* V = Mx[5];  // [V | E] with leading dimension N + ninc
+  hipLaunchKernelGGL(top_hv_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, C, Fs, G2c, H, V, ldv, sv, v);
   EUVL_LAUNCH_CHECK();
-  EUVL_TRY(zgemm(h, N, N, N, MONE, H, N, sm, G, N, sm, ONE, V, N, sm, ZERO, B));            // V = P + H - H G
+  EUVL_TRY(zgemm(h, N, N, N, MONE, H, N, sm, G, N, sm, ONE, V, ldv, sv, ZERO, B));          // V = P + H - H G
   double2* Vc = Mx[0];    // D is no longer needed: a copy of V for the residual check
   double2* Bres = Mx[6];  // nor H: the right-hand sides kept for the check
   const int64_t nsxy = (int64_t)nsx * nsy;
@@ -677,23 +696,32 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
         }
       return EUVL_OK;
     };
-    // V is well conditioned for these stacks and its unpivoted elimination is
-    // stable, so the LU without row interchanges (GEMM panels, no pivot
-    // searches) comes first; its solve is verified by the residual
-    // ||V Z - E|| / (||V|| ||Z||) and redone with partial pivoting if that is
-    // not at rounding level (adaptive), or flagged (fixed).
-    EUVL_TRY(ew_copy(h, sm * B, Vc, V));
-    EUVL_TRY(zgetrf_nopiv(h, N, V, N, sm, ipiv, B, info));
+    auto copy2d = [&](int rows, int cols, const double2* src, int64_t lds, int64_t ss, double2* dst, int64_t ldd,
+                      int64_t sd) -> int {
+      hipLaunchKernelGGL(copy2d_kernel, grid1((int64_t)rows * cols * B), dim3(256), 0, h->stream, rows, cols, B, src,
+                         lds, ss, dst, ldd, sd);
+      EUVL_LAUNCH_CHECK();
+      return EUVL_OK;
+    };
+    // V is well conditioned for these stacks and its elimination without row
+    // interchanges is stable, so the block Gauss-Jordan solve (no pivot
+    // searches) comes first; it is verified by the residual
+    // ||V Z - E|| / (||V|| ||Z||) and redone with a partial-pivoting LU if that
+    // is not at rounding level (adaptive), or flagged (fixed).
+    EUVL_TRY(copy2d(N, N, V, ldv, sv, Vc, N, sm));
+    hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, V + N, ldv, sv);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(gj_solve(h, N, ninc, V, ldv, sv, B, info));
     bool ok = true;
     if (ctl.fixed)
       EUVL_TRY(flag_info(h, B, info, FLAG_ZERO_PIVOT, ctl.d_flags));
     else
       ok = singular(false) == EUVL_OK;
     if (ok) {
-      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
+      EUVL_TRY(copy2d(N, ninc, V + N, ldv, sv, Brhs, ninc, sr));
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Bres,
+                         (int64_t)ninc, sr);
       EUVL_LAUNCH_CHECK();
-      EUVL_TRY(ew_copy(h, sr * B, Bres, Brhs));
-      EUVL_TRY(zgetrs(h, N, ninc, V, N, sm, ipiv, Brhs, ninc, sr, B));
       {
         PrecScope fp64(h, 0);  // the check itself always in FP64
         EUVL_TRY(zgemm(h, N, ninc, N, ONE, Vc, N, sm, Brhs, ninc, sr, MONE, Bres, ninc, sr, ZERO, B));
@@ -715,12 +743,13 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
     }
     if (!ok) {
       lu_path = 1;
-      EUVL_TRY(ew_copy(h, sm * B, V, Vc));
-      EUVL_TRY(zgetrf(h, N, V, N, sm, ipiv, B, info));
+      EUVL_TRY(copy2d(N, N, Vc, N, sm, V, ldv, sv));
+      EUVL_TRY(zgetrf(h, N, V, ldv, sv, ipiv, B, info));
       EUVL_TRY(singular(true));
-      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs);
+      hipLaunchKernelGGL(rhs_kernel, grid1(sr * B), dim3(256), 0, h->stream, N, ninc, B, t.inc_idx, Brhs,
+                         (int64_t)ninc, sr);
       EUVL_LAUNCH_CHECK();
-      EUVL_TRY(zgetrs(h, N, ninc, V, N, sm, ipiv, Brhs, ninc, sr, B));
+      EUVL_TRY(zgetrs(h, N, ninc, V, ldv, sv, ipiv, Brhs, ninc, sr, B));
     }
     // R X = C Y - i G2 Z with Y = diag(lam) (Z - G Z)  (Z in Brhs)
     EUVL_TRY(zgemm(h, N, ninc, N, ONE, G, N, sm, Brhs, ninc, sr, ZERO, FGc, ninc, sr, ZERO, B));

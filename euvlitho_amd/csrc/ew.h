@@ -31,6 +31,10 @@ int maxabs_to_host(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int 
 // LU without row interchanges (blocked, GEMM panels); ipiv is written as the identity.  Only for matrices
 // whose unpivoted elimination is safe; callers verify what they solve with it.
 int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info);
+// A X = B on the augmented [A | B] (n x (n + nrhs), lda >= n + nrhs) by block
+// Gauss-Jordan elimination without row interchanges; X replaces B, the left
+// n columns are overwritten; info[b] = first zero pivot (1-based) or 0.
+int gj_solve(euvl_handle h, int n, int nrhs, double2* A, int64_t lda, int64_t sa, int batch, int* d_info);
 
 // Device-side checks of decisions fixed ahead of time (graph-capturable runs):
 // flags[b] |= bit where the check fails (see flag_kernel in ew.hip).

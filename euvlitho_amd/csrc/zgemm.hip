@@ -195,10 +195,13 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     for (int u = 0; u < SN; ++u) acc_s[t][u] = (V4){0, 0, 0, 0};
 
   // one K step of 4 in two parts: frag() reads the fragments of As/Bs rows
-  // ks..ks+3 into one of two register sets, mma() issues the MFMAs of a set.
-  // The main loop reads step s + 1 before issuing the MFMAs of step s, so the
-  // LDS latency of a step is covered by the MFMAs of the one before.
-  T2 fa[2][TM], fb[2][TN];
+  // ks..ks+3 into a register set, mma() issues the MFMAs of a set.  The FP32
+  // kernels keep two sets and read step s + 1 before issuing the MFMAs of
+  // step s, so the LDS latency of a step is covered by the MFMAs before it
+  // (2-4 % on MI355X); the FP64 kernels need those registers for their
+  // accumulators and use one set.
+  constexpr int NSET = sizeof(T) == 4 ? 2 : 1;
+  T2 fa[NSET][TM], fb[NSET][TN];
   auto frag = [&](int ks, int set) {
 #pragma unroll
     for (int t = 0; t < TM; ++t) fa[set][t] = As[ks + lk][wm * 16 * TM + t * 16 + li];
@@ -270,29 +273,50 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     __syncthreads();
     const bool more = k0 + BK < K;
     if constexpr (!FAST) {
-      frag(0, 0);
+      if (NSET == 2) frag(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
-        if (ks + 4 < BK) frag(ks + 4, ((ks >> 2) + 1) & 1);
+        if (NSET == 1)
+          frag(ks, 0);
+        else if (ks + 4 < BK)
+          frag(ks + 4, ((ks >> 2) + 1) % NSET);
         if (ks == 0 && more) load_slab(k0 + BK);
-        mma((ks >> 2) & 1);
+        mma((ks >> 2) % NSET);
       }
     } else if (active) {
-      frag(0, 0);
+      if (NSET == 2) frag(0, 0);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
-        if (ks + 4 < BK) frag(ks + 4, ((ks >> 2) + 1) & 1);
+        if (NSET == 1)
+          frag(ks, 0);
+        else if (ks + 4 < BK)
+          frag(ks + 4, ((ks >> 2) + 1) % NSET);
         if (ks == 4 && more) load_slab(k0 + BK);
-        mma((ks >> 2) & 1);
+        mma((ks >> 2) % NSET);
       }
     } else if (more) {
       load_slab(k0 + BK);
     }
   }
 
+  // epilogue, one row of MFMA tiles at a time: with beta != 0 the row's C
+  // values are loaded first (addresses clamped into C, no branches), so those
+  // loads are in flight together instead of one exposed latency per element
   const bool use_beta = (beta.x != 0.0 || beta.y != 0.0);
 #pragma unroll
-  for (int t = 0; t < TM; ++t)
+  for (int t = 0; t < TM; ++t) {
+    double2 cin[TN][4];
+    if (use_beta) {
+#pragma unroll
+      for (int u = 0; u < TN; ++u)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int gr = row0 + wm * 16 * TM + t * 16 + P::row(r, lk);
+          const int gc = col0 + wn * 16 * TN + u * 16 + li;
+          const bool in = gr < M && gc < N;
+          cin[u][r] = C[in ? (int64_t)gr * ldc + gc : 0];
+        }
+    }
 #pragma unroll
     for (int u = 0; u < TN; ++u)
 #pragma unroll
@@ -307,12 +331,12 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
           } else {
             v = cmul(alpha, make_double2((double)acc_re[t][u][r], (double)acc_im[t][u][r]));
           }
-          double2* cp = C + (int64_t)gr * ldc + gc;
-          if (use_beta) v = cadd(v, cmul(beta, *cp));
+          if (use_beta) v = cadd(v, cmul(beta, cin[u][r]));
           if (gr == gc) v = cadd(v, gamma);
-          *cp = v;
+          C[(int64_t)gr * ldc + gc] = v;
         }
       }
+  }
 }
 
 // FP64 kernel variants (K slab 16): 0 = 2x2 waves, 64x64, uncapped registers

@@ -436,6 +436,64 @@ __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, in
   if (live) X[(int64_t)i * ldx + j] = x;
 }
 
+// Block Gauss-Jordan solve without row interchanges: the inverse of one
+// GJ_NB x GJ_NB diagonal block per batch entry, by one workgroup with the
+// block in LDS.  Step p turns column p into the unit vector: the pivot row is
+// scaled by 1 / a(p,p) and eliminated from every other row, and column p takes
+// the factors, so after GJ_NB steps the block holds its inverse.  Step p reads
+// buffer p & 1 and writes the other one, so one barrier per step orders it.
+// Outside the nbc x nbc block the identity is used; a zero pivot inside it
+// sets info (1-based column) and leaves the rest as it is.
+constexpr int GJ_NB = 64;
+constexpr int GJ_THREADS = 1024;
+__global__ __launch_bounds__(GJ_THREADS) void gj_inv_kernel(const double2* __restrict__ A, int64_t lda, int64_t sa,
+                                                            int kb, int nbc, double2* __restrict__ D,
+                                                            int* __restrict__ info) {
+  constexpr int RPT = GJ_NB * GJ_NB / GJ_THREADS;  // rows per thread (4)
+  constexpr int RSTEP = GJ_THREADS / GJ_NB;        // row stride between them (16)
+  __shared__ double2 a[2][GJ_NB][GJ_NB + 1];
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int c = t & (GJ_NB - 1), r0 = t / GJ_NB;
+  const double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = r0 + RSTEP * q;
+    a[0][r][c] = (r < nbc && c < nbc) ? A11[(int64_t)r * lda + c] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+#pragma unroll 1
+  for (int p = 0; p < GJ_NB; ++p) {
+    const int cur = p & 1, nxt = cur ^ 1;
+    const double2 app = a[cur][p][p];
+    const bool zero = app.x == 0.0 && app.y == 0.0;
+    if (zero && t == 0 && p < nbc && info[b] == 0) info[b] = kb + p + 1;
+    // 1 / a(p,p) = conj(a(p,p)) / |a(p,p)|^2: one real division
+    const double inv2 = zero ? 0.0 : 1.0 / (app.x * app.x + app.y * app.y);
+    const double2 rp = make_double2(app.x * inv2, -app.y * inv2);
+    const double2 apc = cmul(a[cur][p][c], rp);  // the scaled pivot row, column c
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = r0 + RSTEP * q;
+      const double2 aip = a[cur][i][p], aic = a[cur][i][c];
+      double2 v;
+      if (i == p)
+        v = (c == p) ? rp : apc;
+      else if (c == p)
+        v = make_double2(-(aip.x * rp.x - aip.y * rp.y), -(aip.x * rp.y + aip.y * rp.x));
+      else
+        v = csub(aic, cmul(aip, apc));
+      a[nxt][i][c] = v;
+    }
+    __syncthreads();
+  }
+  // GJ_NB is even: the inverse is in buffer 0
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = r0 + RSTEP * q;
+    D[((int64_t)b * GJ_NB + r) * GJ_NB + c] = a[0][r][c];
+  }
+}
+
 }  // namespace
 
 namespace euvl {
@@ -541,6 +599,61 @@ int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int*
   return EUVL_OK;
 }
 
+// A X = B by block Gauss-Jordan elimination without row interchanges, on the
+// augmented matrix [A | B] (n x (n + nrhs), leading dimension lda >= n + nrhs,
+// batch stride sa): block row k (GJ_NB rows) is multiplied by the inverse of
+// its diagonal block, then eliminated from the rows below (one GEMM with
+// K = GJ_NB over the rest of the matrix and the right-hand sides), and a block
+// back substitution leaves X in columns n .. n + nrhs.  Compared with a
+// factorisation followed by two triangular solves it has one dependent chain
+// of n / GJ_NB short steps (inverse, scaling, update) instead of a 32-wide
+// panel chain per factor and per solve, and every update is one GEMM.  The
+// left n columns are overwritten (no factors are kept).  info[b] receives the
+// first zero pivot of a diagonal block (1-based column), 0 if none.
+int gj_solve(euvl_handle h, int n, int nrhs, double2* A, int64_t lda, int64_t sa, int batch, int* d_info) {
+  PrecScope scope(h, EUVL_PREC_FP32_LU);
+  const double2 one = make_double2(1.0, 0.0), mone = make_double2(-1.0, 0.0), zero = make_double2(0.0, 0.0);
+  if (lda < (int64_t)n + nrhs) {
+    set_error("gj_solve: leading dimension %lld below n + nrhs = %d", (long long)lda, n + nrhs);
+    return EUVL_ERR_ARG;
+  }
+  const size_t b_blk = sizeof(double2) * (size_t)batch * GJ_NB * GJ_NB;
+  double2* dinv = (double2*)aux_reserve(h, b_blk);
+  if (!dinv) {
+    set_error("gj_solve: scratch allocation failed");
+    return EUVL_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(256), 0, h->stream, d_info, batch);
+  EUVL_LAUNCH_CHECK();
+  const int ncols = n + nrhs;
+  for (int kb = 0; kb < n; kb += GJ_NB) {
+    const int nbc = std::min(GJ_NB, n - kb);
+    hipLaunchKernelGGL(gj_inv_kernel, dim3(batch), dim3(GJ_THREADS), 0, h->stream, (const double2*)A, lda, sa, kb,
+                       nbc, dinv, d_info);
+    EUVL_LAUNCH_CHECK();
+    // block row k <- D_k [A_k,(k+1:) | B_k], in place: every tile of the
+    // product spans its nbc <= GJ_NB rows
+    double2* rowk = A + (int64_t)kb * lda + kb + nbc;
+    const int right = ncols - kb - nbc;
+    EUVL_TRY(zgemm(h, nbc, right, nbc, one, dinv, GJ_NB, (int64_t)GJ_NB * GJ_NB, rowk, lda, sa, zero, rowk, lda, sa,
+                   zero, batch));
+    const int below = n - kb - nbc;
+    if (below > 0)
+      EUVL_TRY(zgemm(h, below, right, nbc, mone, A + (int64_t)(kb + nbc) * lda + kb, lda, sa, rowk, lda, sa, one,
+                     rowk + (int64_t)nbc * lda, lda, sa, zero, batch));
+  }
+  // back substitution by columns, last block first: X_k = B'_k is final once
+  // the blocks after it are done, and the rows above subtract U'_(0:k, k) X_k
+  // (one GEMM with K = GJ_NB over all of them, so each step fills the chip)
+  const int last = ((n - 1) / GJ_NB) * GJ_NB;
+  for (int kb = last; kb > 0; kb -= GJ_NB) {
+    const int nbc = std::min(GJ_NB, n - kb);
+    EUVL_TRY(zgemm(h, kb, nrhs, nbc, mone, A + kb, lda, sa, A + (int64_t)kb * lda + n, lda, sa, one, A + n, lda, sa,
+                   zero, batch));
+  }
+  return EUVL_OK;
+}
+
 int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64_t sa, const int* ipiv,
            double2* B, int64_t ldb, int64_t sb, int batch) {
   PrecScope scope(h, EUVL_PREC_FP32_LU);
@@ -615,6 +728,26 @@ extern "C" int euvl_zgetrf_nopiv_batched(euvl_handle h, int n, double* d_A, int6
   for (int b = 0; b < batch; ++b)
     if (hi[b] != 0) {
       euvl::set_error("euvl_zgetrf_nopiv_batched: matrix %d has a zero pivot at column %d", b, hi[b] - 1);
+      return EUVL_ERR_SINGULAR;
+    }
+  return EUVL_OK;
+}
+
+extern "C" int euvl_zgesv_nopiv_batched(euvl_handle h, int n, int nrhs, double* d_A, int64_t lda, int64_t sa,
+                                       int batch) {
+  if (!h || n <= 0 || nrhs <= 0 || !d_A || lda < (int64_t)n + nrhs || batch <= 0 || batch > 256 ||
+      (batch > 1 && sa < (int64_t)n * lda)) {
+    euvl::set_error("euvl_zgesv_nopiv_batched: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  int* info = (int*)h->d_red;  // batch <= 256 ints of the handle's reduction slots
+  EUVL_TRY(euvl::gj_solve(h, n, nrhs, (double2*)d_A, lda, sa, batch, info));
+  int* hi = (int*)h->h_pinned;
+  EUVL_HIP_CHECK(hipMemcpyAsync(hi, info, sizeof(int) * batch, hipMemcpyDeviceToHost, h->stream));
+  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (int b = 0; b < batch; ++b)
+    if (hi[b] != 0) {
+      euvl::set_error("euvl_zgesv_nopiv_batched: matrix %d has a zero pivot at column %d", b, hi[b] - 1);
       return EUVL_ERR_SINGULAR;
     }
   return EUVL_OK;

@@ -138,6 +138,18 @@ int euvl_zgetrf_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t 
 int euvl_zgetrf_nopiv_batched(euvl_handle h, int n, double* d_A, int64_t lda, int64_t sa, int* d_ipiv,
                               int batch);
 
+/* A[b] X = B[b] without row interchanges, on the augmented matrix [A | B]
+ * (n x (n + nrhs), row-major, leading dim lda >= n + nrhs, batch stride sa):
+ * X replaces B in columns n .. n + nrhs - 1, the left n columns are
+ * overwritten.  Block Gauss-Jordan elimination (the inverse of each 64 x 64
+ * diagonal block, GEMM updates with K = 64); for matrices whose elimination
+ * without pivoting is stable (diagonally dominant, or verified by the caller's
+ * residual).  Returns EUVL_ERR_SINGULAR on a zero pivot of a diagonal block.
+ * No reference counterpart as a call: the top solve of ampS (matinv,
+ * cnn/include/header.h:1468-1557 with the products of :1355-1391) uses it on
+ * its reduced system.  batch <= 256. */
+int euvl_zgesv_nopiv_batched(euvl_handle h, int n, int nrhs, double* d_A, int64_t lda, int64_t sa, int batch);
+
 /* Solve A[b] X = B[b] in place of B (n x nrhs, leading dim ldb) using the
  * factors of euvl_zgetrf_batched.  Replaces cusolverDnXgetrs in matinv
  * (cnn/include/header.h:1530-1535). */
