@@ -121,6 +121,32 @@ def test_getrf_nopiv_zero_pivot_reports(lib, handle):
     assert lib.euvl_zgetrf_nopiv_batched(handle, 2, P(A), 2, 4, P(ipiv), 1) == -3
 
 
+@pytest.mark.parametrize("n,nrhs,batch", [(1681, 171, 2), (200, 7, 3), (130, 1, 2), (64, 5, 2), (65, 9, 1),
+                                          (37, 40, 2)])
+def test_gesv_nopiv_block_gauss_jordan(lib, handle, n, nrhs, batch):
+    """[A | B] -> [. | A^-1 B] by block Gauss-Jordan elimination (64-wide
+    diagonal blocks, partial last block, n below one block) against
+    numpy.linalg.solve, on the same well-conditioned operators as the
+    unpivoted LU."""
+    say(f"gesv_nopiv n={n} nrhs={nrhs} batch={batch}")
+    rng = np.random.default_rng(n + 11)
+    A = np.eye(n)[None] + 0.5 * crand(rng, batch, n, n) / np.sqrt(2 * n)
+    Bm = crand(rng, batch, n, nrhs)
+    aug = dev(np.concatenate([A, Bm], axis=2))
+    ld = n + nrhs
+    assert lib.euvl_zgesv_nopiv_batched(handle, n, nrhs, P(aug), ld, n * ld, batch) == 0, err(lib)
+    torch.cuda.synchronize()
+    X = aug.cpu().numpy()[:, :, n:]
+    for b in range(batch):
+        ref = np.linalg.solve(A[b], Bm[b])
+        assert np.abs(X[b] - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_gesv_nopiv_zero_pivot_reports(lib, handle):
+    aug = dev(np.array([[0.0, 1.0, 1.0], [1.0, 0.0, 2.0]], dtype=np.complex128))
+    assert lib.euvl_zgesv_nopiv_batched(handle, 2, 1, P(aug), 3, 6, 1) == -3
+
+
 def test_getrf_singular_reports(lib, handle):
     say("getrf singular")
     A = np.zeros((1, 8, 8), complex)
