@@ -109,7 +109,7 @@ def per_problem(res, cfgs):
 
 
 def measured_traffic(form="fp64"):
-    """HBM bytes per GEMM launch of this workload (form "fp64" or "mixed"),
+    """HBM bytes per GEMM launch of this workload (form "fp64", "mixed" or "fast"),
     measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE, scaled by a
     plain copy of known size) and kept as a summary under profiles/; empty
     when absent."""
@@ -675,7 +675,7 @@ def main():
                                "(x pass on FP32 MFMA, Kahan-compensated source sums); f64 matrix functions, "
                                "multilayer cascade and storage", "mixed")
         mixed["fast_variant"] = form_line(fast, "f32 MFMA for every EM product (matrix functions too) and the "
-                                                "imaging; f64 multilayer and storage", "mixed")
+                                                "imaging; f64 multilayer and storage", "fast")
     tcc_ms = f64["tcc_ms"]
 
     line = {
