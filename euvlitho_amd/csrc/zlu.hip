@@ -3,6 +3,7 @@
 // several workgroups (below); the trailing update is the MFMA GEMM of
 // zgemm.hip.
 #include <algorithm>
+#include <climits>
 
 #include "ew.h"
 
@@ -444,20 +445,31 @@ __global__ __launch_bounds__(256) void trsm_kernel(int nbc, const double2* T, in
 // row takes a(i,c) - a(i,p) s(c), where the pivot row uses a(p,p) - 1 in place
 // of a(i,p): that gives s(c) in the pivot row, 1 / a(p,p) at the pivot and
 // -a(i,p) / a(p,p) in column p, with no branch per element.  Only row p and
-// column p cross threads: their owners publish them into LDS buffers of
-// parity p & 1, so one barrier per step orders them.  After GJ_NB steps the
+// column p cross threads.  The wave holding row p (one wave in RS) reads
+// a(p,p) from its lane p, takes the reciprocal and publishes s; lane p of
+// every wave publishes its elements of column p.  Both go to LDS buffers of
+// parity p & 1, so one barrier per step orders them, and the other waves do
+// nothing per step but their RPT complex updates.  After GJ_NB steps the
 // block holds its inverse.  Outside the nbc x nbc block the identity is used;
-// a zero pivot inside it sets info (1-based column) and 1 / a(p,p) is taken
-// as 0.  The step is issue-bound per wave (FP64 VALU and LDS, one CU per
-// batch entry): 1024 threads (4 elements each) measured 10 % faster than 256
-// or 512 on MI355X.
+// a zero pivot inside it sets info (1-based column, the first one kept) and
+// 1 / a(p,p) is taken as 0.
 constexpr int GJ_NB = 64;
+
+__device__ inline double readlane_f64(double v, int lane) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)bits, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void gj_inv_kernel(const double2* __restrict__ A, int64_t lda, int64_t sa, int kb,
                                                     int nbc, double2* __restrict__ D, int* __restrict__ info) {
-  constexpr int RS = NT / GJ_NB;       // row stride between a thread's elements
+  static_assert(NT % GJ_NB == 0 && GJ_NB == 64, "one row of the block per wave");
+  constexpr int RS = NT / GJ_NB;       // row stride between a thread's elements (= waves)
   constexpr int RPT = GJ_NB / RS;      // elements per thread
   __shared__ double2 rowb[2][GJ_NB], colb[2][GJ_NB];
+  __shared__ int first_zero;  // smallest 1-based column of a zero pivot, INT_MAX if none
   const int b = blockIdx.x, t = threadIdx.x;
   const int c = t & (GJ_NB - 1), r0 = t / GJ_NB;
   const double2* A11 = A + (int64_t)b * sa + (int64_t)kb * lda + kb;
@@ -469,13 +481,24 @@ __global__ __launch_bounds__(NT) void gj_inv_kernel(const double2* __restrict__ 
     x[q] = A11[in ? (int64_t)r * lda + c : 0];
     if (!in) x[q] = make_double2(r == c ? 1.0 : 0.0, 0.0);
   }
+  if (t == 0) first_zero = INT_MAX;
+  __syncthreads();
 #pragma unroll 1
   for (int p = 0; p < GJ_NB; ++p) {
     const int pb = p & 1;
     if ((p % RS) == r0) {  // this wave holds row p, as element p / RS
+      const int qp = p / RS;
+      double2 v = x[0];
 #pragma unroll
-      for (int q = 0; q < RPT; ++q)
-        if (q == p / RS) rowb[pb][c] = x[q];
+      for (int q = 1; q < RPT; ++q)
+        if (q == qp) v = x[q];
+      const double apx = readlane_f64(v.x, p), apy = readlane_f64(v.y, p);
+      const bool zero = apx == 0.0 && apy == 0.0;
+      if (zero && c == 0 && p < nbc) atomicMin(&first_zero, kb + p + 1);
+      // 1 / a(p,p) = conj(a(p,p)) / |a(p,p)|^2: one real division
+      const double inv2 = zero ? 0.0 : 1.0 / (apx * apx + apy * apy);
+      const double2 rp = make_double2(apx * inv2, -apy * inv2);
+      rowb[pb][c] = (c == p) ? make_double2(1.0 + rp.x, rp.y) : cmul(v, rp);
     }
     if (c == p) {
 #pragma unroll
@@ -485,13 +508,7 @@ __global__ __launch_bounds__(NT) void gj_inv_kernel(const double2* __restrict__ 
       }
     }
     __syncthreads();
-    const double2 app = rowb[pb][p];
-    const bool zero = app.x == 0.0 && app.y == 0.0;
-    if (zero && t == 0 && p < nbc && info[b] == 0) info[b] = kb + p + 1;
-    // 1 / a(p,p) = conj(a(p,p)) / |a(p,p)|^2: one real division
-    const double inv2 = zero ? 0.0 : 1.0 / (app.x * app.x + app.y * app.y);
-    const double2 rp = make_double2(app.x * inv2, -app.y * inv2);
-    const double2 sc = (c == p) ? make_double2(1.0 + rp.x, rp.y) : cmul(rowb[pb][c], rp);
+    const double2 sc = rowb[pb][c];
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const double2 aip = colb[pb][r0 + RS * q];
@@ -501,6 +518,7 @@ __global__ __launch_bounds__(NT) void gj_inv_kernel(const double2* __restrict__ 
   }
 #pragma unroll
   for (int q = 0; q < RPT; ++q) D[((int64_t)b * GJ_NB + r0 + RS * q) * GJ_NB + c] = x[q];
+  if (t == 0 && first_zero != INT_MAX && info[b] == 0) info[b] = first_zero;
 }
 
 }  // namespace
