@@ -56,12 +56,13 @@ def copy_calibration(mib=512, reps=3):
     print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps}), flush=True)
 
 
-def run_served(n_req, slots, precision=None):
+def run_served(n_req, slots, precision=None, loads=(0.8,)):
     """Served form: requests of six problems (pitch 256-1024 nm, orders +-8..+-20,
     61-200 source points), mixed precision, `slots` streams with admission by
     device-memory footprint; arrived requests of one problem share a batched
     plan run.  A burst of 24 requests measures the capacity; the Poisson
-    stream is then offered at 80 % of it."""
+    stream is then offered at loads[0] (80 %) of it, and at each further load
+    (reported under "at_load", same seed) for the latency-throughput curve."""
     from euvlitho_amd import _lib
     from euvlitho_amd.serve import Server, poisson_stream, served_configs
     cfgs = served_configs()
@@ -73,7 +74,16 @@ def run_served(n_req, slots, precision=None):
         warm_s = time.perf_counter() - t
         burst = srv.run(poisson_stream(24, 0.0, seed=1, cfgs=cfgs))
         cap = len(burst) / srv.wall
-        rate = 0.8 * cap
+        curve = {}
+        for load in loads[1:]:
+            r2 = srv.run(poisson_stream(n_req, load * cap, seed=2, cfgs=cfgs))
+            l2 = np.array([r.latency for r in r2])
+            curve["%.2f" % load] = {"value": len(r2) / srv.wall, "offered_rate": load * cap,
+                                    "p50_ms": 1000.0 * float(np.percentile(l2, 50)),
+                                    "p99_ms": 1000.0 * float(np.percentile(l2, 99)),
+                                    "p99_over_p50": float(np.percentile(l2, 99) / np.percentile(l2, 50)),
+                                    "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0}
+        rate = loads[0] * cap
         res = srv.run(poisson_stream(n_req, rate, seed=2, cfgs=cfgs))
         lat = np.array([r.latency for r in res])
         svc = np.array([r.finish - r.start for r in res])
@@ -90,7 +100,7 @@ def run_served(n_req, slots, precision=None):
                 "resident_build_s": warm_s, "peak_reserved_gib": srv.peak_reserved / 2**30,
                 "batches": len(srv.batches), "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
                 "adaptive_reruns": srv.reruns, "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
-                "load": 0.8,
+                "load": loads[0], "at_load": curve,
                 "problems": ["%dnm K%d %s %dpts" % (c.ndivx, c.orders[1], "ann" if c.source.kind == 1 else "dip+ann",
                                                        sum(len(x[2]) for x in c.source_points())) for c in cfgs]}
     finally:
@@ -725,7 +735,7 @@ def main():
     if args.batched > 0:
         line["forms"]["batched"] = run_batched(args.batched)
     if args.served > 0:
-        line["forms"]["served"] = run_served(args.served, args.slots)
+        line["forms"]["served"] = run_served(args.served, args.slots, loads=(0.8, 0.6))
         line["forms"]["served_fast"] = run_served(args.served, args.slots, _lib.PREC_FAST)
     if not args.no_mask_producer:
         line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)

@@ -5,7 +5,9 @@
 // TCC_w = sum_c P_c diag(w) P_c^H / S is one MFMA GEMM per weight (the three
 // polarisation components stacked along K); the eigendecomposition is a
 // parallel two-sided Jacobi method (round-robin pairs, n/2 rotations per
-// step applied in one out-of-place launch over 2 x 2 blocks).
+// step applied in one out-of-place launch over 2 x 2 blocks), or, when only
+// a few leading kernels are asked for, block subspace iteration with
+// Rayleigh-Ritz steps by the same Jacobi method on the small projected matrix.
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -431,6 +433,196 @@ __global__ void socs_combine32_kernel(int nout, int n0, int nxy, const float2* f
   small[pix] = (double)acc - (double)cmp;
 }
 
+// ---------------------------------------------------------------- leading eigenpairs
+// When only the leading nk << n kernels are asked for (the SOCS image keeps a
+// few tens of a TCC of several hundred pupil points), block subspace iteration
+// finds them with a few small products per step instead of full Jacobi sweeps
+// over the n x n matrix: Q (n x SUB_NB, orthonormal) <- orth(T Q), and a
+// Rayleigh-Ritz step (the Jacobi method on Q^H T Q, SUB_NB x SUB_NB) gives the
+// Ritz pairs, accepted when every leading residual |T f - theta f| is at
+// rounding level.  The error of the leading nk Ritz vectors falls by
+// |lambda_(SUB_NB+1)| / |lambda_nk| per step.
+constexpr int SUB_NB = 64;      // block width (one gram_orth_kernel workgroup holds SUB_NB x SUB_NB)
+constexpr int SUB_MAX_NK = 24;  // at most this many leading pairs, and n >= 2 SUB_NB
+
+__device__ inline uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// a fixed start block: entries uniform in (-1/2, 1/2) + i (-1/2, 1/2), from a
+// hash of their position (the same block on every call)
+__global__ void start_block_kernel(int64_t total, double2* Y) {
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const double u = (double)(mix64(2 * (uint64_t)e) >> 11) * 0x1.0p-53;
+    const double v = (double)(mix64(2 * (uint64_t)e + 1) >> 11) * 0x1.0p-53;
+    Y[e] = make_double2(u - 0.5, v - 0.5);
+  }
+}
+
+// XH[b][j][i] = conj(X[b][i][j]), X rows x cols
+__global__ void conj_transpose_kernel(int rows, int cols, const double2* X, double2* XH, int batch) {
+  const int64_t per = (int64_t)rows * cols, total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int j = (int)(r / rows), i = (int)(r % rows);  // output index (j, i): writes in order
+    const double2 v = X[(int64_t)b * per + (int64_t)i * cols + j];
+    XH[e] = make_double2(v.x, -v.y);
+  }
+}
+
+// out[b][m][j] = F[b][j][m] (F n x nk)
+__global__ void transpose_kernel(int n, int nk, const double2* F, double2* out, int batch) {
+  const int64_t per = (int64_t)n * nk, total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int m = (int)(r / n), j = (int)(r % n);
+    out[e] = F[(int64_t)b * per + (int64_t)j * nk + m];
+  }
+}
+
+// H <- (H + H^H) / 2 in place (the Jacobi method reads one triangle's pivots)
+__global__ void hermitize_kernel(int n, double2* H, int batch) {
+  const int64_t per = (int64_t)n * n, total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int i = (int)(r / n), j = (int)(r % n);
+    double2* M = H + (int64_t)b * per;
+    if (i < j) {
+      const double2 a = M[(int64_t)i * n + j], c = M[(int64_t)j * n + i];
+      const double2 m = make_double2(0.5 * (a.x + c.x), 0.5 * (a.y - c.y));
+      M[(int64_t)i * n + j] = m;
+      M[(int64_t)j * n + i] = make_double2(m.x, -m.y);
+    } else if (i == j) {
+      M[(int64_t)i * n + i].y = 0.0;
+    }
+  }
+}
+
+// U[b][j][m] = V[b][j][pos[b][m]] (V np x np with its columns in pair order)
+__global__ void ritz_select_kernel(int nb, int nk, const double2* V, const int* pos, double2* U, int batch) {
+  const int64_t per = (int64_t)nb * nk, total = per * batch;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e / per);
+    const int64_t r = e - (int64_t)b * per;
+    const int j = (int)(r / nk), m = (int)(r % nk);
+    U[e] = V[(int64_t)b * nb * nb + (int64_t)j * nb + pos[b * nk + m]];
+  }
+}
+
+// res[b][m] = sum_j |W[b][j][m] - theta[b][m] F[b][j][m]|^2, one workgroup per (m, b)
+__global__ __launch_bounds__(256) void ritz_residual_kernel(int n, int nk, const double2* W, const double2* F,
+                                                            const double* theta, double* res) {
+  const int m = blockIdx.x, b = blockIdx.y;
+  const double th = theta[b * nk + m];
+  const double2* Wb = W + (int64_t)b * n * nk;
+  const double2* Fb = F + (int64_t)b * n * nk;
+  double acc = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const double2 w = Wb[(int64_t)j * nk + m], f = Fb[(int64_t)j * nk + m];
+    const double rx = w.x - th * f.x, ry = w.y - th * f.y;
+    acc += rx * rx + ry * ry;
+  }
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  __shared__ double part[4];
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) res[b * nk + m] = part[0] + part[1] + part[2] + part[3];
+}
+
+// Orthonormalising factor of a block: for the Gram matrix G = Y^H Y
+// (SUB_NB x SUB_NB, Hermitian positive definite) the unpivoted elimination of
+// [G | I] leaves L^-1 on the right (G = L D L^H, L unit lower), and
+// M = L^-H D^-1/2 gives (Y M)^H (Y M) = M^H G M = I.  One workgroup per batch
+// entry; thread (r0, c) keeps rows r0 + q RS of both halves in registers; the
+// pivot row and column cross threads through LDS buffers of parity p & 1 (one
+// barrier per step).  A pivot below 1e-12 of the largest diagonal entry (a
+// block that lost rank) sets fail[b].
+template <int NT>
+__global__ __launch_bounds__(NT) void gram_orth_kernel(const double2* __restrict__ G, double2* __restrict__ M,
+                                                       int* __restrict__ fail) {
+  constexpr int NB = SUB_NB;
+  static_assert(NT % NB == 0 && NB == 64, "one row of the block per wave");
+  constexpr int RS = NT / NB, RPT = NB / RS;
+  __shared__ double2 rowg[2][NB], rowi[2][NB], colg[2][NB];
+  __shared__ double dsh[NB];
+  __shared__ double scale;
+  const int b = blockIdx.x, t = threadIdx.x;
+  const int c = t & (NB - 1), r0 = t / NB;
+  const double2* Gb = G + (int64_t)b * NB * NB;
+  double2 x[RPT], y[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int i = r0 + RS * q;
+    x[q] = Gb[(int64_t)i * NB + c];
+    y[q] = make_double2(i == c ? 1.0 : 0.0, 0.0);
+    if (i == c) dsh[i] = x[q].x;
+  }
+  __syncthreads();
+  if (t == 0) {
+    double m = 0.0;
+    for (int i = 0; i < NB; ++i) m = fmax(m, dsh[i]);
+    scale = m;
+  }
+  __syncthreads();
+  const double tiny = 1e-12 * scale;
+  bool bad = false;
+#pragma unroll 1
+  for (int p = 0; p < NB; ++p) {
+    const int pb = p & 1;
+    if ((p % RS) == r0) {  // this wave holds row p
+      const int qp = p / RS;
+      double2 v = x[0], w = y[0];
+#pragma unroll
+      for (int q = 1; q < RPT; ++q)
+        if (q == qp) {
+          v = x[q];
+          w = y[q];
+        }
+      rowg[pb][c] = v;
+      rowi[pb][c] = w;
+      if (c == p) dsh[p] = v.x;
+    }
+    if (c == p) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) colg[pb][r0 + RS * q] = x[q];
+    }
+    __syncthreads();
+    const double d = dsh[p];
+    bad = bad || !(d > tiny);
+    const double inv = (d > tiny) ? 1.0 / d : 0.0;
+    const double2 rg = rowg[pb][c], ri = rowi[pb][c];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int i = r0 + RS * q;
+      const double2 g = colg[pb][i];
+      const double lx = (i > p) ? g.x * inv : 0.0, ly = (i > p) ? g.y * inv : 0.0;
+      x[q].x -= lx * rg.x - ly * rg.y;
+      x[q].y -= lx * rg.y + ly * rg.x;
+      y[q].x -= lx * ri.x - ly * ri.y;
+      y[q].y -= lx * ri.y + ly * ri.x;
+    }
+  }
+  // M[c'][i] = conj(L^-1[i][c']) / sqrt(d_i)
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int i = r0 + RS * q;
+    const double di = dsh[i];
+    const double sc = (di > tiny) ? 1.0 / sqrt(di) : 0.0;
+    M[(int64_t)b * NB * NB + (int64_t)c * NB + i] = make_double2(y[q].x * sc, -y[q].y * sc);
+  }
+  if (t == 0 && bad) fail[b] = 1;
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0;
+}
+
 std::vector<std::complex<double>> twiddles(int n, double sign) {
   std::vector<std::complex<double>> t(n);
   const std::complex<double> zi(0.0, 1.0);
@@ -440,54 +632,17 @@ std::vector<std::complex<double>> twiddles(int n, double sign) {
 
 }  // namespace
 
-extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
-                                const double* h_src, double kx0, double ky0, double z0, double z, int nk,
-                                double* d_alpha, double* d_phi, int* h_sweeps) {
+namespace {
+
+// Jacobi sweeps on A (np x np in pair order, V accumulating the rotations)
+// until the squared off-diagonal / total Frobenius ratio is at the rounding
+// floor of an n x n matrix, (2 n eps)^2, or stops falling below 1e3 times it;
+// Jacobi converges quadratically, so this takes few sweeps.  A and V (and the
+// spare buffers A2, V2) are swapped as the out-of-place steps go.
+int jacobi_sweeps(euvl_handle h, int n, int np, double2*& A, double2*& V, double2*& A2, double2*& V2, int B,
+                  int* sweeps_out) {
   using namespace euvl;
-  if (!h || !p || ncut <= 1 || !h_pupil || n_src <= 0 || !h_src || nk <= 0 || nk > ncut || !d_alpha || !d_phi) {
-    set_error("euvl_tcc_kernels: bad argument");
-    return EUVL_ERR_ARG;
-  }
-  const int n = ncut, B = 3, S = n_src;
-  // parallel Jacobi over a round-robin tournament of np = n rounded up to even
-  // players, stored in pair order (see pair_next)
-  const int np = n + (n & 1), npair = np / 2, nsteps = np - 1;
-  Upload up;
-  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * n);
-  const size_t o_src = up.add(h_src, sizeof(double) * 2 * S);
-  const size_t mat = sizeof(double2) * (size_t)n * n * B;
-  const size_t matp = sizeof(double2) * (size_t)np * np * B;
-  const size_t pb = sizeof(double2) * (size_t)n * 3 * S;
-  EUVL_TRY(ws_reserve(h, up.host.size() + mat + 4 * matp + 4 * pb + sizeof(double) * np * B +
-                             sizeof(int) * nk * B + 8192));
-  char* blk = (char*)ws_take(h, up.host.size());
-  double2* T0 = (double2*)ws_take(h, mat);
-  double2* A = (double2*)ws_take(h, matp);
-  double2* V = (double2*)ws_take(h, matp);
-  double2* A2 = (double2*)ws_take(h, matp);
-  double2* V2 = (double2*)ws_take(h, matp);
-  double2* P = (double2*)ws_take(h, pb);
-  double2* Q = (double2*)ws_take(h, 3 * pb);
-  double* w = (double*)ws_take(h, sizeof(double) * np * B);
-  int* order_d = (int*)ws_take(h, sizeof(int) * nk * B);
-  if (!T0 || !A || !V || !A2 || !V2 || !P || !Q || !w || !order_d) {
-    set_error("euvl_tcc_kernels: workspace carve failed");
-    return EUVL_ERR_NOMEM;
-  }
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
-  TccSetup a{p->k, p->dx, p->dy, p->na, kx0, ky0, z0, z, p->mx, p->my};
-  hipLaunchKernelGGL(tcc_phase_kernel, grid1((int64_t)n * S), dim3(256), 0, h->stream, a, (const int*)(blk + o_pup),
-                     n, (const double*)(blk + o_src), S, P, Q);
-  EUVL_LAUNCH_CHECK();
-  // TCC_w = P Q_w  (n x 3S x n), the three weights as a batch sharing P
-  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  EUVL_TRY(zgemm(h, n, n, 3 * S, one, P, 3 * S, 0, Q, n, (int64_t)3 * S * n, zero, T0, n, (int64_t)n * n, zero, B));
-  hipLaunchKernelGGL(pair_order_init_kernel, grid1((int64_t)np * np * B), dim3(256), 0, h->stream, n, np, T0, A, V,
-                     B);
-  EUVL_LAUNCH_CHECK();
-  // Stop once the squared off-diagonal / total Frobenius ratio is at the
-  // rounding floor of an n x n matrix, (2 n eps)^2, or stops falling below
-  // 1e3 times it; Jacobi converges quadratically, so this takes few sweeps.
+  const int npair = np / 2, nsteps = np - 1;
   const double eps = 2.220446049250313e-16;
   const double floor2 = (2.0 * n * eps) * (2.0 * n * eps);
   const int64_t step_threads = ((int64_t)npair * npair + (int64_t)np * npair) * B;
@@ -515,42 +670,254 @@ extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut
     }
     EUVL_LAUNCH_CHECK();
   }
-  if (h_sweeps) *h_sweeps = sweep;
+  if (sweeps_out) *sweeps_out = sweep;
   if (!(worst <= 1e3 * floor2)) {
     set_error("euvl_tcc_kernels: Jacobi off-diagonal ratio %.3e after %d sweeps (floor %.3e)", worst, sweep, floor2);
     return EUVL_ERR_NOT_CONVERGED;
   }
-  // eigenvalues (the diagonal, one per position of the pair order; a sweep ends
-  // in the initial order), stable sort by |alpha| descending (the reference's
-  // bubble sort order), top nk; the padding position (odd n) is left out
+  return EUVL_OK;
+}
+
+// The eigenvalues of a converged Jacobi run (the diagonal, one per position of
+// the pair order; a sweep ends in the initial order) read back to the host,
+// and for each batch entry the positions of the n eigenvalues in output order:
+// |alpha| descending, ties in ascending alpha (as after the reference's
+// ascending eigh and its stable bubble sort), then ascending index.  The
+// padding position (odd n) is left out.
+int jacobi_order(euvl_handle h, int n, int np, const double2* A, double* w, int B, std::vector<double>& hw,
+                 std::vector<std::vector<int>>& pos) {
+  using namespace euvl;
   hipLaunchKernelGGL(diag_kernel, grid1((int64_t)np * B), dim3(256), 0, h->stream, np, A, w, B);
   EUVL_LAUNCH_CHECK();
-  std::vector<double> hw((size_t)np * B);
+  hw.assign((size_t)np * B, 0.0);
   EUVL_HIP_CHECK(hipMemcpyAsync(hw.data(), w, sizeof(double) * np * B, hipMemcpyDeviceToHost, h->stream));
   EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  std::vector<int> order((size_t)nk * B);
-  std::vector<double> halpha((size_t)nk * B);
+  pos.assign(B, {});
   for (int b = 0; b < B; ++b) {
+    std::vector<int>& idx = pos[b];
     // positions sorted by the index they hold, so that ties keep ascending index order
-    std::vector<int> idx;
     for (int q = 0; q < n; ++q) {
-      for (int pos = 0; pos < np; ++pos)
-        if (pair_index0(pos, np) == q) {
-          idx.push_back(pos);
+      for (int ps = 0; ps < np; ++ps)
+        if (pair_index0(ps, np) == q) {
+          idx.push_back(ps);
           break;
         }
     }
-    // ties keep ascending eigenvalue order, as after the reference's ascending eigh
-    std::stable_sort(idx.begin(), idx.end(),
-                     [&](int x, int y) { return hw[(size_t)b * np + x] < hw[(size_t)b * np + y]; });
-    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-      return std::fabs(hw[(size_t)b * np + x]) > std::fabs(hw[(size_t)b * np + y]);
-    });
-    for (int m = 0; m < nk; ++m) {
-      order[(size_t)b * nk + m] = idx[m];
-      halpha[(size_t)b * nk + m] = hw[(size_t)b * np + idx[m]];
-    }
+    const double* hb = hw.data() + (size_t)b * np;
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return hb[x] < hb[y]; });
+    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return std::fabs(hb[x]) > std::fabs(hb[y]); });
   }
+  return EUVL_OK;
+}
+
+constexpr int SUB_DECLINE = 1;  // leading_by_subspace: not accepted, use the full decomposition
+
+size_t subspace_bytes(int n, int B, int nk) {
+  const size_t nb = SUB_NB;
+  return sizeof(double2) * B * (3 * (size_t)n * nb + 7 * nb * nb + 2 * (size_t)n * nk + nb * nk) +
+         sizeof(double) * B * (nb + nk) + sizeof(int) * B * (nk + 1) + 16 * 256;
+}
+
+// The leading nk eigenpairs of the B Hermitian n x n matrices T by block
+// subspace iteration with Rayleigh-Ritz steps (see SUB_NB).  The first
+// Rayleigh-Ritz step comes after 8 iterations; from its residuals and the
+// ratio |theta_SUB_NB| / |theta_nk| of its Ritz values the iterations still
+// needed for the tolerance are predicted (plus 2), and the next step checks.
+// Accepted when every leading Ritz pair has |T f - theta f| <= 1e-13 |theta_1|;
+// SUB_DECLINE when a block loses rank, the ratio is above 0.8, or four steps
+// have not reached the tolerance.  Writes d_alpha and d_phi as the full method.
+int leading_by_subspace(euvl_handle h, int n, int B, const double2* T, int nk, double* d_alpha, double* d_phi,
+                        int* iters) {
+  using namespace euvl;
+  const int nb = SUB_NB;
+  const size_t blk = sizeof(double2) * (size_t)n * nb * B, sq = sizeof(double2) * (size_t)nb * nb * B;
+  const size_t fk = sizeof(double2) * (size_t)n * nk * B;
+  double2* Y = (double2*)ws_take(h, blk);
+  double2* Q = (double2*)ws_take(h, blk);
+  double2* YH = (double2*)ws_take(h, blk);
+  double2* G = (double2*)ws_take(h, sq);
+  double2* Mf = (double2*)ws_take(h, sq);
+  double2* H = (double2*)ws_take(h, sq);
+  double2* A = (double2*)ws_take(h, sq);
+  double2* V = (double2*)ws_take(h, sq);
+  double2* A2 = (double2*)ws_take(h, sq);
+  double2* V2 = (double2*)ws_take(h, sq);
+  double2* F = (double2*)ws_take(h, fk);
+  double2* W = (double2*)ws_take(h, fk);
+  double2* U = (double2*)ws_take(h, sizeof(double2) * (size_t)nb * nk * B);
+  double* w = (double*)ws_take(h, sizeof(double) * nb * B);
+  double* res = (double*)ws_take(h, sizeof(double) * nk * B);
+  int* pos_d = (int*)ws_take(h, sizeof(int) * nk * B);
+  int* fail = (int*)ws_take(h, sizeof(int) * B);
+  if (!Y || !Q || !YH || !G || !Mf || !H || !A || !V || !A2 || !V2 || !F || !W || !U || !w || !res || !pos_d || !fail) {
+    set_error("euvl_tcc_kernels: workspace carve failed");
+    return EUVL_ERR_NOMEM;
+  }
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  const int64_t snb = (int64_t)n * nb, snn = (int64_t)n * n, sbb = (int64_t)nb * nb;
+  // Qout = Yin M with M from the Gram matrix of Yin (Qout != Yin)
+  auto orth = [&](const double2* Yin, double2* Qout) -> int {
+    hipLaunchKernelGGL(conj_transpose_kernel, grid1(snb * B), dim3(256), 0, h->stream, n, nb, Yin, YH, B);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, nb, nb, n, one, YH, n, snb, Yin, nb, snb, zero, G, nb, sbb, zero, B));
+    hipLaunchKernelGGL(gram_orth_kernel<1024>, dim3(B), dim3(1024), 0, h->stream, (const double2*)G, Mf, fail);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, n, nb, nb, one, Yin, nb, snb, Mf, nb, sbb, zero, Qout, nb, snb, zero, B));
+    return EUVL_OK;
+  };
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, h->stream, fail, B);
+  EUVL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(start_block_kernel, grid1(snb * B), dim3(256), 0, h->stream, snb * B, Y);
+  EUVL_LAUNCH_CHECK();
+  EUVL_TRY(orth(Y, Q));
+  const double tol = 1e-13;
+  int it = 0, next = 8;
+  for (int rr = 0;;) {
+    EUVL_TRY(zgemm(h, n, nb, n, one, T, n, snn, Q, nb, snb, zero, Y, nb, snb, zero, B));
+    EUVL_TRY(orth(Y, Q));
+    if (++it < next) continue;
+    // Rayleigh-Ritz on a block orthonormalised twice (to rounding)
+    EUVL_TRY(orth(Q, Y));
+    std::swap(Q, Y);
+    EUVL_TRY(zgemm(h, n, nb, n, one, T, n, snn, Q, nb, snb, zero, Y, nb, snb, zero, B));
+    hipLaunchKernelGGL(conj_transpose_kernel, grid1(snb * B), dim3(256), 0, h->stream, n, nb, (const double2*)Q, YH,
+                       B);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, nb, nb, n, one, YH, n, snb, Y, nb, snb, zero, H, nb, sbb, zero, B));
+    hipLaunchKernelGGL(hermitize_kernel, grid1(sbb * B), dim3(256), 0, h->stream, nb, H, B);
+    EUVL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(pair_order_init_kernel, grid1(sbb * B), dim3(256), 0, h->stream, nb, nb, (const double2*)H, A,
+                       V, B);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(jacobi_sweeps(h, nb, nb, A, V, A2, V2, B, nullptr));
+    std::vector<double> hw;
+    std::vector<std::vector<int>> pos;
+    EUVL_TRY(jacobi_order(h, nb, nb, A, w, B, hw, pos));
+    std::vector<int> sel((size_t)nk * B);
+    std::vector<double> theta((size_t)nk * B);
+    double rate = 0.0;
+    for (int b = 0; b < B; ++b) {
+      for (int m = 0; m < nk; ++m) {
+        sel[(size_t)b * nk + m] = pos[b][m];
+        theta[(size_t)b * nk + m] = hw[(size_t)b * nb + pos[b][m]];
+      }
+      const double lead = std::fabs(hw[(size_t)b * nb + pos[b][nk - 1]]);
+      const double last = std::fabs(hw[(size_t)b * nb + pos[b][nb - 1]]);
+      rate = std::max(rate, lead > 0.0 ? last / lead : INFINITY);
+    }
+    EUVL_HIP_CHECK(hipMemcpyAsync(pos_d, sel.data(), sizeof(int) * nk * B, hipMemcpyHostToDevice, h->stream));
+    EUVL_HIP_CHECK(hipMemcpyAsync(d_alpha, theta.data(), sizeof(double) * nk * B, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(ritz_select_kernel, grid1((int64_t)nb * nk * B), dim3(256), 0, h->stream, nb, nk,
+                       (const double2*)V, (const int*)pos_d, U, B);
+    EUVL_LAUNCH_CHECK();
+    EUVL_TRY(zgemm(h, n, nk, nb, one, Q, nb, snb, U, nk, (int64_t)nb * nk, zero, F, nk, (int64_t)n * nk, zero, B));
+    EUVL_TRY(zgemm(h, n, nk, n, one, T, n, snn, F, nk, (int64_t)n * nk, zero, W, nk, (int64_t)n * nk, zero, B));
+    hipLaunchKernelGGL(ritz_residual_kernel, dim3(nk, B), dim3(256), 0, h->stream, n, nk, (const double2*)W,
+                       (const double2*)F, (const double*)d_alpha, res);
+    EUVL_LAUNCH_CHECK();
+    EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, res, sizeof(double) * nk * B, hipMemcpyDeviceToHost, h->stream));
+    int* hfail = (int*)(h->h_pinned + nk * B);
+    EUVL_HIP_CHECK(hipMemcpyAsync(hfail, fail, sizeof(int) * B, hipMemcpyDeviceToHost, h->stream));
+    EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));  // sel, theta are read by the copies above until here
+    double worst = 0.0;
+    bool lost = false;
+    for (int b = 0; b < B; ++b) {
+      lost = lost || hfail[b] != 0;
+      const double t0 = std::fabs(theta[(size_t)b * nk]);
+      for (int m = 0; m < nk; ++m) {
+        const double r = std::sqrt(h->h_pinned[(size_t)b * nk + m]) / t0;
+        worst = std::max(worst, std::isnan(r) ? INFINITY : r);
+      }
+    }
+    if (lost) return SUB_DECLINE;
+    if (worst <= tol) {
+      hipLaunchKernelGGL(transpose_kernel, grid1((int64_t)n * nk * B), dim3(256), 0, h->stream, n, nk,
+                         (const double2*)F, (double2*)d_phi, B);
+      EUVL_LAUNCH_CHECK();
+      if (iters) *iters = it;
+      return EUVL_OK;
+    }
+    if (++rr >= 4 || !(rate < 0.8)) return SUB_DECLINE;
+    next = it + std::max(2, (int)std::ceil(std::log(tol / worst) / std::log(rate)) + 2);
+    if (next > 160) return SUB_DECLINE;
+  }
+}
+
+}  // namespace
+
+extern "C" int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
+                                const double* h_src, double kx0, double ky0, double z0, double z, int nk,
+                                double* d_alpha, double* d_phi, int* h_sweeps) {
+  using namespace euvl;
+  if (!h || !p || ncut <= 1 || !h_pupil || n_src <= 0 || !h_src || nk <= 0 || nk > ncut || !d_alpha || !d_phi) {
+    set_error("euvl_tcc_kernels: bad argument");
+    return EUVL_ERR_ARG;
+  }
+  const int n = ncut, B = 3, S = n_src;
+  // parallel Jacobi over a round-robin tournament of np = n rounded up to even
+  // players, stored in pair order (see pair_next)
+  const int np = n + (n & 1);
+  Upload up;
+  const size_t o_pup = up.add(h_pupil, sizeof(int) * 2 * n);
+  const size_t o_src = up.add(h_src, sizeof(double) * 2 * S);
+  const size_t mat = sizeof(double2) * (size_t)n * n * B;
+  const size_t matp = sizeof(double2) * (size_t)np * np * B;
+  const size_t pb = sizeof(double2) * (size_t)n * 3 * S;
+  const bool sub = nk <= SUB_MAX_NK && n >= 2 * SUB_NB;
+  EUVL_TRY(ws_reserve(h, up.host.size() + mat + 4 * matp + 4 * pb + sizeof(double) * np * B +
+                             sizeof(int) * nk * B + 8192 + (sub ? subspace_bytes(n, B, nk) : 0)));
+  char* blk = (char*)ws_take(h, up.host.size());
+  double2* T0 = (double2*)ws_take(h, mat);
+  double2* P = (double2*)ws_take(h, pb);
+  double2* Q = (double2*)ws_take(h, 3 * pb);
+  if (!blk || !T0 || !P || !Q) {
+    set_error("euvl_tcc_kernels: workspace carve failed");
+    return EUVL_ERR_NOMEM;
+  }
+  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  TccSetup a{p->k, p->dx, p->dy, p->na, kx0, ky0, z0, z, p->mx, p->my};
+  hipLaunchKernelGGL(tcc_phase_kernel, grid1((int64_t)n * S), dim3(256), 0, h->stream, a, (const int*)(blk + o_pup),
+                     n, (const double*)(blk + o_src), S, P, Q);
+  EUVL_LAUNCH_CHECK();
+  // TCC_w = P Q_w  (n x 3S x n), the three weights as a batch sharing P
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  EUVL_TRY(zgemm(h, n, n, 3 * S, one, P, 3 * S, 0, Q, n, (int64_t)3 * S * n, zero, T0, n, (int64_t)n * n, zero, B));
+  if (sub) {
+    int iters = 0;
+    const int rc = leading_by_subspace(h, n, B, T0, nk, d_alpha, d_phi, &iters);
+    if (rc == EUVL_OK) {
+      if (h_sweeps) *h_sweeps = -iters;
+      EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));  // the upload block is read until here
+      return EUVL_OK;
+    }
+    if (rc != SUB_DECLINE) return rc;
+  }
+  double2* A = (double2*)ws_take(h, matp);
+  double2* V = (double2*)ws_take(h, matp);
+  double2* A2 = (double2*)ws_take(h, matp);
+  double2* V2 = (double2*)ws_take(h, matp);
+  double* w = (double*)ws_take(h, sizeof(double) * np * B);
+  int* order_d = (int*)ws_take(h, sizeof(int) * nk * B);
+  if (!A || !V || !A2 || !V2 || !w || !order_d) {
+    set_error("euvl_tcc_kernels: workspace carve failed");
+    return EUVL_ERR_NOMEM;
+  }
+  hipLaunchKernelGGL(pair_order_init_kernel, grid1((int64_t)np * np * B), dim3(256), 0, h->stream, n, np, T0, A, V,
+                     B);
+  EUVL_LAUNCH_CHECK();
+  EUVL_TRY(jacobi_sweeps(h, n, np, A, V, A2, V2, B, h_sweeps));
+  // eigenvalues in output order (|alpha| descending, the reference's tie
+  // order), the leading nk of each set
+  std::vector<double> hw;
+  std::vector<std::vector<int>> pos;
+  EUVL_TRY(jacobi_order(h, n, np, A, w, B, hw, pos));
+  std::vector<int> order((size_t)nk * B);
+  std::vector<double> halpha((size_t)nk * B);
+  for (int b = 0; b < B; ++b)
+    for (int m = 0; m < nk; ++m) {
+      order[(size_t)b * nk + m] = pos[b][m];
+      halpha[(size_t)b * nk + m] = hw[(size_t)b * np + pos[b][m]];
+    }
   EUVL_HIP_CHECK(hipMemcpyAsync(order_d, order.data(), sizeof(int) * nk * B, hipMemcpyHostToDevice, h->stream));
   EUVL_HIP_CHECK(hipMemcpyAsync(d_alpha, halpha.data(), sizeof(double) * nk * B, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(gather_kernel, grid1((int64_t)nk * n * B), dim3(256), 0, h->stream, n, np, nk, V, order_d,

@@ -314,7 +314,9 @@ class Engine:
 
     def eigenfunction(self, z=0.0, nk=100, mesh_deg=0.2):
         """STCC kernels (eigenfunction.py:5-179) kept on the device:
-        (alpha [3, nk], phi [3, nk, ncut]); cached per (z, nk, mesh)."""
+        (alpha [3, nk], phi [3, nk, ncut]); cached per (z, nk, mesh).
+        last_sweeps: the Jacobi sweeps of the full decomposition, or minus the
+        subspace iterations when the leading pairs came from those (nk <= 24)."""
         key = ("kernels", self.cfg, float(z), int(nk), float(mesh_deg))
         got = self.resident.get(key)
         if got is not None:

@@ -293,7 +293,12 @@ int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const double* d_f
  * mesh (w = k^2/(k^2-(kx0+sx)^2) times 1, sx, sy), Hermitian eigenpairs by a
  * parallel Jacobi method, sorted by |alpha| descending; the leading nk of
  * each: d_alpha (3 x nk doubles), d_phi (3 x nk x ncut complex, kernel m of
- * set b at [b][m][:]).  *h_sweeps receives the Jacobi sweep count.
+ * set b at [b][m][:]).  For nk <= 24 and ncut >= 128 the leading pairs come
+ * from block subspace iteration (64 columns, Rayleigh-Ritz by the same
+ * Jacobi method on 64 x 64) once every leading residual |T f - alpha f| is
+ * below 1e-13 |alpha_1|; otherwise, or when that does not converge, from the
+ * full decomposition.  *h_sweeps receives the Jacobi sweep count of the full
+ * decomposition, or minus the subspace iterations.
  * Replaces eigenfunction() (cnn/intensity/stccsocs/eigenfunction.py:5-179)
  * and its cupy.linalg.eigh calls (:130-132). */
 int euvl_tcc_kernels(euvl_handle h, const euvl_em_params* p, int ncut, const int* h_pupil, int n_src,
