@@ -592,7 +592,7 @@ size_t solve_workspace_bytes(int N, int E, int ninc) {
   const size_t vec = sizeof(double2) * (size_t)N * E;
   const size_t rhs = sizeof(double2) * (size_t)N * std::max(ninc, 1) * E;
   return 13 * mat + 8 * vec + 3 * rhs + sizeof(int) * (size_t)N * E + sizeof(double) * 8 * (size_t)E +
-         sizeof(int) * (size_t)E + 64 * 1024;
+         sizeof(int) * (size_t)E + coupling_fft_bytes(E) + 64 * 1024;
 }
 
 // The EM solve of E = n_clips x n_off batch entries (ampS, header.h:1208-1466)
@@ -676,7 +676,20 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   double2* V = Mx[5];  // [V | E] with leading dimension N + ninc
   hipLaunchKernelGGL(top_hv_kernel, grid1(sm * B), dim3(256), 0, h->stream, N, B, k, C, Fs, G2c, H, V, ldv, sv, v);
   EUVL_LAUNCH_CHECK();
-  EUVL_TRY(zgemm(h, N, N, N, MONE, H, N, sm, G, N, sm, ONE, V, ldv, sv, ZERO, B));          // V = P + H - H G
+  // V = P + H - H G: by FFTs over the order grid when the grid fits a
+  // workgroup's LDS (G is sigma^ Toeplitz in the order differences), else a GEMM
+  FftGrid fg;
+  if (fft_grid_for(lmax, mmax, fg)) {
+    double2* Ssp = (double2*)ws_take(h, coupling_fft_bytes(B));
+    if (!Ssp) {
+      set_error("euvl_amp_s: workspace carve failed");
+      return EUVL_ERR_NOMEM;
+    }
+    EUVL_TRY(coupling_fft(h, fg, N, B, t.lidx, t.midx, lmax, mmax, spec, per_clip, spec_stride,
+                          make_double2(eai.real(), eai.imag()), v.kt, k, H, V, ldv, sv, Ssp));
+  } else {
+    EUVL_TRY(zgemm(h, N, N, N, MONE, H, N, sm, G, N, sm, ONE, V, ldv, sv, ZERO, B));
+  }
   double2* Vc = Mx[0];    // D is no longer needed: a copy of V for the residual check
   double2* Bres = Mx[6];  // nor H: the right-hand sides kept for the check
   const int64_t nsxy = (int64_t)nsx * nsy;

@@ -90,6 +90,22 @@ size_t solve_tables_bytes(const euvl_em_params* p, int N, const SolveHost& t);
 int upload_solve_tables(euvl_handle h, const euvl_em_params* p, const int* lindex, const int* mindex, int N,
                         const SolveHost& t, Upload& up, char* blk, SolveDev& d);
 size_t solve_workspace_bytes(int N, int E, int ninc);
+// The coupling product V -= H G of the top solve by 2D FFTs over the order grid
+// (fft.hip).  FftGrid: points along l and m (no prime factor above 3, at least
+// 4 lmax + 1 and 4 mmax + 1) and the radices of their Stockham passes;
+// fft_grid_for is false when the grid does not fit the workgroup's LDS (the
+// product then stays a GEMM).  S: coupling_fft_bytes(E) of scratch.
+constexpr int FFT_MAX_STAGES = 8;
+struct FftGrid {
+  int P[2];
+  int nst[2];
+  int rad[2][FFT_MAX_STAGES];
+};
+bool fft_grid_for(int lmax, int mmax, FftGrid& g);
+size_t coupling_fft_bytes(int E);
+int coupling_fft(euvl_handle h, const FftGrid& g, int N, int E, const int* lidx, const int* midx, int lmax, int mmax,
+                 const double2* W, int per_clip, int64_t w_stride, double2 ea_inv, const double* kt, double k,
+                 const double2* H, double2* V, int64_t ldv, int64_t sv, double2* S);
 int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDev& t, int N, int lmax, int mmax,
                const double2* spec, int per_clip, int64_t spec_stride, int E, double2* Ax, SolveCtl& ctl);
 // indicator spectrum of n_clips packed masks on a window |l| <= lw, |m| <= mw, with the

@@ -1,0 +1,283 @@
+// The coupling product H G of the top solve by 2D FFTs in LDS.
+//
+// G = diag(kt) sigma^ diag(kt) / k^2, where sigma^(dl, dm) is the Fourier
+// coefficient of the inverse permittivity at the order difference (the
+// reference's Toeplitz block, header.h:979-1052).  Row i of H G is therefore
+//   (H G)[i][j] = kt_j / k^2 sum_q h_i(q) sigma^(l_q - l_j, m_q - m_j),
+//   h_i(q) = H[i][q] kt_q,
+// a 2D cross-correlation of h_i with sigma^ over the (l, m) grid of the
+// orders.  On a periodic Pl x Pm grid with Pl >= 4 lmax + 1 and
+// Pm >= 4 mmax + 1 the window |dl| <= 2 lmax, |dm| <= 2 mmax of sigma^ wraps
+// onto distinct points, so the circular correlation equals the linear one at
+// every order:
+//   c = IDFT(DFT(h) S) / (Pl Pm),  S(k) = sum_z sigma^(z) e^{+2 pi i k.z / P}.
+// One workgroup per row of H transforms its grid in LDS (about 15 GFLOP and
+// no HBM traffic beyond reading H and updating V, against 8 N^3 flops per
+// batch entry for the product), and subtracts the row of H G from V.
+//
+// The 1D transforms are mixed-radix Stockham passes (radices 9, 4, 2 and 3;
+// grid sizes with no other prime factor), in place through registers: every
+// thread reads the inputs of its butterflies, a barrier, then it writes the
+// outputs.  Along m only the 2 lmax + 1 rows that hold orders are transformed
+// (the others are zero on the way in and not read on the way out).
+#include <algorithm>
+
+#include "ew.h"
+
+namespace {
+
+constexpr int FFT_MAX_PTS = 8192;  // Pl Pm: the grid in LDS (128 KiB)
+constexpr int FFT_MAX_P = 128;     // points along one axis
+constexpr int FFT_THREADS = 1024;
+
+__device__ __forceinline__ double2 root_of(const double2* root, int t, bool inv) {
+  const double2 w = root[t];
+  return inv ? make_double2(w.x, -w.y) : w;
+}
+
+// a / b for 0 <= a < 2^13 and 0 < b <= 2^7 from rb = 1/b in FP32: (a + 1/2) / b
+// is at least 1/(2b) from an integer and the product's error is below 2^-11,
+// so the truncation is exact (and far cheaper than an integer division)
+__device__ __forceinline__ int fdiv(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
+
+// y = the R-point DFT of x, e^(-+2 pi i r k / R), R = 2, 3, 4, 9, with its
+// constant roots (9 as 3 x 3: DFTs of the three stride-3 subsequences,
+// twiddles e^(-+2 pi i r1 k1 / 9), DFTs across them)
+template <int R>
+__device__ __forceinline__ void dft_small(const double2* x, double2* y, bool inv) {
+  if constexpr (R == 9) {
+    // cos and sin of 2 pi t / 9 for t = 1, 2, 4
+    constexpr double c1 = 0.766044443118978, s1 = 0.6427876096865393;
+    constexpr double c2 = 0.17364817766693041, s2 = 0.984807753012208;
+    constexpr double c4 = -0.9396926207859083, s4 = 0.3420201433256689;
+    const double sg = inv ? 1.0 : -1.0;
+    double2 a[3][3];
+#pragma unroll
+    for (int r1 = 0; r1 < 3; ++r1) {
+      const double2 in[3] = {x[r1], x[r1 + 3], x[r1 + 6]};
+      dft_small<3>(in, a[r1], inv);
+    }
+    a[1][1] = cmul(a[1][1], make_double2(c1, sg * s1));
+    a[1][2] = cmul(a[1][2], make_double2(c2, sg * s2));
+    a[2][1] = cmul(a[2][1], make_double2(c2, sg * s2));
+    a[2][2] = cmul(a[2][2], make_double2(c4, sg * s4));
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+      const double2 in[3] = {a[0][k1], a[1][k1], a[2][k1]};
+      double2 out[3];
+      dft_small<3>(in, out, inv);
+#pragma unroll
+      for (int k2 = 0; k2 < 3; ++k2) y[k1 + 3 * k2] = out[k2];
+    }
+  } else if constexpr (R == 2) {
+    y[0] = cadd(x[0], x[1]);
+    y[1] = csub(x[0], x[1]);
+  } else if constexpr (R == 3) {
+    constexpr double h = 0.86602540378443864676;  // sin(2 pi / 3)
+    const double2 s = cadd(x[1], x[2]), d = csub(x[1], x[2]);
+    const double2 t = make_double2(x[0].x - 0.5 * s.x, x[0].y - 0.5 * s.y);
+    // -i h d (forward), +i h d (inverse)
+    const double2 e = inv ? make_double2(-h * d.y, h * d.x) : make_double2(h * d.y, -h * d.x);
+    y[0] = cadd(x[0], s);
+    y[1] = cadd(t, e);
+    y[2] = csub(t, e);
+  } else {
+    const double2 a = cadd(x[0], x[2]), b = csub(x[0], x[2]);
+    const double2 c = cadd(x[1], x[3]), d = csub(x[1], x[3]);
+    // -i d (forward), +i d (inverse)
+    const double2 e = inv ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+    y[0] = cadd(a, c);
+    y[1] = cadd(b, e);
+    y[2] = csub(a, c);
+    y[3] = csub(b, e);
+  }
+}
+
+// One Stockham pass of radix R over `nlines` lines of the grid X (Pl x Pm,
+// row-major).  Along m a line is a row (the kept rows when keep > 0: rows
+// 0 .. keep/2 and Pl - keep/2 .. Pl - 1), along l a column.  Butterfly j of
+// a line reads positions j + r P/R, multiplies input r by w^(r (j mod Ns)),
+// w = e^(-+2 pi i / (Ns R)), takes the R-point DFT and writes output r to
+// (j / Ns) Ns R + (j mod Ns) + r Ns (Ns: the product of the radices before).
+// (Out of line: inlined into the four axis loops of a kernel it spills.)
+template <int R>
+__device__ __attribute__((noinline)) void fft_pass(double2* X, const double2* root, int P, int Ns, bool inv,
+                                                   bool along_m, int nlines, int Pl, int Pm, int keep) {
+  constexpr int MAXB = (FFT_MAX_PTS / R + FFT_THREADS - 1) / FFT_THREADS;
+  const int per = P / R, total = nlines * per, stride = along_m ? 1 : Pm;
+  const int step = P / (Ns * R);
+  const float rper = 1.0f / (float)per, rlines = 1.0f / (float)nlines, rns = 1.0f / (float)Ns;
+  double2 v[MAXB][R];
+  int dst[MAXB];
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u) {
+    const int beta = threadIdx.x + u * FFT_THREADS;
+    dst[u] = -1;
+    if (beta < total) {
+      int line, j;
+      if (along_m) {  // consecutive threads along a row
+        line = fdiv(beta, rper);
+        j = beta - line * per;
+      } else {  // consecutive threads across columns
+        j = fdiv(beta, rlines);
+        line = beta - j * nlines;
+      }
+      const int base = along_m ? ((keep > 0 && line > keep / 2) ? Pl - keep + line : line) * Pm : line;
+      const int jq = fdiv(j, rns), jn = j - jq * Ns;
+      double2 x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double2 e = X[base + (j + r * per) * stride];
+        x[r] = r ? cmul(e, root_of(root, r * jn * step, inv)) : e;  // r jn step < P
+      }
+      dft_small<R>(x, v[u], inv);
+      dst[u] = base + (jq * Ns * R + jn) * stride;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < MAXB; ++u)
+    if (dst[u] >= 0)
+#pragma unroll
+      for (int r = 0; r < R; ++r) X[dst[u] + r * Ns * stride] = v[u][r];
+  __syncthreads();
+}
+
+// all passes along one axis
+__device__ void fft_axis(double2* X, const double2* root, const euvl::FftGrid& g, int axis, bool inv, int nlines,
+                         int keep) {
+  const bool along_m = axis == 1;
+  const int P = g.P[axis];
+  int Ns = 1;
+  for (int s = 0; s < g.nst[axis]; ++s) {
+    const int R = g.rad[axis][s];
+    switch (R) {
+      case 2: fft_pass<2>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
+      case 3: fft_pass<3>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
+      case 9: fft_pass<9>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
+      default: fft_pass<4>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
+    }
+    Ns *= R;
+  }
+}
+
+// root tables e^(-2 pi i t / P) of both axes
+__device__ void fill_roots(const euvl::FftGrid& g, double2 (*root)[FFT_MAX_P]) {
+  for (int t = threadIdx.x; t < g.P[0] + g.P[1]; t += blockDim.x) {
+    const int a = t < g.P[0] ? 0 : 1, u = t - (a ? g.P[0] : 0);
+    double s, c;
+    sincospi(-2.0 * u / g.P[a], &s, &c);
+    root[a][u] = make_double2(c, s);
+  }
+}
+
+// x mod P for |x| < P
+__device__ __forceinline__ int wrap(int x, int P) { return x < 0 ? x + P : x; }
+
+// S_b = sum_z sigma^(z) e^{+2 pi i k.z / P} for batch entry b, with
+// sigma^ = ea^-1 delta + (1 - ea^-1) m^ (m^: the clip's indicator spectrum,
+// window (4 lmax + 1) x (4 mmax + 1))
+__global__ __launch_bounds__(FFT_THREADS) void sigma_spectrum_kernel(euvl::FftGrid g, int lmax, int mmax,
+                                                                     const double2* W, int per_clip,
+                                                                     int64_t w_stride, double2 ea_inv,
+                                                                     double2* S) {
+  __shared__ double2 X[FFT_MAX_PTS];
+  __shared__ double2 root[2][FFT_MAX_P];
+  const int b = blockIdx.x, Pl = g.P[0], Pm = g.P[1], n = Pl * Pm;
+  fill_roots(g, root);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) X[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const int wl = 4 * lmax + 1, wm = 4 * mmax + 1;
+  const double2* Wc = W + (int64_t)(b / per_clip) * w_stride;
+  const double2 om = make_double2(1.0 - ea_inv.x, -ea_inv.y);
+  for (int e = threadIdx.x; e < wl * wm; e += blockDim.x) {
+    const int dl = e / wm - 2 * lmax, dm = e % wm - 2 * mmax;
+    double2 s = cmul(om, Wc[e]);
+    if (dl == 0 && dm == 0) s = cadd(s, ea_inv);
+    X[wrap(dl, Pl) * Pm + wrap(dm, Pm)] = s;
+  }
+  __syncthreads();
+  fft_axis(X, root[1], g, 1, true, Pl, 0);
+  fft_axis(X, root[0], g, 0, true, Pm, 0);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) S[(int64_t)b * n + e] = X[e];
+}
+
+// V[b][i][:] -= (H G)[b][i][:], one workgroup per (i, b)
+__global__ __launch_bounds__(FFT_THREADS) void coupling_fft_kernel(euvl::FftGrid g, int N, int lmax,
+                                                                   const int* __restrict__ lidx,
+                                                                   const int* __restrict__ midx,
+                                                                   const double* __restrict__ kt, double scale,
+                                                                   const double2* __restrict__ H,
+                                                                   const double2* __restrict__ S, double2* V,
+                                                                   int64_t ldv, int64_t sv) {
+  __shared__ double2 X[FFT_MAX_PTS];
+  __shared__ double2 root[2][FFT_MAX_P];
+  const int i = blockIdx.x, b = blockIdx.y, Pl = g.P[0], Pm = g.P[1], n = Pl * Pm;
+  const int keep = (2 * lmax + 1 < Pl) ? 2 * lmax + 1 : 0;
+  const int rows = keep ? keep : Pl;
+  fill_roots(g, root);
+  for (int e = threadIdx.x; e < n; e += blockDim.x) X[e] = make_double2(0.0, 0.0);
+  __syncthreads();
+  const double2* Hi = H + ((int64_t)b * N + i) * N;
+  const double* ktb = kt + (int64_t)b * N;
+  for (int q = threadIdx.x; q < N; q += blockDim.x)
+    X[wrap(lidx[q], Pl) * Pm + wrap(midx[q], Pm)] = cscale(Hi[q], ktb[q]);
+  __syncthreads();
+  fft_axis(X, root[1], g, 1, false, rows, keep);
+  fft_axis(X, root[0], g, 0, false, Pm, 0);
+  const double2* Sb = S + (int64_t)b * n;
+  for (int e = threadIdx.x; e < n; e += blockDim.x) X[e] = cmul(X[e], Sb[e]);
+  __syncthreads();
+  fft_axis(X, root[0], g, 0, true, Pm, 0);
+  fft_axis(X, root[1], g, 1, true, rows, keep);
+  double2* Vi = V + (int64_t)b * sv + (int64_t)i * ldv;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    const double2 c = X[wrap(lidx[j], Pl) * Pm + wrap(midx[j], Pm)];
+    Vi[j] = csub(Vi[j], cscale(c, ktb[j] * scale));
+  }
+}
+
+// smallest size >= lo with no prime factor above 3, and its radices (nines
+// first, then fours, a two, threes); 0 when above FFT_MAX_P
+int fft_size(int lo, int* rad, int* nst) {
+  for (int P = std::max(lo, 1); P <= FFT_MAX_P; ++P) {
+    int p = P, k = 0;
+    for (int r : {9, 4, 2, 3})
+      while (p % r == 0 && k < euvl::FFT_MAX_STAGES) {
+        rad[k++] = r;
+        p /= r;
+      }
+    if (p == 1) {
+      *nst = k;
+      return P;
+    }
+  }
+  return 0;
+}
+
+}  // namespace
+
+namespace euvl {
+
+bool fft_grid_for(int lmax, int mmax, FftGrid& g) {
+  g.P[0] = fft_size(4 * lmax + 1, g.rad[0], &g.nst[0]);
+  g.P[1] = fft_size(4 * mmax + 1, g.rad[1], &g.nst[1]);
+  return g.P[0] > 0 && g.P[1] > 0 && g.P[0] * g.P[1] <= FFT_MAX_PTS;
+}
+
+size_t coupling_fft_bytes(int E) { return sizeof(double2) * (size_t)FFT_MAX_PTS * E; }
+
+int coupling_fft(euvl_handle h, const FftGrid& g, int N, int E, const int* lidx, const int* midx, int lmax,
+                 int mmax, const double2* W, int per_clip, int64_t w_stride, double2 ea_inv, const double* kt,
+                 double k, const double2* H, double2* V, int64_t ldv, int64_t sv, double2* S) {
+  hipLaunchKernelGGL(sigma_spectrum_kernel, dim3(E), dim3(FFT_THREADS), 0, h->stream, g, lmax, mmax, W, per_clip,
+                     w_stride, ea_inv, S);
+  EUVL_LAUNCH_CHECK();
+  const double scale = 1.0 / (k * k * (double)g.P[0] * (double)g.P[1]);
+  hipLaunchKernelGGL(coupling_fft_kernel, dim3(N, E), dim3(FFT_THREADS), 0, h->stream, g, N, lmax, lidx, midx, kt,
+                     scale, H, (const double2*)S, V, ldv, sv);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+}  // namespace euvl

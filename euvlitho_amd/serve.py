@@ -84,7 +84,8 @@ def batch_size(cfg: Config, max_batch=4):
 def footprint(cfg: Config, n_clips=1):
     """Device bytes one batch of n_clips requests of this configuration needs
     beyond the resident state: the EM workspace (13 N x N complex matrices per
-    clip and offset plus vectors, the incidence columns and the mask spectra)
+    clip and offset plus vectors, the incidence columns, the coupling
+    product's FFT spectra and the mask spectra)
     or the image workspace, whichever is larger, plus the amplitude and image
     tensors."""
     lidx, _, L, M = cfg.order_set()
@@ -95,7 +96,7 @@ def footprint(cfg: Config, n_clips=1):
     mat, vec = 16 * N * N * B, 16 * N * B
     lpx, lpy = cfg.lpmax
     spectra = 16 * n_clips * ((4 * L + 1) * (4 * M + 1) + (2 * lpx + 1) * (2 * lpy + 1))
-    em = 13 * mat + 8 * vec + 2 * 16 * N * ninc * B + 4 * N * B + 2 * MIB
+    em = 13 * mat + 8 * vec + 3 * 16 * N * ninc * B + 4 * N * B + 16 * 8192 * B + 2 * MIB
     imaging = 272 * MIB
     nx, ny = cfg.image_shape
     tensors = 16 * B * (2 * lsx + 1) * (2 * lsy + 1) * N + 3 * 8 * nx * ny * n_clips
