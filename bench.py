@@ -711,9 +711,10 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "flops_counted": "reference algorithm: 164 N^3 per (clip, polarisation, offset) solve "
                                       f"x {n_solves} solves = {algo_flops:.3e} per clip, over the clip's GEMM time",
-                     "frac_note": "the reference algorithm's work per time: above 1 because this route needs fewer "
-                                  "flops than it (17 N x N x N products and a block Gauss-Jordan solve per solve, "
-                                  "no eigendecomposition); the MFMA units' own utilisation is mfma_issue_frac",
+                     "frac_note": "the reference algorithm's work per GEMM time: above 1 because this route needs "
+                                  "fewer flops than it (16 N x N x N products, the coupling product by 2D FFTs "
+                                  "and a block Gauss-Jordan solve per solve, no eigendecomposition); the MFMA "
+                                  "units' own utilisation is mfma_issue_frac",
                      "route_flops_per_clip_8mnk": gemm_flops,
                      "algorithmic_flops_per_launch": algo_flops / max(launches, 1),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
