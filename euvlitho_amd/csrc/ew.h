@@ -14,6 +14,10 @@ int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64
 
 int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
                const double2* X2, double2 a3, const double2* X3, double2 g);
+// the six Paterson-Stockmeyer blocks of cos (c) and sinc (f) from D..D^4 (ps_blocks_kernel)
+int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
+                 const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
+                 double2* Hf, double2* Qf, double2* Of);
 int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* d_s, double2 g);
 int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
                  const double2* col);
@@ -34,6 +38,8 @@ int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int*
 // A X = B on the augmented [A | B] (n x (n + nrhs), lda >= n + nrhs) by block
 // Gauss-Jordan elimination without row interchanges; X replaces B, the left
 // n columns are overwritten; info[b] = first zero pivot (1-based) or 0.
+// device scratch (handle's aux buffer) of gj_solve for `batch` systems
+size_t gj_solve_scratch_bytes(int batch);
 int gj_solve(euvl_handle h, int n, int nrhs, double2* A, int64_t lda, int64_t sa, int batch, int* d_info);
 
 // Device-side checks of decisions fixed ahead of time (graph-capturable runs):

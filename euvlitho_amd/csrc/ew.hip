@@ -28,6 +28,42 @@ __global__ void lincomb_kernel(int n, double2* out, double2 a1, const double2* X
   }
 }
 
+// The six Paterson-Stockmeyer blocks of the cos and sinc polynomials from one
+// read of the powers X1..X4 (D, D^2, D^3, D^4), for coefficients c (cos) and
+// f (sinc), 13 each:
+//   H = c11 X3 + c10 X2 + c9 X1 + c8 I + c12 X4,  Q = c7 X3 + c6 X2 + c5 X1 + c4 I,
+//   O = c3 X3 + c2 X2 + c1 X1 + c0 I   (and Hf, Qf, Of with f).
+// Each element is read before any output is written at its index, so an
+// output may share a buffer with an input (Qf with X2).
+struct PsCoeffs {
+  double c[13], f[13];
+};
+__global__ void ps_blocks_kernel(int n, const double2* X1, const double2* X2, const double2* X3, const double2* X4,
+                                 PsCoeffs k, double2* Hc, double2* Qc, double2* Oc, double2* Hf, double2* Qf,
+                                 double2* Of) {
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
+  const int64_t base = (int64_t)blockIdx.y * per;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
+    const int64_t e = base + r;
+    const double2 x1 = X1[e], x2 = X2[e], x3 = X3[e], x4 = X4[e];
+    const double id = (r % ((uint32_t)n + 1) == 0) ? 1.0 : 0.0;
+    auto comb = [&](const double* c, int j) {  // c[j+3] X3 + c[j+2] X2 + c[j+1] X1 + c[j] I
+      return make_double2(c[j + 3] * x3.x + c[j + 2] * x2.x + c[j + 1] * x1.x + c[j] * id,
+                          c[j + 3] * x3.y + c[j + 2] * x2.y + c[j + 1] * x1.y);
+    };
+    double2 hc = comb(k.c, 8), hf = comb(k.f, 8);
+    hc = make_double2(hc.x + k.c[12] * x4.x, hc.y + k.c[12] * x4.y);
+    hf = make_double2(hf.x + k.f[12] * x4.x, hf.y + k.f[12] * x4.y);
+    const double2 qc = comb(k.c, 4), qf = comb(k.f, 4), oc = comb(k.c, 0), of = comb(k.f, 0);
+    Hc[e] = hc;
+    Qc[e] = qc;
+    Oc[e] = oc;
+    Hf[e] = hf;
+    Qf[e] = qf;
+    Of[e] = of;
+  }
+}
+
 // out[b] = X[b] * s[b] (per-batch complex scalar from device memory) + g*I
 __global__ void bscale_kernel(int n, double2* out, const double2* X, const double2* s, double2 g) {
   const uint32_t per = (uint32_t)n * (uint32_t)n;
@@ -221,6 +257,21 @@ int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const 
   EUVL_TRY(check_ew(n, batch));
   hipLaunchKernelGGL(lincomb_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, a1, X1, a2,
                      X2, a3, X3, g);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
+                 const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
+                 double2* Hf, double2* Qf, double2* Of) {
+  EUVL_TRY(check_ew(n, batch));
+  PsCoeffs k;
+  for (int j = 0; j < 13; ++j) {
+    k.c[j] = c[j];
+    k.f[j] = f[j];
+  }
+  hipLaunchKernelGGL(ps_blocks_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, X1, X2, X3, X4,
+                     k, Hc, Qc, Oc, Hf, Qf, Of);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

@@ -96,18 +96,16 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     else
       cf[m / 2] = sgn * hh * scale / fact;
   }
-  // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2)
-  auto poly = [&](const double* c, double2* out) -> int {
-    EUVL_TRY(ew_lincomb(h, n, batch, H, re(c[11]), X3, re(c[10]), X2, re(c[9]), X1, re(c[8])));
-    EUVL_TRY(ew_lincomb(h, n, batch, H, re(1.0), H, re(c[12]), X4, ZERO, nullptr, ZERO));
-    EUVL_TRY(ew_lincomb(h, n, batch, Q, re(c[7]), X3, re(c[6]), X2, re(c[5]), X1, re(c[4])));
-    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
-    EUVL_TRY(ew_lincomb(h, n, batch, out, re(c[3]), X3, re(c[2]), X2, re(c[1]), X1, re(c[0])));
-    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Q, n, sm, re(1.0), out, n, sm, ZERO, batch));
-    return EUVL_OK;
-  };
-  EUVL_TRY(poly(cc, C));
-  EUVL_TRY(poly(cf, F));
+  // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2).
+  // The blocks of both polynomials come from one pass over D..P4 (the sinc
+  // Q1 over P2, which is not read after it); then two products each.
+  double2* Hf = s[0];
+  double2* Qf = X2;
+  EUVL_TRY(ew_ps_blocks(h, n, batch, X1, X2, X3, X4, cc, cf, H, Q, C, Hf, Qf, F));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Q, n, sm, re(1.0), C, n, sm, ZERO, batch));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Hf, n, sm, re(1.0), Qf, n, sm, ZERO, batch));
+  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Qf, n, sm, re(1.0), F, n, sm, ZERO, batch));
   // doublings: F <- 2 F C, C <- 2 C^2 - I  (angle h -> 2h)
   double2 *c = C, *f = F, *tc = s[0], *tf = X2;
   for (int q = 0; q < sq; ++q) {

@@ -209,7 +209,7 @@ extern "C" int euvl_plan_create(euvl_handle h, const euvl_plan_desc* d, euvl_pla
   body = std::max(body, solve_workspace_bytes(pl->N, E, pl->sd.ninc) + 256 * 16);
   if (pl->n_src > 0) body = std::max(body, abbe_workspace_bytes(pl->n_src, pl->lpx, pl->lpy, pl->nx, pl->ny));
   pl->ws_need = head + body + 4096;
-  pl->aux_need = 2 * al256(sizeof(double2) * (size_t)E * 32 * 32);  // the unpivoted LU's block inverses
+  pl->aux_need = al256(gj_solve_scratch_bytes(E));  // the top solve's diagonal-block inverses
   *out = pl;
   return EUVL_OK;
 }

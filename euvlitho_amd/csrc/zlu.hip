@@ -637,6 +637,8 @@ int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int*
 // panel chain per factor and per solve, and every update is one GEMM.  The
 // left n columns are overwritten (no factors are kept).  info[b] receives the
 // first zero pivot of a diagonal block (1-based column), 0 if none.
+size_t gj_solve_scratch_bytes(int batch) { return sizeof(double2) * (size_t)batch * GJ_NB * GJ_NB; }
+
 int gj_solve(euvl_handle h, int n, int nrhs, double2* A, int64_t lda, int64_t sa, int batch, int* d_info) {
   constexpr int GJ_THREADS = 1024;
   PrecScope scope(h, EUVL_PREC_FP32_LU);
@@ -645,8 +647,7 @@ int gj_solve(euvl_handle h, int n, int nrhs, double2* A, int64_t lda, int64_t sa
     set_error("gj_solve: leading dimension %lld below n + nrhs = %d", (long long)lda, n + nrhs);
     return EUVL_ERR_ARG;
   }
-  const size_t b_blk = sizeof(double2) * (size_t)batch * GJ_NB * GJ_NB;
-  double2* dinv = (double2*)aux_reserve(h, b_blk);
+  double2* dinv = (double2*)aux_reserve(h, gj_solve_scratch_bytes(batch));
   if (!dinv) {
     set_error("gj_solve: scratch allocation failed");
     return EUVL_ERR_NOMEM;
