@@ -355,8 +355,11 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
 // at 1024 threads); in complex-product TFLOP/s (8mnk per time) at 1681 x 4 /
 // 4096: 48/57, 71/93, 64/80, 66/83, 63/85 with unchecked interior loads
 // issued after the first K step (before: 58/72, 66/84, 58/73, 63/77, 56/69;
-// 55/69 for the 4M form).  A register-only loop of f64 16x16x4 MFMAs
-// sustains 77.9 TFLOP/s on MI355X, f32 16x16x4 155.5.
+// 55/69 for the 4M form).  The 3M form also has 5 = 2x2 waves of one tile
+// each (32x32; 74.7 at 1681 x 1681 x 1680 x 2 and 62.6 at 1681 x 171 x 1681 x 4,
+// against 74.0 and 55.3 for 6) and 6 = 2x2 waves of 2x1 tiles (64x32).  A
+// register-only loop of f64 16x16x4 MFMAs sustains 77.9 TFLOP/s on MI355X,
+// f32 16x16x4 155.5.
 // FP32-compute variants (waves WM x WN, each TM x TN MFMA tiles, K slab 16):
 // 0 = 2x2 waves, 64x64; 1 = 2x4 waves, 64x128; 2 = 4x2 waves, 128x64;
 // 3 = 4x4 waves, 128x128; 4 = 2x2 waves of 4x2 tiles, 128x64.
@@ -374,6 +377,7 @@ bool tile_shape(int kind, int v, int& bm, int& bn) {
     case 2: bm = (kind == 1 || kind == 3) ? 128 : 64; bn = (kind == 1 || kind == 3) ? 64 : 128; return true;
     case 3: bm = 128; bn = (kind == 2) ? 64 : 128; return true;
     case 4: bm = 128; bn = 64; return true;
+    case 5: bm = 32; bn = 32; return kind == 2;
     case 6: bm = 64; bn = 32; return kind == 2 || kind == 3;
     default: return false;
   }
@@ -411,9 +415,10 @@ int tuned_variant(int kind, int m, int n, int k, int batch) {
 }
 
 // Untuned FP64 3M products whose 64x64 grid has fewer than two workgroups
-// per CU take the 64x32 tile (variant 6).  On MI355X (us, 64x64 -> 64x32):
-// 1681 x 171 x 1681 x 4: 371 -> 306; the LU panels 1649 x 32 x 32 x 4 and
-// 32 x 1649 x 32 x 4: 9 -> 6; 1617 x 171 x 32 x 4: 13 -> 12.  Full-size
+// per CU take the 32x32 tile (variant 5; the 64x32 tile, variant 6, before).
+// On MI355X (us, 64x64 -> 64x32 -> 32x32): 1681 x 171 x 1681 x 4: 371 -> 306
+// -> 274 (the last two with a copy of C); the LU panels 1649 x 32 x 32 x 4
+// and 32 x 1649 x 32 x 4: 9 -> 6; 1617 x 171 x 32 x 4: 13 -> 12.  Full-size
 // products stay on the 64x64 tile (1681 x 4: 2062 vs 2083).
 int pick_variant(int kind, int m, int n, int k, int batch) {
   if (g_env_variant[kind] == -2) g_env_variant[kind] = env_variant("EUVL_GEMM_TILE", kind);
@@ -421,7 +426,7 @@ int pick_variant(int kind, int m, int n, int k, int batch) {
   const int t = tuned_variant(kind, m, n, k, batch);
   if (t >= 0) return t;
   const int64_t grid64 = (int64_t)((m + 63) / 64) * ((n + 63) / 64) * batch;
-  return (kind == 2 && grid64 < 512) ? 6 : 1;
+  return (kind == 2 && grid64 < 512) ? 5 : 1;
 }
 
 // FP32 compute.  With the FAST form the 64x64 tile (five workgroups per CU)
@@ -539,6 +544,8 @@ void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2
       // 128x64: three accumulator sets of 2x2 tiles spill at 1024 threads
       case 3: launch<F64M3, 4, 4, 2, 1, 16, 1>(EUVL_GEMM_ARGS); break;
       case 4: launch<F64M3, 4, 2, 2, 2, 16, 2>(EUVL_GEMM_ARGS); break;
+      // 32x32 tile (one MFMA tile per wave)
+      case 5: launch<F64M3, 2, 2, 1, 1, 16, 6>(EUVL_GEMM_ARGS); break;
       // 64x32 tile, for grids too small to fill the chip with 64x64 tiles
       case 6: launch<F64M3, 2, 2, 2, 1, 16, 4>(EUVL_GEMM_ARGS); break;
       default: launch<F64M3, 2, 2, 2, 2, 16, 3>(EUVL_GEMM_ARGS); break;
