@@ -620,7 +620,9 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   v.alru = (double2*)ws_take(h, vec);
   v.b2 = (double2*)ws_take(h, vec);
   double2* Mx[7];
-  for (int i = 0; i < 7; ++i) Mx[i] = (i == 5) ? nullptr : (double2*)ws_take(h, mat);
+  for (int i = 0; i < 7; ++i) Mx[i] = (i == 5 || i == 3) ? nullptr : (double2*)ws_take(h, i == 2 ? 2 * mat : mat);
+  // C and F (Mx[2], Mx[3]) contiguous: cos_sinc runs their products as pairs
+  if (Mx[2]) Mx[3] = Mx[2] + (int64_t)N * N * B;
   // the top-solve matrix with its right-hand sides appended: [V | E], N x (N + ninc)
   const int64_t ldv = (int64_t)N + ninc, sv = (int64_t)N * ldv;
   Mx[5] = (double2*)ws_take(h, sizeof(double2) * (size_t)sv * B);

@@ -8,6 +8,11 @@ namespace euvl {
 int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
           const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
           double2 gamma, int batch);
+// zgemm whose batch entry b reads A entry b mod per_a and B entry b mod per_b
+// (0: no period), so that products sharing an operand run as one launch
+int zgemm_periodic(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
+                   const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
+                   double2 gamma, int batch, int per_a, int per_b);
 int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info);
 int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64_t sa, const int* ipiv,
            double2* B, int64_t ldb, int64_t sb, int batch);
@@ -15,6 +20,8 @@ int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64
 int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
                const double2* X2, double2 a3, const double2* X3, double2 g);
 // the six Paterson-Stockmeyer blocks of cos (c) and sinc (f) from D..D^4 (ps_blocks_kernel)
+// X[b] += g I for the batch entries b < batch (the diagonal only)
+int ew_diag_add(euvl_handle h, int n, int batch, double2* X, double2 g);
 int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
                  const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
                  double2* Hf, double2* Qf, double2* Of);

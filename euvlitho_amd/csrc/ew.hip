@@ -64,6 +64,14 @@ __global__ void ps_blocks_kernel(int n, const double2* X1, const double2* X2, co
   }
 }
 
+__global__ void diag_add_kernel(int n, int batch, double2* X, double2 g) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)n * batch) return;
+  const int64_t b = e / n, i = e - b * n;
+  double2* x = X + b * n * n + i * (n + 1);
+  *x = cadd(*x, g);
+}
+
 // out[b] = X[b] * s[b] (per-batch complex scalar from device memory) + g*I
 __global__ void bscale_kernel(int n, double2* out, const double2* X, const double2* s, double2 g) {
   const uint32_t per = (uint32_t)n * (uint32_t)n;
@@ -272,6 +280,14 @@ int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const doubl
   }
   hipLaunchKernelGGL(ps_blocks_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, X1, X2, X3, X4,
                      k, Hc, Qc, Oc, Hf, Qf, Of);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_diag_add(euvl_handle h, int n, int batch, double2* X, double2 g) {
+  EUVL_TRY(check_ew(n, batch));
+  const int64_t total = (int64_t)n * batch;
+  hipLaunchKernelGGL(diag_add_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, n, batch, X, g);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

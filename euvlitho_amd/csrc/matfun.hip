@@ -33,15 +33,16 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   const double theta = cos_sinc_theta(h);
   PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
   const size_t mat = sizeof(double2) * (size_t)n * n * batch;
-  double2* s[6];
-  for (int i = 0; i < 6; ++i) {
-    s[i] = (double2*)ws_take(h, mat);
-    if (!s[i]) {
-      set_error("cos_sinc: workspace too small");
-      return EUVL_ERR_NOMEM;
-    }
+  // six scratch matrices in one block, so that neighbours are contiguous
+  // (pairs of them are the operands of the paired launches below)
+  double2* s0 = (double2*)ws_take(h, 6 * mat);
+  if (!s0) {
+    set_error("cos_sinc: workspace too small");
+    return EUVL_ERR_NOMEM;
   }
   const int64_t sm = (int64_t)n * n;
+  double2* s[6];
+  for (int i = 0; i < 6; ++i) s[i] = s0 + (int64_t)i * sm * batch;
   // Unscaled powers P2 = D^2, P3 = D^3, P4 = D^4 (the Paterson-Stockmeyer
   // blocks).  For every j >= 6, ||D^j||^(1/j) <= alpha = max(||D^3||^(1/3),
   // ||D^4||^(1/4)) (Al-Mohy and Higham, SIAM J. Matrix Anal. Appl. 31 (2009),
@@ -99,23 +100,43 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2).
   // The blocks of both polynomials come from one pass over D..P4 (the sinc
   // Q1 over P2, which is not read after it); then two products each.
-  double2* Hf = s[0];
-  double2* Qf = X2;
-  EUVL_TRY(ew_ps_blocks(h, n, batch, X1, X2, X3, X4, cc, cf, H, Q, C, Hf, Qf, F));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Q, n, sm, re(1.0), C, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Hf, n, sm, re(1.0), Qf, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Qf, n, sm, re(1.0), F, n, sm, ZERO, batch));
-  // doublings: F <- 2 F C, C <- 2 C^2 - I  (angle h -> 2h)
-  double2 *c = C, *f = F, *tc = s[0], *tf = X2;
-  for (int q = 0; q < sq; ++q) {
-    EUVL_TRY(zgemm(h, n, n, n, re(2.0), f, n, sm, c, n, sm, ZERO, tf, n, sm, ZERO, batch));
-    EUVL_TRY(zgemm(h, n, n, n, re(2.0), c, n, sm, c, n, sm, ZERO, tc, n, sm, re(-1.0), batch));
-    std::swap(c, tc);
-    std::swap(f, tf);
+  if (F == C + sm * batch) {
+    // C and F contiguous: the cos and sinc products of a step share their
+    // left operand (P4, or C in a doubling) and run as one launch of 2 batch
+    // entries (the last batch ones read the same operand again); blocks
+    // Hc, Hf in s[3], s[4] and Qc, Qf in s[0], s[1] (= P2), also contiguous
+    EUVL_TRY(ew_ps_blocks(h, n, batch, X1, X2, X3, X4, cc, cf, s[3], s[0], C, s[4], s[1], F));
+    EUVL_TRY(zgemm_periodic(h, n, n, n, re(1.0), X4, n, sm, s[3], n, sm, re(1.0), s[0], n, sm, ZERO, 2 * batch,
+                            batch, 0));
+    EUVL_TRY(zgemm_periodic(h, n, n, n, re(1.0), X4, n, sm, s[0], n, sm, re(1.0), C, n, sm, ZERO, 2 * batch,
+                            batch, 0));
+    // doublings: [C; F] <- 2 [C; F] C, then C -= I  (angle h -> 2h)
+    double2 *c = C, *t = s[0];
+    for (int q = 0; q < sq; ++q) {
+      EUVL_TRY(zgemm_periodic(h, n, n, n, re(2.0), c, n, sm, c, n, sm, ZERO, t, n, sm, ZERO, 2 * batch, 0, batch));
+      EUVL_TRY(ew_diag_add(h, n, batch, t, re(-1.0)));
+      std::swap(c, t);
+    }
+    if (c != C) EUVL_TRY(ew_copy(h, 2 * sm * batch, C, c));
+  } else {
+    double2* Hf = s[0];
+    double2* Qf = X2;
+    EUVL_TRY(ew_ps_blocks(h, n, batch, X1, X2, X3, X4, cc, cf, H, Q, C, Hf, Qf, F));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, H, n, sm, re(1.0), Q, n, sm, ZERO, batch));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Q, n, sm, re(1.0), C, n, sm, ZERO, batch));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Hf, n, sm, re(1.0), Qf, n, sm, ZERO, batch));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X4, n, sm, Qf, n, sm, re(1.0), F, n, sm, ZERO, batch));
+    // doublings: F <- 2 F C, C <- 2 C^2 - I  (angle h -> 2h)
+    double2 *c = C, *f = F, *tc = s[0], *tf = X2;
+    for (int q = 0; q < sq; ++q) {
+      EUVL_TRY(zgemm(h, n, n, n, re(2.0), f, n, sm, c, n, sm, ZERO, tf, n, sm, ZERO, batch));
+      EUVL_TRY(zgemm(h, n, n, n, re(2.0), c, n, sm, c, n, sm, ZERO, tc, n, sm, re(-1.0), batch));
+      std::swap(c, tc);
+      std::swap(f, tf);
+    }
+    if (c != C) EUVL_TRY(ew_copy(h, sm * batch, C, c));
+    if (f != F) EUVL_TRY(ew_copy(h, sm * batch, F, f));
   }
-  if (c != C) EUVL_TRY(ew_copy(h, sm * batch, C, c));
-  if (f != F) EUVL_TRY(ew_copy(h, sm * batch, F, f));
   if (G2) EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, F, n, sm, ZERO, G2, n, sm, ZERO, batch));
   return EUVL_OK;
 }

@@ -87,7 +87,7 @@ template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST>
 __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
     int M, int N, int K, double2 alpha, const double2* __restrict__ A, int64_t lda, int64_t sa,
     const double2* __restrict__ B, int64_t ldb, int64_t sb, double2 beta, double2* C,
-    int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles, int shift) {
+    int64_t ldc, int64_t sc, double2 gamma, int tiles_n, int tiles, int shift, int per_a, int per_b) {
   constexpr int NT = 64 * WM * WN;                      // WM x WN waves
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;   // workgroup tile
   constexpr int QA = BM * BK / NT, QB = BN * BK / NT;   // complex values per thread per slab
@@ -105,8 +105,10 @@ __global__ __launch_bounds__(64 * WM * WN, MINB) void zgemm_mfma_kernel(
   const int ng = (g < 8 * q8) ? (g % 8) * q8 + g / 8 : g;
   const int b = ng / tiles;
   const int tile = ng - b * tiles;
-  A += (int64_t)b * sa;
-  B += (int64_t)b * sb;
+  // per_a / per_b > 0: batch entry b reads operand entry b mod per (one
+  // operand shared by several groups of entries)
+  A += (int64_t)(per_a > 0 ? b % per_a : b) * sa;
+  B += (int64_t)(per_b > 0 ? b % per_b : b) * sb;
   C += (int64_t)b * sc;
   // grouped order: runs of GM tile rows are walked column by column, so the
   // ~128 workgroups an XCD runs at once cover a GM x (128 / GM) block of tiles
@@ -472,10 +474,12 @@ bool blocks_overlap(const double2* x, int64_t ldx, int rx, int cx, const double2
 }
 
 Overlap overlap_of(int m, int n, int k, const double2* A, int64_t lda, int64_t sa, const double2* B,
-                   int64_t ldb, int64_t sb, const double2* C, int64_t ldc, int64_t sc, int batch) {
+                   int64_t ldb, int64_t sb, const double2* C, int64_t ldc, int64_t sc, int batch, int per_a = 0,
+                   int per_b = 0) {
   Overlap o;
   if (k <= 0 || m <= 0 || n <= 0) return o;
-  const int na = sa ? batch : 1, nb = sb ? batch : 1;
+  const int na = sa ? (per_a > 0 ? std::min(per_a, batch) : batch) : 1;
+  const int nb = sb ? (per_b > 0 ? std::min(per_b, batch) : batch) : 1;
   const bool exact = (int64_t)batch * std::max(na, nb) <= 4096;
   for (int bc = 0; bc < batch && !(o.a && o.b); ++bc) {
     const double2* cb = C + bc * sc;
@@ -511,17 +515,17 @@ int safe_variant(int kind, int v, int m, int n, const Overlap& o) {
 template <class P, int WM, int WN, int TM, int TN, int BK, int MINB, bool FAST = P::fast_loads>
 void launch(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
             const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
-            double2 gamma, int batch) {
+            double2 gamma, int batch, int per_a, int per_b) {
   constexpr int BM = 16 * TM * WM, BN = 16 * TN * WN;
   const int tiles_m = (m + BM - 1) / BM, tiles_n = (n + BN - 1) / BN;
   const int tiles = tiles_m * tiles_n;
   // edge tiles may be shifted back inside C unless C overlaps an operand (an
   // in-place product: a shifted tile would read rows another tile writes)
-  const Overlap o = overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch);
+  const Overlap o = overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch, per_a, per_b);
   const int shift = (m >= BM && n >= BN && !o.a && !o.b) ? 1 : 0;
   hipLaunchKernelGGL((zgemm_mfma_kernel<P, WM, WN, TM, TN, BK, MINB, FAST>), dim3(tiles * batch), dim3(64 * WM * WN), 0,
                      h->stream, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, tiles_n, tiles,
-                     shift);
+                     shift, per_a, per_b);
 }
 
 // kind: 0 = FP64 4M, 1 = FP32 compute 4M, 2 = FP64 3M, 3 = FP32 compute 3M
@@ -535,8 +539,8 @@ bool is_fp32_kind(int kind) { return kind == 1 || kind == 3; }
 
 void launch_variant(euvl_handle h, int kind, int v, int m, int n, int k, double2 alpha, const double2* A,
                     int64_t lda, int64_t sa, const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C,
-                    int64_t ldc, int64_t sc, double2 gamma, int batch) {
-#define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch
+                    int64_t ldc, int64_t sc, double2 gamma, int batch, int per_a = 0, int per_b = 0) {
+#define EUVL_GEMM_ARGS h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch, per_a, per_b
   if (kind == 2) {
     switch (v) {
       case 0: launch<F64M3, 2, 2, 2, 2, 16, 1>(EUVL_GEMM_ARGS); break;
@@ -587,6 +591,12 @@ namespace euvl {
 int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
           const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
           double2 gamma, int batch) {
+  return zgemm_periodic(h, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch, 0, 0);
+}
+
+int zgemm_periodic(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, int64_t lda, int64_t sa,
+                   const double2* B, int64_t ldb, int64_t sb, double2 beta, double2* C, int64_t ldc, int64_t sc,
+                   double2 gamma, int batch, int per_a, int per_b) {
   if (m <= 0 || n <= 0 || batch <= 0) return EUVL_OK;
   if ((int64_t)((m + 31) / 32) * ((n + 31) / 32) * batch > (int64_t)1 << 30) {
     set_error("zgemm: grid too large");
@@ -607,14 +617,14 @@ int zgemm(euvl_handle h, int m, int n, int k, double2 alpha, const double2* A, i
   }
   const int kind = gemm_kind(h);
   int v = is_fp32_kind(kind) ? pick_variant32(kind, m, n, k, batch) : pick_variant(kind, m, n, k, batch);
-  v = safe_variant(kind, v, m, n, overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch));
+  v = safe_variant(kind, v, m, n, overlap_of(m, n, k, A, lda, sa, B, ldb, sb, C, ldc, sc, batch, per_a, per_b));
   if (v < 0) {
     if (ev >= 0) h->ev_next -= 2;
     set_error("zgemm: C overlaps an operand and no tile spans it (m %d, n %d): in-place products need n <= 128 "
               "(C on A) or m <= 128 (C on B)", m, n);
     return EUVL_ERR_ARG;
   }
-  launch_variant(h, kind, v, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch);
+  launch_variant(h, kind, v, m, n, k, alpha, A, lda, sa, B, ldb, sb, beta, C, ldc, sc, gamma, batch, per_a, per_b);
   EUVL_LAUNCH_CHECK();
   if (ev >= 0) {
     EUVL_HIP_CHECK(hipEventRecord(h->ev_pool[ev + 1], h->stream));
