@@ -704,6 +704,8 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
   hipEvent_t e0 = nullptr, e1 = nullptr;
   double best_ms = 1e300;
   int best = -1;
+  double times[MAX_VARIANT + 1];
+  for (double& t : times) t = -1.0;
   do {
     // operands of ones: the MFMA time does not depend on the values
     if ((rc = euvl::fill(h, (int64_t)ea, A, make_double2(1.0, 0.0))) != EUVL_OK) break;
@@ -737,7 +739,19 @@ extern "C" int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch,
       (void)hipEventElapsedTime(&ms, e0, e1);
       const double per = ms / reps;
       if (out_ms) out_ms[v] = per;
-      if (per < best_ms) { best_ms = per; best = v; }
+      times[v] = per;
+      if (per < best_ms) best_ms = per;
+    }
+    // among the variants within 1 % of the fastest (a tie at this timing's
+    // resolution), the largest tile: fewest operand re-reads per output
+    int best_area = -1;
+    for (int v = 0; v <= MAX_VARIANT && rc == EUVL_OK; ++v) {
+      int bm, bn;
+      if (times[v] < 0.0 || !tile_shape(kind, v, bm, bn)) continue;
+      if (times[v] <= 1.01 * best_ms && bm * bn > best_area) {
+        best_area = bm * bn;
+        best = v;
+      }
     }
   } while (false);
   if (e0) (void)hipEventDestroy(e0);

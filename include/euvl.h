@@ -93,7 +93,8 @@ int euvl_set_precision(euvl_handle h, int mask);
 
 /* GEMM autotuner: times every kernel variant (tile shape, waves per
  * workgroup) of the m x n x k x batch product in FP64 (fp32 = 0) or
- * FP32-compute (fp32 = 1) on scratch operands, and keeps the fastest for that
+ * FP32-compute (fp32 = 1) on scratch operands, and keeps the fastest (of
+ * those within 1 % of the fastest, the one with the largest tile) for that
  * size bucket (m, n, k rounded up to 64 below 256 and to 128 above; batch
  * exact) for every later GEMM of the process.  *out_variant receives the
  * choice; out_ms (optional, 7 doubles, indexed by variant number) the time

@@ -207,7 +207,8 @@ def test_autotune_keeps_results(lib):
             assert len(timed) == 7
         else:
             assert ms[5] == -1.0 and len(timed) == 6 and v != 5
-        assert ms[v] == min(timed)
+        # the fastest, or a larger tile within 1 % of it
+        assert ms[v] <= 1.01 * min(timed)
     after = eng.amp_s(bits, "X").cpu().numpy()
     e = np.abs(after - before).max() / np.abs(before).max()
     say(f"  amplitudes after tuning vs before: {e:.1e}")
