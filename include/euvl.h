@@ -99,7 +99,8 @@ int euvl_set_precision(euvl_handle h, int mask);
  * exact) for every later GEMM of the process.  *out_variant receives the
  * choice; out_ms (optional, 7 doubles, indexed by variant number) the time
  * per variant, -1 for a number the product form has no variant for (5 in
- * all but the FP64 3M form, 6 in the 4M forms).  Allocates and
+ * all but the FP64 3M form, and there when 64 x 64 tiles would give 512
+ * workgroups or more; 6 in the 4M forms).  Allocates and
  * frees its scratch; synchronises the handle's stream.  No reference
  * counterpart. */
 int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch, int fp32, int* out_variant,

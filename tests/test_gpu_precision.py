@@ -200,8 +200,8 @@ def test_autotune_keeps_results(lib):
     for key in ("fp64", "fp32"):
         v, ms = tuned[key]
         timed = [t for t in ms if t > 0]
-        # the 3M forms (the default): FP64 has variants 0-6 (5 is the 32x32 tile),
-        # FP32 0-4 and 6 (number 5 is not one of its variants)
+        # the 3M forms (the default): FP64 has variants 0-6 (5 is the 32x32 tile,
+        # timed on this small grid), FP32 0-4 and 6 (5 is not one of its variants)
         assert len(ms) == 7
         if key == "fp64":
             assert len(timed) == 7
