@@ -17,17 +17,12 @@ int zgetrf(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv,
 int zgetrs(euvl_handle h, int n, int nrhs, const double2* LU, int64_t lda, int64_t sa, const int* ipiv,
            double2* B, int64_t ldb, int64_t sb, int batch);
 
-int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
-               const double2* X2, double2 a3, const double2* X3, double2 g);
-// the six Paterson-Stockmeyer blocks of cos (c) and sinc (f) from D..D^4 (ps_blocks_kernel)
 // X[b] += g I for the batch entries b < batch (the diagonal only)
 int ew_diag_add(euvl_handle h, int n, int batch, double2* X, double2 g);
+// the six Paterson-Stockmeyer blocks of cos (c) and sinc (f) from D..D^4 (ps_blocks_kernel)
 int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
                  const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
                  double2* Hf, double2* Qf, double2* Of);
-int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* d_s, double2 g);
-int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
-                 const double2* col);
 int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X);
 // out[0:count) = v (a fill kernel: safe inside graph capture, unlike hipMemsetAsync)
 int fill(euvl_handle h, int64_t count, double2* out, double2 v);
@@ -35,10 +30,8 @@ int fill(euvl_handle h, int64_t count, double2* out, double2 v);
 int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red);
 int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out);
 // max |re| + |im| over an m x ncols block of each batch entry (leading dim ld, batch stride sb) into
-// d_slot[0:batch) (maxabs), or to the host, synchronising (maxabs_to_host)
+// d_slot[0:batch)
 int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X, double* d_slot);
-int maxabs_to_host(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X,
-                   double* out);
 // LU without row interchanges (blocked, GEMM panels); ipiv is written as the identity.  Only for matrices
 // whose unpivoted elimination is safe; callers verify what they solve with it.
 int zgetrf_nopiv(euvl_handle h, int n, double2* A, int64_t lda, int64_t sa, int* ipiv, int batch, int* d_info);

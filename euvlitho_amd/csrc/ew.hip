@@ -13,21 +13,6 @@ __device__ inline int64_t gidx() { return (int64_t)blockIdx.x * blockDim.x + thr
 // inside one n x n matrix (n*n < 2^31, checked by the callers): the row and
 // column of an element come from a 32-bit division, not a 64-bit one.
 
-// out = a1*X1 + a2*X2 + a3*X3 + g*I  (X2/X3 may be null)
-__global__ void lincomb_kernel(int n, double2* out, double2 a1, const double2* X1, double2 a2, const double2* X2,
-                               double2 a3, const double2* X3, double2 g) {
-  const uint32_t per = (uint32_t)n * (uint32_t)n;
-  const int64_t base = (int64_t)blockIdx.y * per;
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
-    const int64_t e = base + r;
-    double2 v = cmul(a1, X1[e]);
-    if (X2) v = cadd(v, cmul(a2, X2[e]));
-    if (X3) v = cadd(v, cmul(a3, X3[e]));
-    if (r % ((uint32_t)n + 1) == 0) v = cadd(v, g);  // the diagonal of a row-major n x n block
-    out[e] = v;
-  }
-}
-
 // The six Paterson-Stockmeyer blocks of the cos and sinc polynomials from one
 // read of the powers X1..X4 (D, D^2, D^3, D^4), for coefficients c (cos) and
 // f (sinc), 13 each:
@@ -70,33 +55,6 @@ __global__ void diag_add_kernel(int n, int batch, double2* X, double2 g) {
   const int64_t b = e / n, i = e - b * n;
   double2* x = X + b * n * n + i * (n + 1);
   *x = cadd(*x, g);
-}
-
-// out[b] = X[b] * s[b] (per-batch complex scalar from device memory) + g*I
-__global__ void bscale_kernel(int n, double2* out, const double2* X, const double2* s, double2 g) {
-  const uint32_t per = (uint32_t)n * (uint32_t)n;
-  const int b = blockIdx.y;
-  const int64_t base = (int64_t)b * per;
-  const double2 sb = s[b];
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
-    double2 v = cmul(X[base + r], sb);
-    if (r % ((uint32_t)n + 1) == 0) v = cadd(v, g);
-    out[base + r] = v;
-  }
-}
-
-// X[b](i,j) *= col[b*n + j] (column scaling) and/or row[b*n + i]
-__global__ void diagscale_kernel(int n, double2* out, const double2* X, const double2* row, const double2* col) {
-  const uint32_t per = (uint32_t)n * (uint32_t)n;
-  const int b = blockIdx.y;
-  const int64_t base = (int64_t)b * per;
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
-    const uint32_t i = r / (uint32_t)n, j = r - i * (uint32_t)n;
-    double2 v = X[base + r];
-    if (row) v = cmul(row[(int64_t)b * n + i], v);
-    if (col) v = cmul(v, col[(int64_t)b * n + j]);
-    out[base + r] = v;
-  }
 }
 
 // Per-batch reductions into red[b] (red zeroed beforehand for kinds 0 and 1).
@@ -260,15 +218,6 @@ __global__ void zero_int_kernel2(int* p, int n) {
 
 namespace euvl {
 
-int ew_lincomb(euvl_handle h, int n, int batch, double2* out, double2 a1, const double2* X1, double2 a2,
-               const double2* X2, double2 a3, const double2* X3, double2 g) {
-  EUVL_TRY(check_ew(n, batch));
-  hipLaunchKernelGGL(lincomb_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, a1, X1, a2,
-                     X2, a3, X3, g);
-  EUVL_LAUNCH_CHECK();
-  return EUVL_OK;
-}
-
 int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
                  const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
                  double2* Hf, double2* Qf, double2* Of) {
@@ -288,22 +237,6 @@ int ew_diag_add(euvl_handle h, int n, int batch, double2* X, double2 g) {
   EUVL_TRY(check_ew(n, batch));
   const int64_t total = (int64_t)n * batch;
   hipLaunchKernelGGL(diag_add_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, n, batch, X, g);
-  EUVL_LAUNCH_CHECK();
-  return EUVL_OK;
-}
-
-int ew_bscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* d_s, double2 g) {
-  EUVL_TRY(check_ew(n, batch));
-  hipLaunchKernelGGL(bscale_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, X, d_s, g);
-  EUVL_LAUNCH_CHECK();
-  return EUVL_OK;
-}
-
-int ew_diagscale(euvl_handle h, int n, int batch, double2* out, const double2* X, const double2* row,
-                 const double2* col) {
-  EUVL_TRY(check_ew(n, batch));
-  hipLaunchKernelGGL(diagscale_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, out, X, row,
-                     col);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }
@@ -372,15 +305,6 @@ int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, c
   hipLaunchKernelGGL(reduce_maxabs_kernel, dim3((unsigned)blocks, batch), dim3(256), 0, h->stream, m, ncols, ld, sb, X,
                      d_slot);
   EUVL_LAUNCH_CHECK();
-  return EUVL_OK;
-}
-
-int maxabs_to_host(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X,
-                   double* out) {
-  EUVL_TRY(maxabs(h, m, ncols, ld, sb, batch, X, h->d_red));
-  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * batch, hipMemcpyDeviceToHost, h->stream));
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  for (int b = 0; b < batch; ++b) out[b] = h->h_pinned[b];
   return EUVL_OK;
 }
 
