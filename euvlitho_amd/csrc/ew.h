@@ -28,7 +28,6 @@ int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X);
 int fill(euvl_handle h, int64_t count, double2* out, double2 v);
 // kind 0: max|X-I| (|re|+|im|), 1: infinity norm (max row sum of |x|), 2: mean Re diag
 int reduce(euvl_handle h, int kind, int n, int batch, const double2* X, double* d_red);
-int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out);
 // max |re| + |im| over an m x ncols block of each batch entry (leading dim ld, batch stride sb) into
 // d_slot[0:batch)
 int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X, double* d_slot);

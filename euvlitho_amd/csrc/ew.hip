@@ -288,14 +288,6 @@ int zero_ints(euvl_handle h, int count, int* p) {
   return EUVL_OK;
 }
 
-int reduce_to_host(euvl_handle h, int kind, int n, int batch, const double2* X, double* out) {
-  EUVL_TRY(reduce(h, kind, n, batch, X, h->d_red));
-  EUVL_HIP_CHECK(hipMemcpyAsync(h->h_pinned, h->d_red, sizeof(double) * batch, hipMemcpyDeviceToHost, h->stream));
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  for (int b = 0; b < batch; ++b) out[b] = h->h_pinned[b];
-  return EUVL_OK;
-}
-
 int maxabs(euvl_handle h, int m, int ncols, int64_t ld, int64_t sb, int batch, const double2* X, double* d_slot) {
   hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(256), 0, h->stream, d_slot, batch);
   EUVL_LAUNCH_CHECK();
