@@ -29,6 +29,7 @@ events on the library's stream).  The GEMM kernel's own executed rate
 import argparse
 import gc
 import json
+import math
 import os
 import sys
 import time
@@ -56,7 +57,7 @@ def copy_calibration(mib=512, reps=3):
     print(json.dumps({"copy_calibration_bytes": mib << 20, "reps": reps}), flush=True)
 
 
-def run_served(n_req, slots, precision=None, loads=(0.8,)):
+def run_served(n_req, slots, precision=None, loads=(0.8,), graphs=True):
     """Served form: requests of six problems (pitch 256-1024 nm, orders +-8..+-20,
     61-200 source points), mixed precision, `slots` streams with admission by
     device-memory footprint; arrived requests of one problem share a batched
@@ -67,7 +68,7 @@ def run_served(n_req, slots, precision=None, loads=(0.8,)):
     from euvlitho_amd.serve import Server, poisson_stream, served_configs
     cfgs = served_configs()
     prec = _lib.PREC_MIXED if precision is None else precision
-    srv = Server(n_slots=slots, precision=prec)
+    srv = Server(n_slots=slots, precision=prec, graphs=graphs)
     try:
         t = time.perf_counter()
         srv.warm(cfgs)
@@ -87,10 +88,16 @@ def run_served(n_req, slots, precision=None, loads=(0.8,)):
         res = srv.run(poisson_stream(n_req, rate, seed=2, cfgs=cfgs))
         lat = np.array([r.latency for r in res])
         svc = np.array([r.finish - r.start for r in res])
+        wait = np.array([r.start - r.arrival for r in res])
+        p_wait, w99 = erlang_c_wait(slots, cap, rate)
         return {"value": len(res) / srv.wall, "unit": "clips/s", "requests": len(res),
                 "offered_rate": rate, "burst_capacity": cap, "slots": slots,
                 "p50_ms": 1000.0 * float(np.percentile(lat, 50)), "p99_ms": 1000.0 * float(np.percentile(lat, 99)),
                 "service_p50_ms": 1000.0 * float(np.percentile(svc, 50)),
+                "wait_p99_ms": 1000.0 * float(np.percentile(wait, 99)),
+                "queue_model": {"model": "M/M/c (Erlang C), c = slots, service rate burst_capacity / c, "
+                                         "arrivals at offered_rate",
+                                "p_wait": p_wait, "wait_p99_ms": 1000.0 * w99},
                 "dtype": ("f32 MFMA for the coupling / top-solve products and the imaging; f64 matrix functions, "
                           "multilayer and storage" if prec == _lib.PREC_MIXED else
                           "f32 MFMA for every EM product and the imaging; f64 multilayer and storage"),
@@ -100,11 +107,31 @@ def run_served(n_req, slots, precision=None, loads=(0.8,)):
                 "resident_build_s": warm_s, "peak_reserved_gib": srv.peak_reserved / 2**30,
                 "batches": len(srv.batches), "mean_batch": float(np.mean(srv.batches)) if srv.batches else 0.0,
                 "adaptive_reruns": srv.reruns, "p99_over_p50": float(np.percentile(lat, 99) / np.percentile(lat, 50)),
+                "graphs_captured": sum(len(s.graphs) for s in srv.slots),
+                "graph_recaptures": sum(s.recaptures for s in srv.slots),
                 "load": loads[0], "at_load": curve,
                 "problems": ["%dnm K%d %s %dpts" % (c.ndivx, c.orders[1], "ann" if c.source.kind == 1 else "dip+ann",
                                                        sum(len(x[2]) for x in c.source_points())) for c in cfgs]}
     finally:
         srv.close()
+
+
+def erlang_c_wait(slots, capacity, rate, q=0.99):
+    """Waiting time of an M/M/c queue (Erlang C) with c = slots servers of rate
+    capacity / c each and Poisson arrivals at `rate`: the probability that a
+    request waits, and the q-quantile of the wait (seconds), which is
+    ln(P_wait / (1 - q)) / (c mu - lambda) when P_wait > 1 - q, else 0.  It is
+    the tail that first-come-first-served admission on `slots` streams cannot
+    avoid at that load."""
+    c, mu = int(slots), capacity / slots
+    a = rate / mu
+    if a >= c:
+        return 1.0, float("inf")
+    terms = [a ** k / math.factorial(k) for k in range(c)]
+    top = a ** c / math.factorial(c) * c / (c - a)
+    p_wait = top / (sum(terms) + top)
+    tq = math.log(p_wait / (1.0 - q)) / (c * mu - rate) if p_wait > 1.0 - q else 0.0
+    return p_wait, tq
 
 
 def per_problem(res, cfgs):
@@ -529,6 +556,9 @@ def main():
     ap.add_argument("--served", type=int, default=2000,
                     help="requests of the served Poisson stream (0 = skip; 2000 = the full stream)")
     ap.add_argument("--slots", type=int, default=4, help="streams of the served form")
+    ap.add_argument("--served-only", action="store_true", help="only run the served form (mixed precision)")
+    ap.add_argument("--served-eager", action="store_true",
+                    help="served form with eager plan runs instead of replayed graphs")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy that scales the HBM byte counters")
     ap.add_argument("--no-mask-producer", action="store_true", help="skip the mask producer leg")
@@ -552,6 +582,10 @@ def main():
         return
     if args.mask_producer_only:
         print(json.dumps({"mask_producer": mask_producer(with_cpu=not args.no_cpu_baseline)}), flush=True)
+        return
+    if args.served_only:
+        print(json.dumps({"served": run_served(args.served, args.slots, loads=(0.8, 0.6),
+                                               graphs=not args.served_eager)}), flush=True)
         return
     if args.batched_only:
         print(json.dumps({"batched": run_batched(args.batched)}), flush=True)
@@ -736,8 +770,10 @@ def main():
     if args.batched > 0:
         line["forms"]["batched"] = run_batched(args.batched)
     if args.served > 0:
-        line["forms"]["served"] = run_served(args.served, args.slots, loads=(0.8, 0.6))
-        line["forms"]["served_fast"] = run_served(args.served, args.slots, _lib.PREC_FAST)
+        line["forms"]["served"] = run_served(args.served, args.slots, loads=(0.8, 0.6),
+                                             graphs=not args.served_eager)
+        line["forms"]["served_fast"] = run_served(args.served, args.slots, _lib.PREC_FAST,
+                                                  graphs=not args.served_eager)
     if not args.no_mask_producer:
         line["mask_producer"] = mask_producer(with_cpu=not args.no_cpu_baseline)
     if not args.no_cnn:

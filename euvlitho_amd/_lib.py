@@ -18,6 +18,10 @@ class EuvlError(RuntimeError):
     pass
 
 
+# status of euvl_graph_launch when the handle's workspace moved since the capture (include/euvl.h)
+ERR_STALE = -6
+
+
 # euvl_set_precision bits (include/euvl.h)
 PREC_FP32_MATFUN, PREC_FP32_COUPLING, PREC_FP32_LU, PREC_FP32_GEMM = 1, 2, 4, 8
 PREC_GEMM_4M = 16  # complex products with 4 real MFMAs, FP64 and FP32 (default: 3, the zgemm3m form)

@@ -42,6 +42,12 @@ struct euvl_plan_s {
 struct euvl_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the capture's handle and the device buffers of that handle its nodes address
+  euvl_handle h = nullptr;
+  char* ws = nullptr;
+  size_t ws_cap = 0;
+  char* aux = nullptr;
+  size_t aux_cap = 0;
 };
 
 namespace {
@@ -348,14 +354,25 @@ extern "C" int euvl_graph_end(euvl_handle h, euvl_graph* out) {
     delete g;
     return EUVL_ERR_HIP;
   }
+  g->h = h;
+  g->ws = h->ws;
+  g->ws_cap = h->ws_cap;
+  g->aux = h->aux;
+  g->aux_cap = h->aux_cap;
   *out = g;
   return EUVL_OK;
 }
 
 extern "C" int euvl_graph_launch(euvl_graph g, euvl_handle h) {
-  if (!g || !h) {
-    euvl::set_error("euvl_graph_launch: bad argument");
+  if (!g || !h || g->h != h) {
+    euvl::set_error("euvl_graph_launch: bad argument (a graph replays on the handle it was captured on)");
     return EUVL_ERR_ARG;
+  }
+  // the nodes hold addresses inside the workspace and LU scratch of the
+  // capture: a buffer that moved or shrank since would be read out of bounds
+  if (h->ws != g->ws || h->ws_cap < g->ws_cap || h->aux != g->aux || h->aux_cap < g->aux_cap) {
+    euvl::set_error("euvl_graph_launch: the handle's workspace was reallocated after the capture; capture again");
+    return EUVL_ERR_STALE;
   }
   EUVL_HIP_CHECK(hipGraphLaunch(g->exec, h->stream));
   return EUVL_OK;

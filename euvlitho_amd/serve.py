@@ -118,6 +118,8 @@ class _Slot:
         self.resident = resident
         self.engines = {}
         self.runs = {}  # (cfg, n) -> _Run of a plan for n clips
+        self.graphs = {}  # (cfg, n) -> HIP graph of that run in fixed mode (euvl_graph), when captured
+        self.recaptures = 0
         self.reserved = 0
 
     def bind(self):
@@ -149,7 +151,55 @@ class _Slot:
     def workspace(self):
         return int(self.lib.euvl_workspace_bytes(self.h))
 
+    def reserve_plans(self):
+        """Grow this slot's workspace and LU scratch to what every plan of the
+        slot needs (graphs are captured after this, so that they stay valid)."""
+        plans = [r.plan.pl for r in self.runs.values()]
+        arr = (ctypes.c_void_p * len(plans))(*plans)
+        with torch.cuda.stream(self.stream):
+            self.bind()
+            _lib.check(self.lib.euvl_plan_reserve(self.h, len(plans), arr), "euvl_plan_reserve")
+
+    def capture(self, key):
+        """Capture the fixed-decision run `key` = (cfg, n) of this slot as a HIP
+        graph on the slot's stream (the workspace must be reserved for it)."""
+        r = self.runs[key]
+        with torch.cuda.stream(self.stream):
+            self.bind()
+            _lib.check(self.lib.euvl_graph_begin(self.h), "euvl_graph_begin")
+            g = ctypes.c_void_p()
+            try:
+                r.launch(fixed=True)
+            finally:
+                rc = self.lib.euvl_graph_end(self.h, ctypes.byref(g))
+            _lib.check(rc, "euvl_graph_end")
+        self.graphs[key] = g
+
+    def drop_graphs(self):
+        for g in self.graphs.values():
+            self.lib.euvl_graph_destroy(g)
+        self.graphs = {}
+
+    def launch_fixed(self, key, run):
+        """Enqueue the fixed-decision run on the slot's stream: its graph when
+        one was captured (captured again first if the workspace has moved since),
+        else the plan's kernels one by one."""
+        g = self.graphs.get(key)
+        if g is None:
+            run.launch(fixed=True)
+            return
+        self.bind()
+        rc = self.lib.euvl_graph_launch(g, self.h)
+        if rc == _lib.ERR_STALE:  # every graph of the slot addresses the old workspace
+            self.drop_graphs()
+            self.recaptures += 1
+            self.reserve_plans()
+            self.capture(key)
+            rc = self.lib.euvl_graph_launch(self.graphs[key], self.h)
+        _lib.check(rc, "euvl_graph_launch")
+
     def close(self):
+        self.drop_graphs()
         for r in self.runs.values():
             r.plan.close()
         self.runs = {}
@@ -162,7 +212,7 @@ class _Slot:
 
 class Server:
     def __init__(self, n_slots=4, precision=_lib.PREC_MIXED, hbm_budget=None, device="cuda:0",
-                 socs_kernels=24, socs_xy=20, max_batch=4):
+                 socs_kernels=24, socs_xy=20, max_batch=4, graphs=True):
         if not torch.cuda.is_available():
             raise _lib.EuvlError("a ROCm GPU is required")
         self.device = torch.device(device)
@@ -175,6 +225,7 @@ class Server:
         self.precision = precision
         self.socs_kernels, self.socs_xy = socs_kernels, socs_xy
         self.max_batch = max_batch
+        self.use_graphs = graphs
         self.peak_reserved = 0
         self.reruns = 0
         self.batches = []  # sizes of the batches run by the last run()
@@ -233,6 +284,16 @@ class Server:
         for s in self.slots:
             if self._reserved_total() - s.reserved + need <= self.budget:
                 _lib.check(self.lib_reserve(s, need), "euvl_reserve_workspace")
+                if self.use_graphs:
+                    # every plan of the slot fits the reservation (the footprint covers the plans), so
+                    # its fixed runs are captured once here and replayed by process()
+                    s.reserve_plans()
+                    s.reserved = max(s.reserved, s.workspace())
+                    self.peak_reserved = max(self.peak_reserved, self._reserved_total())
+                    for key in s.runs:
+                        if key not in s.graphs:
+                            s.capture(key)
+        torch.cuda.synchronize(self.device)
 
     def lib_reserve(self, slot, nbytes):
         rc = slot.lib.euvl_reserve_workspace(slot.h, ctypes.c_int64(int(nbytes)))
@@ -256,7 +317,7 @@ class Server:
         with torch.cuda.stream(slot.stream):
             slot.bind()
             run.bits.copy_(torch.from_numpy(np.frombuffer(raw, dtype=np.uint8).copy()))
-            run.launch(fixed=True)
+            slot.launch_fixed((cfg, len(reqs)), run)
             if int(run.flags.max().item()) != 0:  # a device-side check failed: host decisions for this batch
                 self.reruns += 1
                 run.launch(fixed=False)
@@ -292,11 +353,14 @@ class Server:
 
         def pick(slot, now):
             # the oldest arrived request whose batch fits this slot under the budget, with the
-            # later arrived requests of its problem (up to the batch size), largest batch that fits
-            for i, r in enumerate(pending):
-                if r.arrival > now:
-                    break
-                same = [j for j in range(i, len(pending)) if pending[j].arrival <= now and pending[j].cfg == r.cfg]
+            # later arrived requests of its problem (up to the batch size), largest batch that fits;
+            # pending is in arrival order, so the arrived requests are a prefix of it
+            arrived = 0
+            while arrived < len(pending) and pending[arrived].arrival <= now:
+                arrived += 1
+            for i in range(arrived):
+                r = pending[i]
+                same = [j for j in range(i, arrived) if pending[j].cfg == r.cfg]
                 for n in range(min(len(same), self.batch_size(r.cfg)), 0, -1):
                     grow = max(0, footprint(r.cfg, n) - slot.reserved)
                     if self._reserved_total() + grow <= self.budget:
@@ -321,8 +385,8 @@ class Server:
                                     slot.reserved += grow  # claim the growth before releasing the lock
                                     self.peak_reserved = max(self.peak_reserved, self._reserved_total())
                                 break
-                            nxt = [r.arrival for r in pending if r.arrival > now]
-                            cv.wait(timeout=min(0.05, max(1e-4, (min(nxt) - now) if nxt else 0.05)))
+                            nxt = next((r.arrival for r in pending if r.arrival > now), None)
+                            cv.wait(timeout=min(0.05, max(1e-4, (nxt - now) if nxt is not None else 0.05)))
                     if grow:
                         _lib.check(slot.lib.euvl_reserve_workspace(slot.h, ctypes.c_int64(slot.reserved)),
                                    "euvl_reserve_workspace")

@@ -33,6 +33,7 @@ extern "C" {
 #define EUVL_ERR_SINGULAR -3     /* zero pivot in an LU factorisation */
 #define EUVL_ERR_NOT_CONVERGED -4 /* a series or iteration bound failed (cos/sinc norms, Jacobi sweeps) */
 #define EUVL_ERR_NOMEM -5        /* workspace allocation failed */
+#define EUVL_ERR_STALE -6        /* a graph's captured workspace was reallocated (capture it again) */
 
 typedef struct euvl_handle_s* euvl_handle;
 
@@ -478,8 +479,12 @@ int euvl_plan_set_decisions(euvl_plan pl, int doublings);
 
 /* HIP graph of the work enqueued on the handle's stream between begin and
  * end (fixed-mode plan runs; the stream must not be the null stream).  A
- * graph keeps the workspace addresses of the capture: do not grow that
- * handle's workspace while the graph is in use (euvl_plan_reserve first). */
+ * graph keeps the workspace addresses of the capture: reserve first
+ * (euvl_plan_reserve) so that the handle's workspace does not grow while the
+ * graph is in use.  euvl_graph_launch replays it on the handle's current
+ * stream; it returns EUVL_ERR_ARG for another handle than the capture's and
+ * EUVL_ERR_STALE, launching nothing, when that handle's workspace or LU
+ * scratch has been reallocated or shrunk since the capture. */
 int euvl_graph_begin(euvl_handle h);
 int euvl_graph_end(euvl_handle h, euvl_graph* out);
 int euvl_graph_launch(euvl_graph g, euvl_handle h);

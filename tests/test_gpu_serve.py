@@ -4,7 +4,11 @@ every result equals the same request run alone (same precision mode, to
 1e-12 of the peak / largest value), work of different requests overlaps in
 time, the
 admission keeps the reserved device memory under the budget, and the
-footprint estimate covers what the library actually reserves."""
+footprint estimate covers what the library actually reserves.  The slots
+replay HIP graphs of their fixed-decision runs: the results equal eager
+launches, and a graph whose workspace has moved is refused and captured
+again."""
+import ctypes
 import sys
 
 import numpy as np
@@ -98,3 +102,40 @@ def test_admission_respects_budget():
         assert srv.peak_reserved <= budget
     finally:
         srv.close()
+
+
+def test_graph_replay_equals_eager_and_stale_graphs_are_recaptured():
+    from euvlitho_amd import _lib
+    from euvlitho_amd.serve import Server, poisson_stream
+    cfgs = small_mix()
+    reqs = poisson_stream(6, 0.0, seed=6, cfgs=cfgs)
+    out = {}
+    for graphs in (True, False):
+        srv = Server(n_slots=1, precision=_lib.PREC_MIXED, graphs=graphs)
+        try:
+            srv.warm(cfgs)
+            s = srv.slots[0]
+            say(f"graphs={graphs}: {len(s.graphs)} captured runs")
+            assert (len(s.graphs) > 0) == graphs
+            out[graphs] = srv.run(reqs)
+            if graphs:
+                # a workspace smaller than at the capture: the replay is refused (nothing is launched)
+                _lib.check(s.lib.euvl_release_workspace(s.h), "euvl_release_workspace")
+                _lib.check(s.lib.euvl_reserve_workspace(s.h, ctypes.c_int64(1 << 20)), "euvl_reserve_workspace")
+                g = next(iter(s.graphs.values()))
+                assert s.lib.euvl_graph_launch(g, s.h) == _lib.ERR_STALE
+                # serving again captures the runs again and gives the same results
+                again = srv.run(reqs)
+                say(f"  recaptures {s.recaptures}")
+                assert s.recaptures >= 1
+                for a, b in zip(out[True], again):
+                    assert np.array_equal(a.image, b.image) and np.array_equal(a.a0, b.a0)
+                    assert np.array_equal(a.socs, b.socs)
+        finally:
+            srv.close()
+    for a, b in zip(out[True], out[False]):
+        e = np.abs(a.image - b.image).max() / b.image.max()
+        es = np.abs(a.socs - b.socs).max() / np.abs(b.socs).max()
+        e0 = np.abs(a.a0 - b.a0).max() / np.abs(b.a0).max()
+        say(f"  request {a.rid}: graph vs eager image {e:.1e}, socs {es:.1e}, a0 {e0:.1e}")
+        assert e <= 1e-12 and es <= 1e-12 and e0 <= 1e-12

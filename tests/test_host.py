@@ -144,3 +144,19 @@ def test_bench_defines_every_leg():
     import builtins
     missing = [c for c in called if not hasattr(bench, c) and not hasattr(builtins, c) and c not in local]
     assert not missing, missing
+
+
+def test_erlang_c_queue_model():
+    """The served leg's queue model: with one server it is M/M/1 (the wait
+    exceeds t with probability rho exp(-(mu - lambda) t)); with c servers
+    P_wait matches the Erlang C value of a textbook case (c = 2, a = 1: 1/3),
+    and a load below the quantile's mass gives no wait."""
+    import math
+    import bench
+    p, t = bench.erlang_c_wait(1, 10.0, 8.0)
+    assert abs(p - 0.8) < 1e-12 and abs(t - math.log(0.8 / 0.01) / 2.0) < 1e-12
+    p, t = bench.erlang_c_wait(2, 2.0, 1.0)
+    assert abs(p - 1.0 / 3.0) < 1e-12 and abs(t - math.log((1.0 / 3.0) / 0.01) / 1.0) < 1e-12
+    p, t = bench.erlang_c_wait(4, 100.0, 1.0)
+    assert p < 0.01 and t == 0.0
+    assert bench.erlang_c_wait(4, 10.0, 12.0)[1] == float("inf")
