@@ -655,13 +655,44 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
                      make_double2(eai.real(), eai.imag()), make_double2(lea.real(), lea.imag()), v, D, G);
   EUVL_LAUNCH_CHECK();
 
+  // Where the order grid fits a workgroup's LDS, the products by D (D is
+  // Toeplitz in the order differences up to diagonal scalings) and the
+  // coupling product H G run as FFT correlations with the clip's spectrum;
+  // one spectrum buffer serves both (the coupling product refills it).
+  FftGrid fg;
+  const bool grid_fits = fft_grid_for(lmax, mmax, fg);
+  double2* Ssp = grid_fits ? (double2*)ws_take(h, coupling_fft_bytes(B)) : nullptr;
+  if (grid_fits && !Ssp) {
+    set_error("euvl_amp_s: workspace carve failed");
+    return EUVL_ERR_NOMEM;
+  }
+  XdOp xd;
+  const bool use_xd = grid_fits && xd_fft_fits(N);
+  if (use_xd) {
+    const double c = 2.0 * PI / (polar == 'X' ? p->dx : p->dy);
+    const std::complex<double> a_p = k * k * (1.0 - ea), b_u = c * lea, a_x = k * k * ea;
+    xd.g = fg;
+    xd.N = N;
+    xd.lmax = lmax;
+    xd.lidx = t.lidx;
+    xd.midx = t.midx;
+    xd.uidx = polar == 'X' ? t.lidx : t.midx;
+    xd.kt = v.kt;
+    xd.kxy2 = v.kxy2;
+    xd.a_p = make_double2(a_p.real(), a_p.imag());
+    xd.b_u = make_double2(b_u.real(), b_u.imag());
+    xd.a_x = make_double2(a_x.real(), a_x.imag());
+    xd.S = Ssp;
+    EUVL_TRY(xd_fft_spectra(h, fg, lmax, mmax, spec, per_clip, spec_stride, B, Ssp));
+  }
+
   // C = cos(d sqrt D), F = sin(d sqrt D)/sqrt D, G2 = D F (6 scratch matrices from the workspace)
   double2* C = Mx[2];
   double2* Fs = Mx[3];
   double2* G2c = Mx[4];
   int ndbl = 0;
   EUVL_TRY(cos_sinc(h, N, B, D, p->dabs, C, Fs, G2c, &ndbl, ctl.fixed ? ctl.doublings : -1, red, ctl.d_flags,
-                    &ctl.alpha));
+                    &ctl.alpha, use_xd ? &xd : nullptr));
 
   // The top solve.  With M = I - G (G = diag(kt) sigma^ diag(kt) / k^2),
   // S = diag(b2 (uu + ub) / (i k)) and L2 = diag(alru (uu - ub)), the layer
@@ -680,13 +711,7 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   EUVL_LAUNCH_CHECK();
   // V = P + H - H G: by FFTs over the order grid when the grid fits a
   // workgroup's LDS (G is sigma^ Toeplitz in the order differences), else a GEMM
-  FftGrid fg;
-  if (fft_grid_for(lmax, mmax, fg)) {
-    double2* Ssp = (double2*)ws_take(h, coupling_fft_bytes(B));
-    if (!Ssp) {
-      set_error("euvl_amp_s: workspace carve failed");
-      return EUVL_ERR_NOMEM;
-    }
+  if (grid_fits) {
     EUVL_TRY(coupling_fft(h, fg, N, B, t.lidx, t.midx, lmax, mmax, spec, per_clip, spec_stride,
                           make_double2(eai.real(), eai.imag()), v.kt, k, H, V, ldv, sv, Ssp));
   } else {

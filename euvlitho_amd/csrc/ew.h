@@ -59,9 +59,12 @@ constexpr int FLAG_ZERO_PIVOT = 8;   // the unpivoted LU met a zero pivot
 // fixed_s < 0: the doubling count is chosen from ||D^3||, ||D^4|| read back to the host (*doublings
 // receives it).  fixed_s >= 0: that count is used without a host read-back; the norms go to
 // d_red[0, 2 batch) and entries whose bound needs more doublings get FLAG_DOUBLINGS in d_flags.
+// xd (optional): D is the EM solve's absorber operator and its right products
+// (the powers D^2..D^4 and G2 = F D, which equals D F) run by FFTs (xd_fft).
+struct XdOp;
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
              int* doublings, int fixed_s = -1, double* d_red = nullptr, int* d_flags = nullptr,
-             double* alpha_out = nullptr);
+             double* alpha_out = nullptr, const XdOp* xd = nullptr);
 // the theta of the doubling bound in the current precision scope
 double cos_sinc_theta(euvl_handle h);
 
@@ -111,6 +114,30 @@ size_t coupling_fft_bytes(int E);
 int coupling_fft(euvl_handle h, const FftGrid& g, int N, int E, const int* lidx, const int* midx, int lmax, int mmax,
                  const double2* W, int per_clip, int64_t w_stride, double2 ea_inv, const double* kt, double k,
                  const double2* H, double2* V, int64_t ldv, int64_t sv, double2* S);
+// Right products Y = X D by the absorber operator D of the EM solve (the
+// matrix of toeplitz_kernel), by 2D FFTs over the order grid (fft.hip).  With
+// m^ the clip's indicator spectrum, u the order index along the
+// polarisation's axis (l for X, m for Y) and c = 2 pi / dx (dy),
+//   D[i][j] = k^2 (1 - ea) m^(i - j) + k^2 ea delta_ij
+//             - c log(ea) (u_i - u_j) m^(i - j) kt_j - delta_ij kxy2_j,
+// so row r of X D needs the correlations of x = X[r][:] and of u x with m^.
+// a_p = k^2 (1 - ea), b_u = c log(ea), a_x = k^2 ea; S: the spectra of m^
+// (xd_fft_spectra, coupling_fft_bytes(E)); kt, kxy2: batch x N.
+struct XdOp {
+  FftGrid g;
+  int N = 0, lmax = 0;
+  const int* lidx = nullptr;
+  const int* midx = nullptr;
+  const int* uidx = nullptr;
+  const double* kt = nullptr;
+  const double* kxy2 = nullptr;
+  double2 a_p, b_u, a_x;
+  const double2* S = nullptr;
+};
+bool xd_fft_fits(int N);
+int xd_fft_spectra(euvl_handle h, const FftGrid& g, int lmax, int mmax, const double2* W, int per_clip,
+                   int64_t w_stride, int E, double2* S);
+int xd_fft(euvl_handle h, const XdOp& op, int E, const double2* X, double2* Y);
 int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDev& t, int N, int lmax, int mmax,
                const double2* spec, int per_clip, int64_t spec_stride, int E, double2* Ax, SolveCtl& ctl);
 // indicator spectrum of n_clips packed masks on a window |l| <= lw, |m| <= mw, with the

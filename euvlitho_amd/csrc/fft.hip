@@ -30,15 +30,31 @@ constexpr int FFT_MAX_PTS = 8192;  // Pl Pm: the grid in LDS (128 KiB)
 constexpr int FFT_MAX_P = 128;     // points along one axis
 constexpr int FFT_THREADS = 1024;
 
-__device__ __forceinline__ double2 root_of(const double2* root, int t, bool inv) {
-  const double2 w = root[t];
+// The grid and the root tables live in LDS; the out-of-line passes receive
+// them as generic pointers and cast them back, so that their accesses are
+// LDS instructions with 32-bit addresses rather than flat ones.
+// (Element access through double components: the vector class has no
+// constructors for another address space.)
+typedef __attribute__((address_space(3))) double lds_f64;
+
+__device__ __forceinline__ double2 lds_ld(const lds_f64* p, int i) { return make_double2(p[2 * i], p[2 * i + 1]); }
+__device__ __forceinline__ void lds_st(lds_f64* p, int i, double2 v) {
+  p[2 * i] = v.x;
+  p[2 * i + 1] = v.y;
+}
+
+__device__ __forceinline__ double2 root_of(const lds_f64* root, int t, bool inv) {
+  const double2 w = lds_ld(root, t);
   return inv ? make_double2(w.x, -w.y) : w;
 }
 
-// a / b for 0 <= a < 2^13 and 0 < b <= 2^7 from rb = 1/b in FP32: (a + 1/2) / b
-// is at least 1/(2b) from an integer and the product's error is below 2^-11,
-// so the truncation is exact (and far cheaper than an integer division)
+// a / b for 0 <= a < 2^13 and 0 < b <= 2^7 from rb ~ 1/b in FP32 (v_rcp_f32,
+// within 1 ulp): (a + 1/2) / b is at least 1/(2b) from an integer, and the
+// product's relative error is below 2^-22, i.e. its absolute error below
+// 2^13 2^-22 / b < 1/(2b), so the truncation is exact (and far cheaper than
+// an integer division)
 __device__ __forceinline__ int fdiv(int a, float rb) { return (int)(((float)a + 0.5f) * rb); }
+__device__ __forceinline__ float recip(int b) { return __builtin_amdgcn_rcpf((float)b); }
 
 // y = the R-point DFT of x, e^(-+2 pi i r k / R), R = 2, 3, 4, 9, with its
 // constant roots (9 as 3 x 3: DFTs of the three stride-3 subsequences,
@@ -100,13 +116,18 @@ __device__ __forceinline__ void dft_small(const double2* x, double2* y, bool inv
 // w = e^(-+2 pi i / (Ns R)), takes the R-point DFT and writes output r to
 // (j / Ns) Ns R + (j mod Ns) + r Ns (Ns: the product of the radices before).
 // (Out of line: inlined into the four axis loops of a kernel it spills.)
-template <int R>
-__device__ __attribute__((noinline)) void fft_pass(double2* X, const double2* root, int P, int Ns, bool inv,
+// INV: the inverse direction; FIRST: the first pass of an axis (Ns = 1, every
+// twiddle is 1).
+template <int R, bool INV, bool FIRST>
+__device__ __attribute__((noinline)) void fft_pass(double2* Xg, const double2* rootg, int P, int Ns,
                                                    bool along_m, int nlines, int Pl, int Pm, int keep) {
+  constexpr bool inv = INV;
   constexpr int MAXB = (FFT_MAX_PTS / R + FFT_THREADS - 1) / FFT_THREADS;
+  lds_f64* X = (lds_f64*)Xg;
+  const lds_f64* root = (const lds_f64*)rootg;
   const int per = P / R, total = nlines * per, stride = along_m ? 1 : Pm;
   const int step = P / (Ns * R);
-  const float rper = 1.0f / (float)per, rlines = 1.0f / (float)nlines, rns = 1.0f / (float)Ns;
+  const float rper = recip(per), rlines = recip(nlines), rns = recip(Ns);
   double2 v[MAXB][R];
   int dst[MAXB];
 #pragma unroll
@@ -123,12 +144,12 @@ __device__ __attribute__((noinline)) void fft_pass(double2* X, const double2* ro
         line = beta - j * nlines;
       }
       const int base = along_m ? ((keep > 0 && line > keep / 2) ? Pl - keep + line : line) * Pm : line;
-      const int jq = fdiv(j, rns), jn = j - jq * Ns;
+      const int jq = FIRST ? j : fdiv(j, rns), jn = FIRST ? 0 : j - jq * Ns;
       double2 x[R];
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double2 e = X[base + (j + r * per) * stride];
-        x[r] = r ? cmul(e, root_of(root, r * jn * step, inv)) : e;  // r jn step < P
+        const double2 e = lds_ld(X, base + (j + r * per) * stride);
+        x[r] = (r && !FIRST) ? cmul(e, root_of(root, r * jn * step, inv)) : e;  // r jn step < P
       }
       dft_small<R>(x, v[u], inv);
       dst[u] = base + (jq * Ns * R + jn) * stride;
@@ -139,7 +160,7 @@ __device__ __attribute__((noinline)) void fft_pass(double2* X, const double2* ro
   for (int u = 0; u < MAXB; ++u)
     if (dst[u] >= 0)
 #pragma unroll
-      for (int r = 0; r < R; ++r) X[dst[u] + r * Ns * stride] = v[u][r];
+      for (int r = 0; r < R; ++r) lds_st(X, dst[u] + r * Ns * stride, v[u][r]);
   __syncthreads();
 }
 
@@ -151,12 +172,27 @@ __device__ void fft_axis(double2* X, const double2* root, const euvl::FftGrid& g
   int Ns = 1;
   for (int s = 0; s < g.nst[axis]; ++s) {
     const int R = g.rad[axis][s];
-    switch (R) {
-      case 2: fft_pass<2>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
-      case 3: fft_pass<3>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
-      case 9: fft_pass<9>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
-      default: fft_pass<4>(X, root, P, Ns, inv, along_m, nlines, g.P[0], g.P[1], keep); break;
+#define EUVL_FFT_PASS(RR, INV)                                                        \
+  if (s == 0)                                                                        \
+    fft_pass<RR, INV, true>(X, root, P, Ns, along_m, nlines, g.P[0], g.P[1], keep);  \
+  else                                                                               \
+    fft_pass<RR, INV, false>(X, root, P, Ns, along_m, nlines, g.P[0], g.P[1], keep);
+    if (inv) {
+      switch (R) {
+        case 2: EUVL_FFT_PASS(2, true) break;
+        case 3: EUVL_FFT_PASS(3, true) break;
+        case 9: EUVL_FFT_PASS(9, true) break;
+        default: EUVL_FFT_PASS(4, true) break;
+      }
+    } else {
+      switch (R) {
+        case 2: EUVL_FFT_PASS(2, false) break;
+        case 3: EUVL_FFT_PASS(3, false) break;
+        case 9: EUVL_FFT_PASS(9, false) break;
+        default: EUVL_FFT_PASS(4, false) break;
+      }
     }
+#undef EUVL_FFT_PASS
     Ns *= R;
   }
 }
@@ -237,6 +273,75 @@ __global__ __launch_bounds__(FFT_THREADS) void coupling_fft_kernel(euvl::FftGrid
   }
 }
 
+// Y[b][i][:] = (X D)[b][i][:] for the absorber operator D of the EM solve,
+// one workgroup per (i, b).  With x = X[b][i][:], P = x * m^ and
+// Q = (u x) * m^ (correlations with the indicator spectrum over the order
+// grid, as in the coupling product), row i of X D is
+//   y_j = (a_p + b_u kt_j u_j) P_j - b_u kt_j Q_j + (a_x - kxy2_j) x_j.
+// The two correlations share the spectrum S of m^; each is a forward and an
+// inverse 2D transform of the grid in LDS, and P, Q are kept in registers
+// (outputs j = thread + t FFT_THREADS, t < XD_PER).
+constexpr int XD_PER = 2;
+
+__global__ __launch_bounds__(FFT_THREADS) void xd_fft_kernel(euvl::FftGrid g, int N, int lmax,
+                                                             const int* __restrict__ lidx,
+                                                             const int* __restrict__ midx,
+                                                             const int* __restrict__ uidx,
+                                                             const double* __restrict__ kt,
+                                                             const double* __restrict__ kxy2, double2 a_p,
+                                                             double2 b_u, double2 a_x, double scale,
+                                                             const double2* __restrict__ X, int64_t sx,
+                                                             const double2* __restrict__ S, double2* __restrict__ Y,
+                                                             int64_t sy) {
+  __shared__ double2 G[FFT_MAX_PTS];
+  __shared__ double2 root[2][FFT_MAX_P];
+  const int i = blockIdx.x, b = blockIdx.y, Pl = g.P[0], Pm = g.P[1], n = Pl * Pm;
+  const int keep = (2 * lmax + 1 < Pl) ? 2 * lmax + 1 : 0;
+  const int rows = keep ? keep : Pl;
+  fill_roots(g, root);
+  const double2* Xi = X + (int64_t)b * sx + (int64_t)i * N;
+  const double2* Sb = S + (int64_t)b * n;
+  double2 P[XD_PER], Q[XD_PER];
+#pragma unroll 1
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int e = threadIdx.x; e < n; e += blockDim.x) G[e] = make_double2(0.0, 0.0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+      const double2 x = Xi[q];
+      G[wrap(lidx[q], Pl) * Pm + wrap(midx[q], Pm)] = pass ? cscale(x, (double)uidx[q]) : x;
+    }
+    __syncthreads();
+    fft_axis(G, root[1], g, 1, false, rows, keep);
+    fft_axis(G, root[0], g, 0, false, Pm, 0);
+    for (int e = threadIdx.x; e < n; e += blockDim.x) G[e] = cmul(G[e], Sb[e]);
+    __syncthreads();
+    fft_axis(G, root[0], g, 0, true, Pm, 0);
+    fft_axis(G, root[1], g, 1, true, rows, keep);
+#pragma unroll
+    for (int t = 0; t < XD_PER; ++t) {
+      const int j = threadIdx.x + t * FFT_THREADS;
+      const double2 c = j < N ? G[wrap(lidx[j], Pl) * Pm + wrap(midx[j], Pm)] : make_double2(0.0, 0.0);
+      if (pass)
+        Q[t] = c;
+      else
+        P[t] = c;
+    }
+    __syncthreads();  // every read of the grid precedes the next pass's clearing
+  }
+  double2* Yi = Y + (int64_t)b * sy + (int64_t)i * N;
+#pragma unroll
+  for (int t = 0; t < XD_PER; ++t) {
+    const int j = threadIdx.x + t * FFT_THREADS;
+    if (j < N) {
+      const double ktj = kt[(int64_t)b * N + j];
+      const double2 cp = cadd(a_p, cscale(b_u, ktj * (double)uidx[j]));
+      const double2 cq = cscale(b_u, -ktj);
+      const double2 cx = make_double2(a_x.x - kxy2[(int64_t)b * N + j], a_x.y);
+      Yi[j] = cadd(cscale(cadd(cmul(cp, P[t]), cmul(cq, Q[t])), scale), cmul(cx, Xi[j]));
+    }
+  }
+}
+
 // smallest size >= lo with no prime factor above 3, and its radices (nines
 // first, then fours, a two, threes); 0 when above FFT_MAX_P
 int fft_size(int lo, int* rad, int* nst) {
@@ -276,6 +381,30 @@ int coupling_fft(euvl_handle h, const FftGrid& g, int N, int E, const int* lidx,
   const double scale = 1.0 / (k * k * (double)g.P[0] * (double)g.P[1]);
   hipLaunchKernelGGL(coupling_fft_kernel, dim3(N, E), dim3(FFT_THREADS), 0, h->stream, g, N, lmax, lidx, midx, kt,
                      scale, H, (const double2*)S, V, ldv, sv);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+bool xd_fft_fits(int N) { return N <= XD_PER * FFT_THREADS; }
+
+int xd_fft_spectra(euvl_handle h, const FftGrid& g, int lmax, int mmax, const double2* W, int per_clip,
+                   int64_t w_stride, int E, double2* S) {
+  // the spectrum of m^ alone: sigma_spectrum_kernel with ea^-1 = 0
+  hipLaunchKernelGGL(sigma_spectrum_kernel, dim3(E), dim3(FFT_THREADS), 0, h->stream, g, lmax, mmax, W, per_clip,
+                     w_stride, make_double2(0.0, 0.0), S);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int xd_fft(euvl_handle h, const XdOp& op, int E, const double2* X, double2* Y) {
+  if (!xd_fft_fits(op.N) || X == Y) {
+    set_error("xd_fft: N = %d above %d, or Y aliases X", op.N, XD_PER * FFT_THREADS);
+    return EUVL_ERR_ARG;
+  }
+  const double scale = 1.0 / ((double)op.g.P[0] * (double)op.g.P[1]);
+  const int64_t sm = (int64_t)op.N * op.N;
+  hipLaunchKernelGGL(xd_fft_kernel, dim3(op.N, E), dim3(FFT_THREADS), 0, h->stream, op.g, op.N, op.lmax, op.lidx,
+                     op.midx, op.uidx, op.kt, op.kxy2, op.a_p, op.b_u, op.a_x, scale, X, sm, op.S, Y, sm);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

@@ -16,7 +16,9 @@ inline double2 re(double x) { return make_double2(x, 0.0); }
 
 namespace euvl {
 
-// D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.
+// D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.  With xd, the
+// products with D as a factor (D^2, D^3, D^4, G2) are FFT correlations over
+// the order grid (xd_fft) and the other twelve stay MFMA GEMMs.
 double cos_sinc_theta(euvl_handle h) {
   // The degree-12 remainder is bounded by (h^2 alpha)^13 / 26!: h^2 alpha <= 5.9
   // keeps it below 2.6e-17 (FP64 products); with FP32 products (the precision
@@ -29,7 +31,7 @@ double cos_sinc_theta(euvl_handle h) {
 }
 
 int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double2* C, double2* F, double2* G2,
-             int* doublings, int fixed_s, double* d_red, int* d_flags, double* alpha_out) {
+             int* doublings, int fixed_s, double* d_red, int* d_flags, double* alpha_out, const XdOp* xd) {
   const double theta = cos_sinc_theta(h);
   PrecScope scope(h, EUVL_PREC_FP32_MATFUN);
   const size_t mat = sizeof(double2) * (size_t)n * n * batch;
@@ -51,9 +53,15 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   // below rounding.
   const double2* X1 = D;
   double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *X4 = s[5];
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
-  EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, X2, n, sm, ZERO, X4, n, sm, ZERO, batch));
+  if (xd) {  // D^2 = D D, D^3 = D^2 D, D^4 = D^3 D: right products by D over the order grid
+    EUVL_TRY(xd_fft(h, *xd, batch, D, X2));
+    EUVL_TRY(xd_fft(h, *xd, batch, X2, X3));
+    EUVL_TRY(xd_fft(h, *xd, batch, X3, X4));
+  } else {
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
+    EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, X2, n, sm, ZERO, X4, n, sm, ZERO, batch));
+  }
   int sq = 0;
   if (fixed_s >= 0) {
     // the bound is checked on the device; no read-back
@@ -137,7 +145,12 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     if (c != C) EUVL_TRY(ew_copy(h, sm * batch, C, c));
     if (f != F) EUVL_TRY(ew_copy(h, sm * batch, F, f));
   }
-  if (G2) EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, F, n, sm, ZERO, G2, n, sm, ZERO, batch));
+  if (G2) {
+    if (xd)  // F is a power series in D, so D F = F D
+      EUVL_TRY(xd_fft(h, *xd, batch, F, G2));
+    else
+      EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, F, n, sm, ZERO, G2, n, sm, ZERO, batch));
+  }
   return EUVL_OK;
 }
 
