@@ -23,6 +23,10 @@ int ew_diag_add(euvl_handle h, int n, int batch, double2* X, double2 g);
 int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const double2* X2, const double2* X3,
                  const double2* X4, const double* c, const double* f, double2* Hc, double2* Qc, double2* Oc,
                  double2* Hf, double2* Qf, double2* Of);
+// the two blocks per polynomial in powers of D^6 from X[0..5] = D..D^6 (ps6_blocks_kernel):
+// q = O + D^6 T; Tc and Tf may be the buffers of X[1] and X[2]
+int ew_ps6_blocks(euvl_handle h, int n, int batch, const double2* const* X, const double* c, const double* f,
+                  double2* Oc, double2* Tc, double2* Of, double2* Tf);
 int ew_copy(euvl_handle h, int64_t count, double2* out, const double2* X);
 // out[0:count) = v (a fill kernel: safe inside graph capture, unlike hipMemsetAsync)
 int fill(euvl_handle h, int64_t count, double2* out, double2 v);

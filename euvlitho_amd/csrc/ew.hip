@@ -49,6 +49,38 @@ __global__ void ps_blocks_kernel(int n, const double2* X1, const double2* X2, co
   }
 }
 
+// The two blocks of each polynomial in powers of X6 = D^6, from one read of
+// X1..X6 (D .. D^6): O = c5 X5 + .. + c1 X1 + c0 I (written to Oc, Of) and
+// T = c12 X6 + c11 X5 + .. + c7 X1 + c6 I (written to Tc, Tf), so that
+// q = O + X6 T.  Each element is read before any output is written at its
+// index, so Tc and Tf may be the buffers of X2 and X3.
+__global__ void ps6_blocks_kernel(int n, const double2* X1, const double2* X2, const double2* X3, const double2* X4,
+                                  const double2* X5, const double2* X6, PsCoeffs k, double2* Oc, double2* Tc,
+                                  double2* Of, double2* Tf) {
+  const uint32_t per = (uint32_t)n * (uint32_t)n;
+  const int64_t base = (int64_t)blockIdx.y * per;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < per; r += gridDim.x * blockDim.x) {
+    const int64_t e = base + r;
+    const double2 x[6] = {X1[e], X2[e], X3[e], X4[e], X5[e], X6[e]};
+    const double id = (r % ((uint32_t)n + 1) == 0) ? 1.0 : 0.0;
+    auto comb = [&](const double* c, int j, int top) {  // c[j] I + sum_{t=1..top} c[j+t] X_t
+      double re = c[j] * id, im = 0.0;
+#pragma unroll
+      for (int t = 1; t <= 6; ++t)
+        if (t <= top) {
+          re += c[j + t] * x[t - 1].x;
+          im += c[j + t] * x[t - 1].y;
+        }
+      return make_double2(re, im);
+    };
+    const double2 oc = comb(k.c, 0, 5), tc = comb(k.c, 6, 6), of = comb(k.f, 0, 5), tf = comb(k.f, 6, 6);
+    Oc[e] = oc;
+    Tc[e] = tc;
+    Of[e] = of;
+    Tf[e] = tf;
+  }
+}
+
 __global__ void diag_add_kernel(int n, int batch, double2* X, double2 g) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)n * batch) return;
@@ -229,6 +261,20 @@ int ew_ps_blocks(euvl_handle h, int n, int batch, const double2* X1, const doubl
   }
   hipLaunchKernelGGL(ps_blocks_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, X1, X2, X3, X4,
                      k, Hc, Qc, Oc, Hf, Qf, Of);
+  EUVL_LAUNCH_CHECK();
+  return EUVL_OK;
+}
+
+int ew_ps6_blocks(euvl_handle h, int n, int batch, const double2* const* X, const double* c, const double* f,
+                  double2* Oc, double2* Tc, double2* Of, double2* Tf) {
+  EUVL_TRY(check_ew(n, batch));
+  PsCoeffs k;
+  for (int j = 0; j < 13; ++j) {
+    k.c[j] = c[j];
+    k.f[j] = f[j];
+  }
+  hipLaunchKernelGGL(ps6_blocks_kernel, ew_grid2((int64_t)n * n, batch), dim3(256), 0, h->stream, n, X[0], X[1],
+                     X[2], X[3], X[4], X[5], k, Oc, Tc, Of, Tf);
   EUVL_LAUNCH_CHECK();
   return EUVL_OK;
 }

@@ -51,12 +51,21 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
   // Lemma 4.1), which is often well below ||D||: s is the smallest with
   // h^2 alpha <= theta, so that the remainder of both series (terms j >= 13) is
   // below rounding.
+  //
+  // With xd (C and F contiguous) the products by D are cheap FFT correlations,
+  // so the blocks are taken in powers of D^6 instead: D^2 .. D^6 by xd_fft
+  // (D^(j+1) = D^j D) and q = O + D^6 T, one paired GEMM for both series in
+  // place of the two below.
+  if (xd && F != C + sm * batch) xd = nullptr;
   const double2* X1 = D;
   double2 *X2 = s[1], *X3 = s[2], *H = s[3], *Q = s[4], *X4 = s[5];
-  if (xd) {  // D^2 = D D, D^3 = D^2 D, D^4 = D^3 D: right products by D over the order grid
-    EUVL_TRY(xd_fft(h, *xd, batch, D, X2));
-    EUVL_TRY(xd_fft(h, *xd, batch, X2, X3));
-    EUVL_TRY(xd_fft(h, *xd, batch, X3, X4));
+  if (xd) {
+    X4 = s[3];
+    const double2* in = D;
+    for (int j = 1; j <= 5; ++j) {  // s[j] = D^(j+1)
+      EUVL_TRY(xd_fft(h, *xd, batch, in, s[j]));
+      in = s[j];
+    }
   } else {
     EUVL_TRY(zgemm(h, n, n, n, re(1.0), D, n, sm, D, n, sm, ZERO, X2, n, sm, ZERO, batch));
     EUVL_TRY(zgemm(h, n, n, n, re(1.0), X2, n, sm, D, n, sm, ZERO, X3, n, sm, ZERO, batch));
@@ -105,10 +114,25 @@ int cos_sinc(euvl_handle h, int n, int batch, const double2* D, double d, double
     else
       cf[m / 2] = sgn * hh * scale / fact;
   }
-  // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2).
-  // The blocks of both polynomials come from one pass over D..P4 (the sinc
-  // Q1 over P2, which is not read after it); then two products each.
-  if (F == C + sm * batch) {
+  if (xd) {
+    // O (into C, F) and T (over D^2, D^3: contiguous, the right operands of
+    // the paired launch), then [C; F] += D^6 [Tc; Tf]
+    const double2* P[6] = {D, s[1], s[2], s[3], s[4], s[5]};
+    EUVL_TRY(ew_ps6_blocks(h, n, batch, P, cc, cf, C, s[1], F, s[2]));
+    EUVL_TRY(zgemm_periodic(h, n, n, n, re(1.0), s[5], n, sm, s[1], n, sm, re(1.0), C, n, sm, ZERO, 2 * batch,
+                            batch, 0));
+    // doublings as below
+    double2 *c = C, *t = s[0];
+    for (int q = 0; q < sq; ++q) {
+      EUVL_TRY(zgemm_periodic(h, n, n, n, re(2.0), c, n, sm, c, n, sm, ZERO, t, n, sm, ZERO, 2 * batch, 0, batch));
+      EUVL_TRY(ew_diag_add(h, n, batch, t, re(-1.0)));
+      std::swap(c, t);
+    }
+    if (c != C) EUVL_TRY(ew_copy(h, 2 * sm * batch, C, c));
+  } else if (F == C + sm * batch) {
+    // q = Q0 + P4 (Q1 + P4 Q2), Qi = c_4i I + c_4i+1 D + c_4i+2 P2 + c_4i+3 P3 (+ c12 P4 in Q2).
+    // The blocks of both polynomials come from one pass over D..P4 (the sinc
+    // Q1 over P2, which is not read after it); then two products each.
     // C and F contiguous: the cos and sinc products of a step share their
     // left operand (P4, or C in a doubling) and run as one launch of 2 batch
     // entries (the last batch ones read the same operand again); blocks
