@@ -210,7 +210,19 @@ class Config:
 
     # ---- source lattice (intensity.cpp:467-502) ----
     def source_points(self):
-        """List of (nsx, nsy, l0s array, m0s array) per division."""
+        """(nsx, nsy, l0s array, m0s array) per division; computed once per
+        configuration (the arrays are read-only)."""
+        return self._source_points
+
+    @cached_property
+    def _source_points(self):
+        out = self._lattice_points()
+        for _, _, l0, m0 in out:
+            l0.flags.writeable = False
+            m0.flags.writeable = False
+        return tuple(out)
+
+    def _lattice_points(self):
         nd = self.ndivs
         k, NA = self.k, self.na
         mem = self.source.members()

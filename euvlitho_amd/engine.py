@@ -80,7 +80,24 @@ class Engine:
             eps.ctypes.data_as(_lib.P_double), thk.ctypes.data_as(_lib.P_double))
         self.lidx, self.midx, self.L, self.M = cfg.order_set()
         self.modemap = Config.mode_map(self.lidx, self.midx, self.L, self.M)
+        self._modemap_c = _lib.arr_i(self.modemap)
+        self._abbe_tab = None  # host tables of the Abbe image (built on first use)
         self.last_info = None
+
+    def _abbe_tables(self):
+        """(n_off, offsets, n_src, (division, l0, m0) per source point, n_pupil,
+        pupil (l, m)) as ctypes-ready arrays, built once per engine."""
+        if self._abbe_tab is None:
+            cfg = self.cfg
+            offs = cfg.offsets()
+            src = []
+            for o, (_, _, l0, m0) in enumerate(cfg.source_points()):
+                src += [(o, int(a), int(b)) for a, b in zip(l0, m0)]
+            src = np.array(src, dtype=np.int32).reshape(-1, 3)
+            lin, min_, _ = cfg.pupil
+            self._abbe_tab = (len(offs), _lib.arr_d(offs), len(src), _lib.arr_i(src), len(lin),
+                              _lib.arr_i(np.stack([lin, min_], axis=1)))
+        return self._abbe_tab
 
     def close(self):
         if self.h and self._owns:
@@ -220,7 +237,7 @@ class Engine:
         a0 = torch.empty(ncut, dtype=torch.complex128, device=self.device)
         ax = torch.empty_like(a0)
         ay = torch.empty_like(a0)
-        mm, mmp = _lib.arr_i(self.modemap)
+        mmp = self._modemap_c[1]
         pu, pup_p = _lib.arr_i(pup)
         vc, vcp = _lib.arr_d(np.stack([vcxx.real, vcxx.imag], -1))
         ab, abp = _lib.arr_d(np.stack([abxx.real, abxx.imag], -1))
@@ -244,25 +261,15 @@ class Engine:
             Ax = self.amp_s(bits, polar)
         if np.ndim(z) > 0:
             return torch.stack([self.abbe(bits, float(zz), Ax, polar) for zz in z])
-        offs = cfg.offsets()
-        src = []
-        divs = cfg.source_points()
-        for o, (nsx, nsy, l0, m0) in enumerate(divs):
-            src += [(o, int(a), int(b)) for a, b in zip(l0, m0)]
-        src = np.array(src, dtype=np.int32).reshape(-1, 3)
-        lin, min_, _ = cfg.pupil
-        pup = np.stack([lin, min_], axis=1)
+        n_off, (_, ofp), n_src, (_, srp), n_pup, (_, pup_p) = self._abbe_tables()
         nx, ny = cfg.image_shape
         img = torch.zeros((nx, ny), dtype=torch.float64, device=self.device)
-        mm, mmp = _lib.arr_i(self.modemap)
-        of, ofp = _lib.arr_d(offs)
-        sr, srp = _lib.arr_i(src)
-        pu, pup_p = _lib.arr_i(pup)
+        mmp = self._modemap_c[1]
         self._bind_stream()
         _lib.check(self.lib.euvl_abbe_image(self.h, polar.encode(), ctypes.byref(self.params), _lib.ptr(Ax),
                                             len(self.lidx), self.L,
-                                            self.M, mmp, len(offs), ofp, len(src), srp, len(lin), pup_p, cfg.kx0,
-                                            cfg.ky0, cfg.z0, float(z), nx, ny, 1.0 / max(len(src), 1),
+                                            self.M, mmp, n_off, ofp, n_src, srp, n_pup, pup_p, cfg.kx0,
+                                            cfg.ky0, cfg.z0, float(z), nx, ny, 1.0 / max(n_src, 1),
                                             _lib.ptr(img)), "euvl_abbe_image")
         return img
 
