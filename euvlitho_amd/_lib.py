@@ -75,6 +75,7 @@ _SIGS = {
     "euvl_gemm_stats": [c_void_p, c_int, P_double],
     "euvl_set_precision": [c_void_p, c_int],
     "euvl_autotune_gemm": [c_void_p, c_int, c_int, c_int, c_int, c_int, P_int, P_double],
+    "euvl_set_fft_min_n": [c_int, c_int],
     "euvl_zgemm_batched": [c_void_p, c_int, c_int, c_int, P_double, c_void_p, c_int64, c_int64, c_void_p, c_int64,
                            c_int64, P_double, c_void_p, c_int64, c_int64, P_double, c_int],
     "euvl_zgetrf_batched": [c_void_p, c_int, c_void_p, c_int64, c_int64, c_void_p, c_int],

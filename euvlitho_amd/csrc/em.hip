@@ -667,7 +667,7 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
     return EUVL_ERR_NOMEM;
   }
   XdOp xd;
-  const bool use_xd = grid_fits && xd_fft_fits(N);
+  const bool use_xd = grid_fits && xd_fft_fits(N) && N >= fft_min_n(false);
   if (use_xd) {
     const double c = 2.0 * PI / (polar == 'X' ? p->dx : p->dy);
     const std::complex<double> a_p = k * k * (1.0 - ea), b_u = c * lea, a_x = k * k * ea;
@@ -711,7 +711,7 @@ int amp_s_core(euvl_handle h, char polar, const euvl_em_params* p, const SolveDe
   EUVL_LAUNCH_CHECK();
   // V = P + H - H G: by FFTs over the order grid when the grid fits a
   // workgroup's LDS (G is sigma^ Toeplitz in the order differences), else a GEMM
-  if (grid_fits) {
+  if (grid_fits && N >= fft_min_n(true)) {
     EUVL_TRY(coupling_fft(h, fg, N, B, t.lidx, t.midx, lmax, mmax, spec, per_clip, spec_stride,
                           make_double2(eai.real(), eai.imag()), v.kt, k, H, V, ldv, sv, Ssp));
   } else {

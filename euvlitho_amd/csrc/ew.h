@@ -139,6 +139,8 @@ struct XdOp {
   const double2* S = nullptr;
 };
 bool xd_fft_fits(int N);
+// smallest N for the FFT form of the products by D (coupling = false) or of H G (true)
+int fft_min_n(bool coupling);
 int xd_fft_spectra(euvl_handle h, const FftGrid& g, int lmax, int mmax, const double2* W, int per_clip,
                    int64_t w_stride, int E, double2* S);
 int xd_fft(euvl_handle h, const XdOp& op, int E, const double2* X, double2* Y);

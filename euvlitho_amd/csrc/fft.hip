@@ -21,6 +21,8 @@
 // outputs.  Along m only the 2 lmax + 1 rows that hold orders are transformed
 // (the others are zero on the way in and not read on the way out).
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
 
 #include "ew.h"
 
@@ -387,6 +389,28 @@ int coupling_fft(euvl_handle h, const FftGrid& g, int N, int E, const int* lidx,
 
 bool xd_fft_fits(int N) { return N <= XD_PER * FFT_THREADS; }
 
+// The FFT kernels hold a grid of FFT_MAX_PTS points in LDS and run one
+// workgroup per CU, so below some N the GEMMs they replace are faster.  EM
+// solve on MI355X (FP64, ms, products by D and H G both by FFT / both by
+// GEMM): N = 289 x 16 entries 3.9 / 2.1, 625 x 9: 9.4 / 6.0, 1089 x 4:
+// 14.2 / 12.9, 1681 x 4: 31.9 / 38.2; the products by D alone pay from
+// N ~ 1200, H G alone from N ~ 1000 (1089 x 4: 12.8 against 12.9; 625 x 9:
+// 6.3 against 6.0).  EUVL_FFT_MIN_N (products by D) and
+// EUVL_COUPLING_FFT_MIN_N (H G) override the defaults (read once).
+std::atomic<int> g_fft_min_n[2] = {-1, -1};  // -1: not read from the environment yet
+
+int fft_min_n(bool coupling) {
+  const int k = coupling ? 1 : 0;
+  int v = g_fft_min_n[k].load(std::memory_order_relaxed);
+  if (v < 0) {
+    const char* e = getenv(coupling ? "EUVL_COUPLING_FFT_MIN_N" : "EUVL_FFT_MIN_N");
+    v = e ? std::max(0, atoi(e)) : (coupling ? 800 : 1200);
+    int expect = -1;
+    if (!g_fft_min_n[k].compare_exchange_strong(expect, v)) v = expect;
+  }
+  return v;
+}
+
 int xd_fft_spectra(euvl_handle h, const FftGrid& g, int lmax, int mmax, const double2* W, int per_clip,
                    int64_t w_stride, int E, double2* S) {
   // the spectrum of m^ alone: sigma_spectrum_kernel with ea^-1 = 0
@@ -410,3 +434,9 @@ int xd_fft(euvl_handle h, const XdOp& op, int E, const double2* X, double2* Y) {
 }
 
 }  // namespace euvl
+
+extern "C" int euvl_set_fft_min_n(int products, int coupling) {
+  if (products >= 0) euvl::g_fft_min_n[0].store(products);
+  if (coupling >= 0) euvl::g_fft_min_n[1].store(coupling);
+  return EUVL_OK;
+}

@@ -107,6 +107,15 @@ int euvl_set_precision(euvl_handle h, int mask);
 int euvl_autotune_gemm(euvl_handle h, int m, int n, int k, int batch, int fp32, int* out_variant,
                        double* out_ms);
 
+/* Smallest order count N at which the EM solve runs its products by the
+ * absorber operator D (D^2..D^6 and D F; default 1200) and the coupling
+ * product H G (default 800) as 2D FFT correlations over the order grid in
+ * place of GEMMs, when the grid fits a workgroup's LDS; a negative value
+ * leaves that threshold as it is.  Process-wide; the defaults can also be
+ * set by EUVL_FFT_MIN_N and EUVL_COUPLING_FFT_MIN_N.  Both paths give the
+ * same results to rounding.  No reference counterpart (tuning). */
+int euvl_set_fft_min_n(int products, int coupling);
+
 /* ---------------------------------------------------------------------------
  * Dense complex linear algebra (batched, row-major, interleaved complex).
  * ------------------------------------------------------------------------- */

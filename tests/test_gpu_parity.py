@@ -252,6 +252,41 @@ def test_amp_s_c1_vs_oracle(pol):
         assert e < 1e-9
 
 
+@pytest.mark.parametrize("pol", ["X", "Y"])
+def test_amp_s_fft_products_vs_gemm_and_oracle(lib, pol):
+    """The FFT forms of the products by D (D^2..D^6, D F over the order grid)
+    and of the coupling product H G, forced at the 512 nm clip (N = 441, below
+    their default thresholds), against the GEMM forms and the oracle, in both
+    polarisations (the Y form correlates with the m index and 2 pi / dy)."""
+    from euvlitho_amd.engine import Engine
+    from euvlitho_amd.masks import manhattan
+    from oracle import em as OE
+    cfg = c1_problem()
+    m = manhattan(512, 512, 0.5, 4)
+    offs = cfg.offsets()[[0, 3], :2]
+    out = {}
+    try:
+        for name, thr in (("fft", 0), ("gemm", 1 << 30)):
+            assert lib.euvl_set_fft_min_n(thr, thr) == 0
+            eng = Engine(cfg)
+            out[name] = eng.amp_s(eng.upload_mask(m), pol, offs).cpu().numpy()
+            say(f"  {name}: info {eng.last_info.tolist()}")
+            eng.close()
+    finally:
+        assert lib.euvl_set_fft_min_n(1200, 800) == 0  # the defaults
+    p, d = oracle_problem(cfg)
+    li, mi = np.asarray(cfg.order_set()[0], np.int64), np.asarray(cfg.order_set()[1], np.int64)
+    L, M = cfg.order_set()[2:]
+    e_pair = np.abs(out["fft"] - out["gemm"]).max() / np.abs(out["gemm"]).max()
+    say(f"  {pol}: fft vs gemm {e_pair:.3e}")
+    assert e_pair < 1e-11
+    for o in range(2):
+        ref = OE.amp_s(pol, m, p, d, li, mi, L, M, offs[o, 0], offs[o, 1])
+        e = np.abs(out["fft"][o] - ref).max() / np.abs(ref).max()
+        say(f"  offset {o}: fft vs oracle {e:.3e}")
+        assert e < 1e-9
+
+
 @pytest.mark.parametrize("ndiv,K", [(512, 10), (256, 8)])
 def test_m3d_vs_oracle(ndiv, K):
     """512 nm (C1) and 256 nm, where the blank-mask order set needs the in-NA
