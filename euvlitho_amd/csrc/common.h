@@ -99,6 +99,15 @@ struct euvl_handle_s {
   int prec_scope = 0;
   // events around the last mask raster launch (euvl_mask_raster_ms)
   hipEvent_t raster_ev[2] = {nullptr, nullptr};
+  // pinned staging buffers of the host tables that entry points upload
+  // (upload_async): a copy may read its buffer after the call has returned,
+  // so a buffer is reused only once the event after its last copy is done
+  static constexpr int NSTAGE = 8;
+  char* stage_buf[NSTAGE] = {};
+  size_t stage_cap[NSTAGE] = {};
+  hipEvent_t stage_ev[NSTAGE] = {};
+  bool stage_used[NSTAGE] = {};
+  int stage_next = 0;
 };
 
 namespace euvl {
@@ -119,4 +128,9 @@ int ws_reserve(euvl_handle h, size_t bytes);
 void* ws_take(euvl_handle h, size_t bytes);
 // Ensure the aux buffer holds at least `bytes` and return it.
 void* aux_reserve(euvl_handle h, size_t bytes);
+// Copies `bytes` of host memory at src to dst (device) on the handle's stream
+// through a pinned staging buffer, without waiting for the copy: src may be
+// freed when this returns.  Not for code under graph capture or that must not
+// allocate (a staging buffer may be allocated or grown).
+int upload_async(euvl_handle h, void* dst, const void* src, size_t bytes);
 }  // namespace euvl

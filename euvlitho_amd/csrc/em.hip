@@ -370,11 +370,8 @@ int mask_spectrum(euvl_handle h, const uint8_t* bits, int nx, int ny, int n_clip
                   double2* tw /* nx+1 + ny+1 */) {
   std::vector<double2> t;
   dft_twiddles(nx, ny, t);
-  EUVL_HIP_CHECK(hipMemcpyAsync(tw, t.data(), sizeof(double2) * t.size(), hipMemcpyHostToDevice, h->stream));
-  EUVL_TRY(mask_spectrum_enqueue(h, bits, nx, ny, n_clips, lw, mw, out, scratch, part, tw));
-  // the host table must outlive the copy
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  return EUVL_OK;
+  EUVL_TRY(upload_async(h, tw, t.data(), sizeof(double2) * t.size()));
+  return mask_spectrum_enqueue(h, bits, nx, ny, n_clips, lw, mw, out, scratch, part, tw);
 }
 }  // namespace euvl
 
@@ -560,8 +557,8 @@ size_t solve_tables_bytes(const euvl_em_params* p, int N, const SolveHost& t) {
   return top;
 }
 
-// `up` receives the packed host copy; the caller keeps it alive until the
-// stream has been synchronised (the copy may still read it).
+// `up` receives the packed host copy (uploaded through a pinned staging
+// buffer, so it may be freed when this returns).
 int upload_solve_tables(euvl_handle h, const euvl_em_params* p, const int* lindex, const int* mindex, int N,
                         const SolveHost& t, Upload& up, char* blk, SolveDev& d) {
   const size_t o_l = up.add(lindex, sizeof(int) * N);
@@ -573,7 +570,7 @@ int upload_solve_tables(euvl_handle h, const euvl_em_params* p, const int* linde
   const size_t o_idx = up.add(t.inc_idx.data(), sizeof(int) * std::max<size_t>(t.inc_idx.size(), 1));
   const size_t o_slot = up.add(t.inc_slot.data(), sizeof(int) * std::max<size_t>(t.inc_slot.size(), 1));
   const size_t o_kz = up.add(t.kz.data(), sizeof(double) * t.kz.size());
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(upload_async(h, blk, up.host.data(), up.host.size()));
   d.lidx = (const int*)(blk + o_l);
   d.midx = (const int*)(blk + o_m);
   d.sx0 = (const double*)(blk + o_sx);
@@ -830,7 +827,7 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
   EUVL_TRY(ws_reserve(h, tb + solve_workspace_bytes(N, B, (int)th.inc_idx.size())));
   char* blk = (char*)ws_take(h, tb);
   SolveDev td;
-  Upload up;  // read by the copy until the synchronisation at the end
+  Upload up;
   EUVL_TRY(upload_solve_tables(h, p, h_lindex, h_mindex, N, th, up, blk, td));
   SolveCtl ctl;
   EUVL_TRY(amp_s_core(h, polar, p, td, N, lmax, mmax, (const double2*)d_spec, B, 0, B, (double2*)d_Ax, ctl));
@@ -842,6 +839,5 @@ extern "C" int euvl_amp_s(euvl_handle h, char polar, const euvl_em_params* p, co
       h_info[4 * b + 3] = td.ninc;
     }
   }
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
 }

@@ -1,5 +1,7 @@
 // Handle lifecycle, error reporting and the device workspace of libeuvl.
+#include <algorithm>
 #include <cstdarg>
+#include <cstring>
 
 #include "common.h"
 
@@ -51,6 +53,32 @@ void* aux_reserve(euvl_handle h, size_t bytes) {
   return h->aux;
 }
 
+int upload_async(euvl_handle h, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return EUVL_OK;
+  const int k = h->stage_next;
+  h->stage_next = (k + 1) % euvl_handle_s::NSTAGE;
+  if (h->stage_used[k]) EUVL_HIP_CHECK(hipEventSynchronize(h->stage_ev[k]));  // its last copy has read it
+  if (!h->stage_ev[k]) EUVL_HIP_CHECK(hipEventCreateWithFlags(&h->stage_ev[k], hipEventDisableTiming));
+  if (bytes > h->stage_cap[k]) {
+    if (h->stage_buf[k]) EUVL_HIP_CHECK(hipHostFree(h->stage_buf[k]));
+    h->stage_buf[k] = nullptr;
+    h->stage_cap[k] = 0;
+    const size_t cap = std::max<size_t>((bytes + 4095) & ~size_t(4095), 64 * 1024);
+    if (hipHostMalloc((void**)&h->stage_buf[k], cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      h->stage_buf[k] = nullptr;
+      set_error("upload: pinned staging buffer of %zu bytes failed", cap);
+      return EUVL_ERR_NOMEM;
+    }
+    h->stage_cap[k] = cap;
+  }
+  memcpy(h->stage_buf[k], src, bytes);
+  EUVL_HIP_CHECK(hipMemcpyAsync(dst, h->stage_buf[k], bytes, hipMemcpyHostToDevice, h->stream));
+  EUVL_HIP_CHECK(hipEventRecord(h->stage_ev[k], h->stream));
+  h->stage_used[k] = true;
+  return EUVL_OK;
+}
+
 void* ws_take(euvl_handle h, size_t bytes) {
   bytes = (bytes + 255) & ~size_t(255);
   if (h->ws_top + bytes > h->ws_cap) return nullptr;
@@ -84,6 +112,10 @@ extern "C" int euvl_destroy(euvl_handle h) {
   for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->raster_ev)
     if (e) (void)hipEventDestroy(e);
+  for (int k = 0; k < euvl_handle_s::NSTAGE; ++k) {
+    if (h->stage_ev[k]) (void)hipEventDestroy(h->stage_ev[k]);
+    if (h->stage_buf[k]) (void)hipHostFree(h->stage_buf[k]);
+  }
   if (h->ws) (void)hipFree(h->ws);
   if (h->aux) (void)hipFree(h->aux);
   if (h->d_red) (void)hipFree(h->d_red);
@@ -94,6 +126,9 @@ extern "C" int euvl_destroy(euvl_handle h) {
 
 extern "C" int euvl_set_stream(euvl_handle h, void* s) {
   if (!h) return EUVL_ERR_ARG;
+  // work enqueued on the old stream may still read the workspace that the
+  // next call reuses on the new one
+  if ((hipStream_t)s != h->stream) EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   h->stream = (hipStream_t)s;
   return EUVL_OK;
 }

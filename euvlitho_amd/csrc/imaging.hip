@@ -611,7 +611,7 @@ extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* 
   const size_t o_twy = up.add(twy.data(), sizeof(double2) * ny);
   EUVL_TRY(ws_reserve(h, up.host.size() + abbe_workspace_bytes(n_src, lpx, lpy, nx, ny) + 4096));
   char* blk = (char*)ws_take(h, up.host.size());
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(upload_async(h, blk, up.host.data(), up.host.size()));
   ImagingDev t;
   t.map = (const int*)(blk + o_map);
   t.off = (const double*)(blk + o_off);
@@ -619,9 +619,7 @@ extern "C" int euvl_abbe_image(euvl_handle h, char polar, const euvl_em_params* 
   t.pup2 = (const int*)(blk + o_pup);
   t.twx = (const double2*)(blk + o_twx);
   t.twy = (const double2*)(blk + o_twy);
-  EUVL_TRY(abbe_enqueue(h, a, (const double2*)d_Ax, 0, 1, t, n_src, ncut, nx, ny, weight, d_img, 0, false));
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  return EUVL_OK;
+  return abbe_enqueue(h, a, (const double2*)d_Ax, 0, 1, t, n_src, ncut, nx, ny, weight, d_img, 0, false);
 }
 
 extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x, int lpmax_y,
@@ -647,16 +645,14 @@ extern "C" int euvl_m3d_fit(euvl_handle h, const euvl_em_params* p, const double
   const size_t o_pup = up.add(h_pupil, sizeof(int) * 3 * ncut);
   EUVL_TRY(ws_reserve(h, up.host.size() + 4096));
   char* blk = (char*)ws_take(h, up.host.size());
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(upload_async(h, blk, up.host.data(), up.host.size()));
   ImagingDev t;
   t.map = (const int*)(blk + o_map);
   t.vc = (const double2*)(blk + o_vc);
   t.ab = (const double2*)(blk + o_ab);
   t.pup3 = (const int*)(blk + o_pup);
-  EUVL_TRY(m3d_enqueue(h, a, (const double2*)d_fmask, 0, (const double2*)d_Ax, 0, 1, t, ncut, (double2*)d_a0,
-                       (double2*)d_ax, (double2*)d_ay));
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
-  return EUVL_OK;
+  return m3d_enqueue(h, a, (const double2*)d_fmask, 0, (const double2*)d_Ax, 0, 1, t, ncut, (double2*)d_a0,
+                     (double2*)d_ax, (double2*)d_ay);
 }
 
 extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const double* d_fmask, int lpmax_x,
@@ -722,7 +718,7 @@ extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const 
   double2* Ft = (double2*)ws_take(h, fbytes);
   double2* G = (double2*)ws_take(h, gbytes);
   double* Pxp = (double*)ws_take(h, pbytes);
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(upload_async(h, blk, up.host.data(), up.host.size()));
   hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, d_intlinear);
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(zero_d_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, (int64_t)nx * ny, d_intft);
@@ -744,6 +740,5 @@ extern "C" int euvl_linear_images(euvl_handle h, const euvl_em_params* p, const 
                      which == 0 ? d_intlinear : d_intft, Pxp));
     }
   }
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
 }

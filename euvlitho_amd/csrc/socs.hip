@@ -991,7 +991,7 @@ extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const dou
   double2* Rp = (double2*)ws_take(h, b_S);
   double2* Sp = (double2*)ws_take(h, b_S);
   double2* T = (double2*)ws_take(h, b_T);
-  EUVL_HIP_CHECK(hipMemcpyAsync(blk, up.host.data(), up.host.size(), hipMemcpyHostToDevice, h->stream));
+  EUVL_TRY(upload_async(h, blk, up.host.data(), up.host.size()));
   SocsSetup a;
   a.ampvc = make_double2(ampvc[0], ampvc[1]);
   a.ampab = make_double2(ampab[0], ampab[1]);
@@ -1056,6 +1056,5 @@ extern "C" int euvl_socs_image(euvl_handle h, const euvl_em_params* p, const dou
   hipLaunchKernelGGL(socs_up_x_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, nout, nx, ny, T,
                      (const double2*)(blk + o_twx), d_img);
   EUVL_LAUNCH_CHECK();
-  EUVL_HIP_CHECK(hipStreamSynchronize(h->stream));
   return EUVL_OK;
 }
