@@ -42,6 +42,10 @@ typedef struct euvl_handle_s* euvl_handle;
  * (cnn/include/header.h:796-804, 1501-1508, 1590-1593). */
 int euvl_create(euvl_handle* out);
 int euvl_destroy(euvl_handle h);
+/* Entry points return with their work queued on this stream (host tables go
+ * through pinned staging buffers of the handle).  Changing the stream
+ * synchronises the previous one first, since the next call reuses the
+ * handle's workspace: the previous stream must still exist then. */
 int euvl_set_stream(euvl_handle h, void* hip_stream);
 const char* euvl_last_error(void);
 /* Device bytes currently held by the handle's workspace. */
