@@ -35,15 +35,18 @@ constexpr int FFT_THREADS = 1024;
 // The grid and the root tables live in LDS; the out-of-line passes receive
 // them as generic pointers and cast them back, so that their accesses are
 // LDS instructions with 32-bit addresses rather than flat ones.
-// (Element access through double components: the vector class has no
-// constructors for another address space.)
-typedef __attribute__((address_space(3))) double lds_f64;
+// Elements are accessed as a native 16-byte vector (one ds_read_b128 /
+// ds_write_b128 each: two 8-byte accesses per element conflicted on the LDS
+// banks 9 times as often); the vector class has no constructors for
+// another address space.
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) dvec2 lds_f64;
 
-__device__ __forceinline__ double2 lds_ld(const lds_f64* p, int i) { return make_double2(p[2 * i], p[2 * i + 1]); }
-__device__ __forceinline__ void lds_st(lds_f64* p, int i, double2 v) {
-  p[2 * i] = v.x;
-  p[2 * i + 1] = v.y;
+__device__ __forceinline__ double2 lds_ld(const lds_f64* p, int i) {
+  const dvec2 t = p[i];
+  return make_double2(t.x, t.y);
 }
+__device__ __forceinline__ void lds_st(lds_f64* p, int i, double2 v) { p[i] = (dvec2){v.x, v.y}; }
 
 __device__ __forceinline__ double2 root_of(const lds_f64* root, int t, bool inv) {
   const double2 w = lds_ld(root, t);
