@@ -746,9 +746,12 @@ def main():
                      "flops_counted": "reference algorithm: 164 N^3 per (clip, polarisation, offset) solve "
                                       f"x {n_solves} solves = {algo_flops:.3e} per clip, over the clip's GEMM time",
                      "frac_note": "the reference algorithm's work per GEMM time: above 1 because this route needs "
-                                  "fewer flops than it (16 N x N x N products, the coupling product by 2D FFTs "
-                                  "and a block Gauss-Jordan solve per solve, no eigendecomposition); the MFMA "
-                                  "units' own utilisation is mfma_issue_frac",
+                                  "fewer flops than it (per solve 8 N x N x N GEMM products, the products by the "
+                                  "absorber operator and the coupling product as 2D FFT correlations, a block "
+                                  "Gauss-Jordan solve; no eigendecomposition) and runs part of its work in the FFT "
+                                  "kernels; the MFMA units' own utilisation is mfma_issue_frac, the reference "
+                                  "algorithm's work over the whole clip time clip_reference_tflops",
+                     "clip_reference_tflops": algo_flops / (ms * 1e-3) / 1e12,
                      "route_flops_per_clip_8mnk": gemm_flops,
                      "algorithmic_flops_per_launch": algo_flops / max(launches, 1),
                      "traffic": traffic.get("traffic_bytes_per_launch"),
