@@ -3,8 +3,9 @@
 // All three are even in sqrt(D), i.e. power series in D: no square root, no
 // branch choice and no eigendecomposition.  They are evaluated by a degree-12
 // Taylor series in x = h^2 D (h = d / 2^s, Paterson-Stockmeyer in blocks of
-// x^4) followed by s doublings C <- 2 C^2 - I, F <- 2 F C; s is set by the
-// norms of D^3 and D^4 rather than of D.
+// x^4, or of x^6 where the products by D run as FFT correlations) followed by
+// s doublings C <- 2 C^2 - I, F <- 2 F C; s is set by the norms of D^3 and
+// D^4 rather than of D.
 #include <cmath>
 
 #include "ew.h"
@@ -17,8 +18,9 @@ inline double2 re(double x) { return make_double2(x, 0.0); }
 namespace euvl {
 
 // D, C, F, G2: batch x n x n.  Takes 6 workspace matrices.  With xd, the
-// products with D as a factor (D^2, D^3, D^4, G2) are FFT correlations over
-// the order grid (xd_fft) and the other twelve stay MFMA GEMMs.
+// products with D as a factor (D^2 .. D^6 and G2) are FFT correlations over
+// the order grid (xd_fft), and the series step and the doublings stay MFMA
+// GEMMs.
 double cos_sinc_theta(euvl_handle h) {
   // The degree-12 remainder is bounded by (h^2 alpha)^13 / 26!: h^2 alpha <= 5.9
   // keeps it below 2.6e-17 (FP64 products); with FP32 products (the precision
