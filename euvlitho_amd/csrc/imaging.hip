@@ -79,12 +79,14 @@ __global__ void abbe_ypass_kernel(int n_sc, int nl, int nm, int lpy, int ny, con
   }
 }
 
-// img[x][y] += w * sum_{s,c} | sum_l exp(2 pi i (l x mod nx)/nx) G[s][c][l][y] |^2
-// img[x][y] += w sum_sc |sum_l tw_x(l x) G[sc][l][y]|^2, in two steps: a
-// workgroup (one wave) takes 64 y values and 16 x values of one split of the
-// source/component range, so each G value read serves 16 x (the sum over l
-// runs in the same order as a per-pixel loop); partials P[s][x][y] are then
-// added in split order (deterministic).
+// img[x][y] += w * sum_{s,c} | sum_l exp(2 pi i (l x mod nx)/nx) G[s][c][l][y] |^2,
+// in two steps: the source/component range is cut into splits whose partial
+// images P[s][x][y] are added in split order (abbe_xreduce_kernel, so the sum
+// is deterministic).  The callers size P for xpass_splits splits; the MFMA
+// passes below use at most that many.  abbe_xpass_kernel, on vector FMAs,
+// serves the FP64 pass when the pupil is too wide for the MFMA pass's
+// twiddle tile in LDS: a workgroup (one wave) takes 64 y values and 16 x
+// values of one split, so each G value read serves 16 x.
 constexpr int XP_X = 16, XP_Y = 64;
 __global__ __launch_bounds__(XP_Y) void abbe_xpass_kernel(int n_sc, int nl, int lpx, int nx, int ny,
                                                           const double2* __restrict__ G,
@@ -222,8 +224,57 @@ __global__ __launch_bounds__(256) void abbe_xpass_mfma_kernel(int n_sc, int nl, 
   }
 }
 
-// splits of the source/component range: about 4096 waves in all (four per
-// SIMD, so that the G loads of one wave overlap another's arithmetic)
+// The x pass in FP64 on v_mfma_f64_16x16x4_f64: abbe_xpass_mfma_kernel with
+// FP64 operands and accumulation (no compensation).  Lane l feeds
+// A = tw(x0 + l%16, k0 + l/16), B = G(k0 + l/16, y0 + l%16) as there, and its
+// FP64 accumulator holds D(x0 + 4 r + l/16, y0 + l%16).
+__global__ __launch_bounds__(256) void abbe_xpass_mfma64_kernel(int n_sc, int nl, int lpx, int nx, int ny,
+                                                                const double2* __restrict__ G,
+                                                                const double2* __restrict__ twx, int scs,
+                                                                double* __restrict__ P) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int xw = blockIdx.y * XM_T + (w >> 1) * 16, yw = blockIdx.x * XM_T + (w & 1) * 16;
+  const int s = blockIdx.z;
+  const int nlp = (nl + 3) & ~3;
+  extern __shared__ double2 sh_twd[];  // [XM_T][nlp] twiddles of this block's x values (zero past nl)
+  for (int e = threadIdx.x; e < XM_T * nlp; e += blockDim.x) {
+    const int xb = e / nlp, kk = e % nlp;
+    const int64_t x = blockIdx.y * XM_T + xb;
+    double2 t = make_double2(0.0, 0.0);
+    if (kk < nl) t = twx[(int)((((int64_t)(kk - lpx) * x) % nx + nx) % nx)];
+    sh_twd[e] = t;
+  }
+  __syncthreads();
+  const int y = yw + li;
+  const bool yin = y < ny;
+  double4_t acc = {0.0, 0.0, 0.0, 0.0};
+  const int sc1 = min(n_sc, (s + 1) * scs);
+  const double2* tw = sh_twd + (int64_t)((w >> 1) * 16 + li) * nlp;
+  for (int sc = s * scs; sc < sc1; ++sc) {
+    const double2* g = G + (int64_t)sc * nl * ny;
+    double4_t dre = {0.0, 0.0, 0.0, 0.0}, dim = {0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < nlp; k0 += 4) {
+      const int k = k0 + lk;
+      const double2 a = tw[k];
+      const double2 b = (k < nl && yin) ? g[(int64_t)k * ny + y] : make_double2(0.0, 0.0);
+      dre = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, dre, 0, 0, 0);
+      dre = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.y, dre, 0, 0, 0);
+      dim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, dim, 0, 0, 0);
+      dim = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.x, dim, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) acc[r] += dre[r] * dre[r] + dim[r] * dim[r];
+  }
+  if (!yin) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int x = xw + 4 * r + lk;
+    if (x < nx) P[((int64_t)s * nx + x) * ny + y] = acc[r];
+  }
+}
+
+// splits of the source/component range for P's size: about 4096 tiles of
+// XP_X x XP_Y pixels x splits
 int xpass_splits(int n_sc, int nx, int ny) {
   const int64_t blocks = (int64_t)((ny + XP_Y - 1) / XP_Y) * ((nx + XP_X - 1) / XP_X);
   const int64_t want = (4096 + blocks - 1) / blocks;
@@ -232,10 +283,25 @@ int xpass_splits(int n_sc, int nx, int ny) {
 
 int xpass(euvl_handle h, int n_sc, int nl, int lpx, int nx, int ny, const double2* G, const double2* twx, double w,
           double* img, double* P) {
-  const int S0 = xpass_splits(n_sc, nx, ny);
-  const int scs = (n_sc + S0 - 1) / S0, S = (n_sc + scs - 1) / scs;
-  hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + XP_Y - 1) / XP_Y, (nx + XP_X - 1) / XP_X, S), dim3(XP_Y),
-                     sizeof(double2) * XP_X * nl, h->stream, n_sc, nl, lpx, nx, ny, G, twx, scs, P);
+  // about 1024 workgroups of 4 waves, and never more splits than P holds
+  const int64_t blocks = (int64_t)((ny + XM_T - 1) / XM_T) * ((nx + XM_T - 1) / XM_T);
+  const int S0 = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(1024 + blocks - 1) / blocks, (int64_t)std::min(n_sc, 64),
+                            (int64_t)xpass_splits(n_sc, nx, ny)}));
+  const int nlp = (nl + 3) & ~3;
+  int S;
+  if (sizeof(double2) * XM_T * nlp <= 48 * 1024) {
+    const int scs = (n_sc + S0 - 1) / S0;
+    S = (n_sc + scs - 1) / scs;
+    hipLaunchKernelGGL(abbe_xpass_mfma64_kernel, dim3((ny + XM_T - 1) / XM_T, (nx + XM_T - 1) / XM_T, S),
+                       dim3(256), sizeof(double2) * XM_T * nlp, h->stream, n_sc, nl, lpx, nx, ny, G, twx, scs, P);
+  } else {  // a wide pupil: the vector pass (16 x values per workgroup: half the LDS per l)
+    const int S1 = xpass_splits(n_sc, nx, ny);
+    const int scs = (n_sc + S1 - 1) / S1;
+    S = (n_sc + scs - 1) / scs;
+    hipLaunchKernelGGL(abbe_xpass_kernel, dim3((ny + XP_Y - 1) / XP_Y, (nx + XP_X - 1) / XP_X, S), dim3(XP_Y),
+                       sizeof(double2) * XP_X * nl, h->stream, n_sc, nl, lpx, nx, ny, G, twx, scs, P);
+  }
   EUVL_LAUNCH_CHECK();
   hipLaunchKernelGGL(abbe_xreduce_kernel, grid1((int64_t)nx * ny), dim3(256), 0, h->stream, S, (int64_t)nx * ny, P,
                      w, img);
