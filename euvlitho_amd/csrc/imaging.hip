@@ -70,10 +70,13 @@ __global__ void abbe_ypass_kernel(int n_sc, int nl, int nm, int lpy, int ny, con
     const int64_t sl = e / ny;  // (s*3+c)*nl + l
     const double2* f = F + sl * nm;
     double2 acc = make_double2(0.0, 0.0);
+    // (m y) mod ny for m = mm - lpy, stepped by y (0 <= y < ny) instead of a
+    // 64-bit remainder per term: the same twiddles in the same order
+    int idx = (int)((((int64_t)(-lpy) * y) % ny + ny) % ny);
     for (int mm = 0; mm < nm; ++mm) {
-      const int m = mm - lpy;
-      const int idx = (int)((((int64_t)m * y) % ny + ny) % ny);
       acc = cadd(acc, cmul(f[mm], twy[idx]));
+      idx += y;
+      if (idx >= ny) idx -= ny;
     }
     G[e] = acc;
   }
@@ -153,9 +156,11 @@ __global__ void abbe_ypass32_kernel(int n_sc, int nl, int nm, int lpy, int ny, c
     const int64_t sl = e / ny;
     const double2* f = F + sl * nm;
     float re = 0.0f, im = 0.0f;
+    int idx = (int)((((int64_t)(-lpy) * y) % ny + ny) % ny);  // (m y) mod ny, stepped by y
     for (int mm = 0; mm < nm; ++mm) {
-      const int m = mm - lpy;
-      const float2 t = twy[(int)((((int64_t)m * y) % ny + ny) % ny)];
+      const float2 t = twy[idx];
+      idx += y;
+      if (idx >= ny) idx -= ny;
       const float fr = (float)f[mm].x, fi = (float)f[mm].y;
       re = fmaf(fr, t.x, fmaf(-fi, t.y, re));
       im = fmaf(fr, t.y, fmaf(fi, t.x, im));
